@@ -1,0 +1,157 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own tests.
+
+Run once in a container that has the reference checkout:
+
+    python tests/golden/make_fixtures.py /root/reference
+
+Sources (paths relative to the reference root):
+  * src/materials/unit_test/utPeridigm_ElasticMaterial.cpp  testTwoPts (:78-186),
+    testEightPts (:189-452), testThreePts (:456-659): inputs and the asserted
+    weighted volume, dilatation and force values with their tolerances.
+  * src/io/pdneigh/unit_test/ut_neighborhood_list.cxx:68-152 (1x1x11 bar,
+    horizon 3.1 cells, self included, expected sets and sizes).
+  * src/io/discretization/unit_test/utPeridigm_PdQuickGridDiscretization.cpp:70-201
+    (2x2x2 lattice, horizon 0.501, spherical, 3 neighbours, centres, volume).
+  * src/io/pdneigh/unit_test/utFinitePlane.cxx:52-104 (plane/bond intersection).
+  * test/verification/BondBreakingInitialVelocity/BondBreakingInitialVelocity.xml
+    and _gold.e (netCDF-3 read with scipy): the deck parameters and the nine
+    output frames of displacement, velocity, dilatation, weighted volume and
+    damage, with the .comp tolerances.
+Only data (inputs and expected outputs) is written; nothing of the reference's
+source text.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+
+def elastic_unit_tests():
+    out = {}
+    out["params"] = dict(density=7800.0, K=130.0e9, mu=78.0e9, horizon=10.0)
+    # testTwoPts
+    out["two"] = dict(
+        x=[[0, 0, 0], [1, 0, 0]], y=[[0, 0, 0], [2, 0, 0]], vol=[1.0, 1.0],
+        nlist=[1, 1, 1, 0],
+        m=[1.0, 1.0], m_tol=1e-15, theta=[3.0, 3.0], theta_tol=1e-15,
+        fx=[2.34e12, -2.34e12], f_tol=1e-2)
+    # testEightPts
+    x8 = [[0, 0, 0], [0, 0, 1], [0, 1, 0], [0, 1, 1], [1, 0, 0], [1, 0, 1], [1, 1, 0], [1, 1, 1]]
+    y8 = [[a, b, 0.98 if c == 1 else 0.0] for a, b, c in x8]
+    nl8 = []
+    for i in range(8):
+        nl8.append(7)
+        nl8.extend(j for j in range(8) if j != i)
+    # hand solution (pairwise t = (|Y|-|X|)*9.75e10, factor 2 for equal and
+    # opposite pair forces), force on cell 0; the others differ by sign.
+    f = np.zeros(3)
+    def add(t, uv):
+        nonlocal f
+        f += 2.0 * np.asarray(uv) * t
+    t = (0.98 - 1.0) * 9.75e10; add(t, [0, 0, 0.98 / 0.98])
+    vm = np.sqrt(1.0 + 0.98 * 0.98); t = (vm - np.sqrt(2.0)) * 9.75e10; add(t, [0, 1 / vm, 0.98 / vm])
+    add(t, [1 / vm, 0, 0.98 / vm])
+    vm = np.sqrt(2.0 + 0.98 * 0.98); t = (vm - np.sqrt(3.0)) * 9.75e10; add(t, [1 / vm, 1 / vm, 0.98 / vm])
+    signs = [[1, 1, 1], [1, 1, -1], [1, -1, 1], [1, -1, -1], [-1, 1, 1], [-1, 1, -1], [-1, -1, 1], [-1, -1, -1]]
+    out["eight"] = dict(x=x8, y=y8, vol=[1.0] * 8, nlist=nl8, m=12.0, m_tol=1e-12,
+                        theta=-0.01991593994643333, theta_tol=1e-12,
+                        f0=f.tolist(), signs=signs, f_tol=1e-11)
+    # testThreePts
+    vol3 = [0.9, 1.1, 0.8]
+    nl3 = [2, 1, 2, 2, 0, 2, 2, 0, 1]
+    r = {}
+    r[0] = np.array([-2753550531.144094, -1510011581.595148, -621769474.7744727]) * vol3[1] \
+        - np.array([3398655528.6806289, 1863778838.308732, 767438345.18594845]) * vol3[1] \
+        + np.array([7736514797.0571524, 15402697641.41379, -11956431959.088315]) * vol3[2] \
+        - np.array([-8687228655.850972, -17295482505.73966, 13425717013.587864]) * vol3[2]
+    r[1] = np.array([3398655528.6806289, 1863778838.308732, 767438345.18594845]) * vol3[0] \
+        - np.array([-2753550531.144094, -1510011581.595148, -621769474.7744727]) * vol3[0] \
+        + np.array([5110873051.7924152, -1252163897.689138, 6133047662.15088]) * vol3[2] \
+        - np.array([-4649613866.523322, 1139155397.2982139, -5579536639.827986]) * vol3[2]
+    r[2] = np.array([-8687228655.850972, -17295482505.73966, 13425717013.587864]) * vol3[0] \
+        - np.array([7736514797.0571524, 15402697641.41379, -11956431959.088315]) * vol3[0] \
+        + np.array([-4649613866.523322, 1139155397.2982139, -5579536639.827986]) * vol3[1] \
+        - np.array([5110873051.7924152, -1252163897.689138, 6133047662.15088]) * vol3[1]
+    out["three"] = dict(
+        x=[[1.1, 2.6, -0.1], [-2.0, 0.9, -0.3], [0.0, 0.01, 1.8]],
+        y=[[1.2, 2.4, -0.1], [-1.9, 0.7, -0.8], [0.1, 0.21, 1.6]], vol=vol3, nlist=nl3,
+        m=[23.016479999999931, 18.64767999999997, 20.497599999999963], m_tol=1e-12,
+        theta=[-0.114127034572639, 0.08257537372985, -0.12166177890553], theta_tol=1e-11,
+        f=[r[0].tolist(), r[1].tolist(), r[2].tolist()],
+        f_tol=[[1e-11, 1e-11, 1e-11], [1e-11, 1e-10, 1e-11], [1e-11, 1e-11, 1e-11]])
+    return out
+
+
+def neighborhood_list_bar():
+    sets = [[0, 1, 2, 3], [0, 1, 2, 3, 4], [0, 1, 2, 3, 4, 5], [0, 1, 2, 3, 4, 5, 6],
+            [1, 2, 3, 4, 5, 6, 7], [2, 3, 4, 5, 6, 7, 8], [3, 4, 5, 6, 7, 8, 9],
+            [4, 5, 6, 7, 8, 9, 10], [5, 6, 7, 8, 9, 10], [6, 7, 8, 9, 10], [7, 8, 9, 10]]
+    return dict(nx=1, ny=1, nz=11, origin=[0.0, 0.0, 0.0], length=[1.0, 1.0, 1.0],
+                scale=3.1, include_self=True, sizes=[4, 5, 6, 7, 7, 7, 7, 7, 6, 5, 4],
+                total=65, sets=sets)
+
+
+def pdquickgrid_2x2x2():
+    return dict(n=[2, 2, 2], origin=[0.0, 0.0, 0.0], length=[1.0, 1.0, 1.0], horizon=0.501,
+                num_bonds=24, neighbors_each=3,
+                x=[[0.25, 0.25, 0.25], [0.75, 0.25, 0.25], [0.25, 0.75, 0.25], [0.75, 0.75, 0.25],
+                   [0.25, 0.25, 0.75], [0.75, 0.25, 0.75], [0.25, 0.75, 0.75], [0.75, 0.75, 0.75]],
+                x_tol=1e-16, volume=0.125)
+
+
+def finite_plane():
+    # normal, lower-left corner, bottom unit vector, bottom length, side length
+    plane = [1, 0, 0, 0, 1, 0, 0, -1, 0, 1.0, 1.0]
+    return dict(plane=plane,
+                cases=[dict(p0=[-.5, .5, .5], p1=[.5, .5, .5], cuts=True, t=0.5, x=[0.0, 0.5, 0.5]),
+                       dict(p0=[-.5, .5, .5], p1=[-.5, 1.5, .5], cuts=False)])
+
+
+def bond_breaking(ref):
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/BondBreakingInitialVelocity")
+    f = netcdf_file(os.path.join(d, "BondBreakingInitialVelocity_gold.e"), "r", mmap=False)
+
+    def names(k):
+        return [b"".join(r).strip(b"\x00").decode() for r in f.variables[k][:]]
+
+    nod = names("name_nod_var")
+    ele = names("name_elem_var")
+    out = dict(
+        deck=dict(n=[2, 2, 1], origin=[0.0, 0.0, 0.0], length=[2.1, 2.0, 1.0], horizon=0.51,
+                  spherical=False, density=7800.0, K=130.0e9, mu=60.0e9, critical_stretch=0.02,
+                  dt=1.0e-8, final_time=0.00008, output_every=1000,
+                  initial_velocity_y=[-244.939, -244.959, 244.939, 244.959]),
+        tolerances=dict(displacement=1e-9, velocity=1e-9, weighted_volume=1e-12,
+                        dilatation=1e-12, damage=1e-3),
+        time=np.asarray(f.variables["time_whole"][:]).tolist(),
+        coordx=np.asarray(f.variables["coordx"][:]).tolist(),
+        coordy=np.asarray(f.variables["coordy"][:]).tolist(),
+        coordz=np.asarray(f.variables["coordz"][:]).tolist())
+    for i, nme in enumerate(nod):
+        out[nme] = np.asarray(f.variables["vals_nod_var%d" % (i + 1)][:]).tolist()
+    for i, nme in enumerate(ele):
+        out[nme] = np.asarray(f.variables["vals_elem_var%deb1" % (i + 1)][:]).tolist()
+    f.close()
+    return out
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    here = os.path.dirname(os.path.abspath(__file__))
+    fx = {
+        "elastic_unit_tests.json": elastic_unit_tests(),
+        "neighborhood_list_bar.json": neighborhood_list_bar(),
+        "pdquickgrid_2x2x2.json": pdquickgrid_2x2x2(),
+        "finite_plane.json": finite_plane(),
+        "bond_breaking_initial_velocity.json": bond_breaking(ref),
+    }
+    for name, obj in fx.items():
+        with open(os.path.join(here, name), "w") as fh:
+            json.dump(obj, fh, indent=1)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,328 @@
+"""Pins the FP64 oracle against the reference's own golden values and against
+what the mathematics fixes (closed forms, invariants, brute force, finite
+differences). CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def load(golden_dir, name):
+    with open(os.path.join(golden_dir, name)) as fh:
+        return json.load(fh)
+
+
+def rel_ok(a, b, tol):
+    # TEST_FLOATING_EQUALITY semantics: |a-b|/max(|a|,|b|) <= tol
+    a = float(a); b = float(b)
+    den = max(abs(a), abs(b))
+    return den == 0.0 or abs(a - b) / den <= tol
+
+
+# ----------------------------------------------------- reference unit tests
+def _run_material(case, p):
+    x = np.array(case["x"], float); y = np.array(case["y"], float)
+    vol = np.array(case["vol"], float); nl = np.array(case["nlist"], np.int32)
+    m = O.weighted_volume(x, vol, nl, p["horizon"])
+    bd = np.zeros(O.num_bonds(nl, len(vol)))
+    th, f = O.compute_force(x, y, m, vol, bd, nl, p["K"], p["mu"], p["horizon"])
+    return m, th, f
+
+
+def test_two_points(golden_dir):
+    g = load(golden_dir, "elastic_unit_tests.json")
+    c = g["two"]
+    m, th, f = _run_material(c, g["params"])
+    for i in range(2):
+        assert rel_ok(m[i], c["m"][i], c["m_tol"])
+        assert rel_ok(th[i], c["theta"][i], c["theta_tol"])
+        assert rel_ok(f[i, 0], c["fx"][i], c["f_tol"])
+        assert f[i, 1] <= 1e-14 and f[i, 2] <= 1e-14
+
+
+def test_eight_points(golden_dir):
+    g = load(golden_dir, "elastic_unit_tests.json")
+    c = g["eight"]
+    m, th, f = _run_material(c, g["params"])
+    f0 = np.array(c["f0"])
+    for i in range(8):
+        assert rel_ok(m[i], c["m"], c["m_tol"])
+        assert rel_ok(th[i], c["theta"], c["theta_tol"])
+        for d in range(3):
+            assert rel_ok(c["signs"][i][d] * f[i, d], f0[d], c["f_tol"])
+
+
+def test_three_points(golden_dir):
+    g = load(golden_dir, "elastic_unit_tests.json")
+    c = g["three"]
+    m, th, f = _run_material(c, g["params"])
+    for i in range(3):
+        assert rel_ok(m[i], c["m"][i], c["m_tol"])
+        assert rel_ok(th[i], c["theta"][i], c["theta_tol"])
+        for d in range(3):
+            assert rel_ok(f[i, d], c["f"][i][d], c["f_tol"][i][d])
+
+
+# ------------------------------------------------------- neighbour lists
+def test_neighborhood_list_axial_bar(golden_dir):
+    g = load(golden_dir, "neighborhood_list_bar.json")
+    x, vol, _ = O.quickgrid(g["nx"], g["ny"], g["nz"], g["origin"], g["length"], 0.01)
+    horizon = g["scale"] * g["length"][2] / g["nz"]
+    for brute in (False, True):
+        nl = O.radius_search(x, horizon, include_self=g["include_self"], brute=brute)
+        rp, col = O.list_to_csr(nl, len(vol))
+        assert col.size == g["total"]
+        for i in range(len(vol)):
+            s = sorted(col[rp[i]:rp[i + 1]].tolist())
+            assert len(s) == g["sizes"][i]
+            assert s == g["sets"][i]
+
+
+def test_pdquickgrid_2x2x2(golden_dir):
+    g = load(golden_dir, "pdquickgrid_2x2x2.json")
+    x, vol, nl = O.quickgrid(*g["n"], g["origin"], g["length"], g["horizon"], spherical=True)
+    rp, col = O.list_to_csr(nl, 8)
+    assert col.size == g["num_bonds"]
+    assert np.all(np.diff(rp) == g["neighbors_each"])
+    assert np.all(np.abs(x - np.array(g["x"])) <= g["x_tol"] * np.abs(np.array(g["x"])))
+    assert np.all(vol == g["volume"])
+
+
+def test_quickgrid_3015_stencil():
+    # horizon 3.015h: 122 lattice offsets with a^2+b^2+c^2 <= 9 (max neighbour
+    # count 122, test/verification/InitialDamage/InitialDamage.comp)
+    h = 1e-3
+    x, vol, nl = O.quickgrid(9, 9, 9, [0, 0, 0], [9 * h] * 3, 3.015 * h)
+    rp, col = O.list_to_csr(nl, 729)
+    deg = np.diff(rp)
+    assert deg.max() == 122
+    centre = 4 + 4 * 9 + 4 * 81
+    assert deg[centre] == 122
+    off = np.array([(a, b, c) for a in range(-3, 4) for b in range(-3, 4) for c in range(-3, 4)
+                    if 0 < a * a + b * b + c * c <= 9])
+    assert len(off) == 122
+    # closed-form weighted volume at an interior node: m = h^5 * sum |offset|^2 = 708 h^5
+    m = O.weighted_volume(x, vol, nl, 3.015 * h)
+    assert int((off ** 2).sum()) == 708
+    assert abs(m[centre] - 708 * h ** 5) <= 1e-12 * 708 * h ** 5
+
+
+def test_radius_search_cell_grid_matches_brute_force():
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, 1, size=(600, 3))
+    for R in (0.05, 0.11, 0.2):
+        a = O.radius_search(x, R)
+        b = O.radius_search(x, R, brute=True)
+        assert np.array_equal(a, b)
+
+
+def test_finite_plane(golden_dir):
+    g = load(golden_dir, "finite_plane.json")
+    for c in g["cases"]:
+        assert O.plane_cuts(g["plane"], c["p0"], c["p1"]) == c["cuts"]
+        assert O.plane_cuts(g["plane"], c["p1"], c["p0"]) == c["cuts"]
+
+
+def test_plane_filter_notch_lattice():
+    # a plane between two lattice layers cuts exactly the bonds that cross it
+    h = 1.0
+    x, vol, nl = O.quickgrid(8, 8, 2, [0, 0, 0], [8, 8, 2], 3.015)
+    # plane x = 4 (between columns 3 and 4), covering y in [0, 3.2), full z.
+    # ua = cross(ub, n) = cross((0,1,0),(1,0,0)) = (0,0,-1): side A runs to -z,
+    # so the lower-left corner sits at z = 3 and side length 4 spans z in (-1, 3]
+    plane = [1, 0, 0, 4.0, 0.0, 3.0, 0, 1, 0, 3.2, 4.0]
+    out = O.filter_planes(x, nl, plane)
+    rp0, c0 = O.list_to_csr(nl, len(vol))
+    rp1, c1 = O.list_to_csr(out, len(vol))
+    removed = 0
+    for i in range(len(vol)):
+        a = set(c0[rp0[i]:rp0[i + 1]].tolist()); b = set(c1[rp1[i]:rp1[i + 1]].tolist())
+        assert b <= a
+        for j in a - b:
+            xi, xj = x[i], x[j]
+            assert (xi[0] - 4) * (xj[0] - 4) < 0
+            t = (4 - xi[0]) / (xj[0] - xi[0])
+            yc = xi[1] + t * (xj[1] - xi[1])
+            assert 0 <= yc < 3.2
+            removed += 1
+        for j in b:
+            xi, xj = x[i], x[j]
+            if (xi[0] - 4) * (xj[0] - 4) < 0:
+                t = (4 - xi[0]) / (xj[0] - xi[0])
+                assert xi[1] + t * (xj[1] - xi[1]) > 3.2
+    assert removed > 0
+    # symmetric list
+    pairs = {(i, j) for i in range(len(vol)) for j in c1[rp1[i]:rp1[i + 1]]}
+    assert all((j, i) in pairs for i, j in pairs)
+
+
+# ----------------------------------------------------------- explicit run
+def test_bond_breaking_initial_velocity_gold(golden_dir):
+    g = load(golden_dir, "bond_breaking_initial_velocity.json")
+    dk = g["deck"]
+    x, vol, nl = O.quickgrid(*dk["n"], dk["origin"], dk["length"], dk["horizon"],
+                             spherical=dk["spherical"])
+    assert np.allclose(x[:, 0], g["coordx"], atol=1e-12)
+    assert np.allclose(x[:, 1], g["coordy"], atol=1e-12)
+    v = np.zeros((4, 3)); v[:, 1] = dk["initial_velocity_y"]
+    u = None; bd = None
+    tol = g["tolerances"]
+    for k in range(1, len(g["time"])):
+        r = O.explicit_run(x, vol, dk["density"], nl, dk["K"], dk["mu"], dk["horizon"],
+                           dk["critical_stretch"], dk["dt"], dk["output_every"], u0=u, v0=v,
+                           bond_damage=bd, t0=g["time"][k - 1])
+        u, v, bd = r["u"], r["v"], r["bond_damage"]
+        for c, nm in enumerate("XYZ"):
+            assert np.abs(u[:, c] - g["Displacement" + nm][k]).max() <= tol["displacement"]
+            assert np.abs(v[:, c] - g["Velocity" + nm][k]).max() <= tol["velocity"]
+        assert np.abs(r["theta"] - g["Dilatation"][k]).max() <= tol["dilatation"]
+        assert np.abs(r["m"] - g["Weighted_Volume"][k]).max() <= tol["weighted_volume"]
+        assert np.abs(r["damage"] - g["Damage"][k]).max() <= tol["damage"]
+
+
+def test_critical_time_step_closed_form():
+    x = np.array([[0, 0, 0], [0.7, 0, 0]], float)
+    vol = np.array([0.3, 0.5])
+    nl = np.array([1, 1, 1, 0], np.int32)
+    K, rho, delta = 1e9, 2000.0, 1.0
+    c = 18 * K / (np.pi * delta ** 4)
+    expect = min(np.sqrt(2 * rho / (0.5 * c / 0.7)), np.sqrt(2 * rho / (0.3 * c / 0.7)))
+    assert abs(O.critical_time_step(x, vol, nl, rho, K, delta) - expect) <= 1e-14 * expect
+
+
+# ------------------------------------------------------------- invariants
+def _lattice(n=(7, 6, 5), h=1e-3, delta=3.015):
+    x, vol, nl = O.quickgrid(*n, [0, 0, 0], [k * h for k in n], delta * h)
+    return x, vol, nl, delta * h
+
+
+def test_uniform_stretch_dilatation():
+    x, vol, nl, dl = _lattice()
+    m = O.weighted_volume(x, vol, nl, dl)
+    eps = 1e-4
+    y = x * (1 + eps)
+    bd = np.zeros(O.num_bonds(nl, len(vol)))
+    th, f = O.compute_force(x, y, m, vol, bd, nl, 130e9, 50e9, dl)
+    # every node, boundary included: theta = 3 eps (to rounding)
+    assert np.abs(th - 3 * eps).max() <= 1e-9 * 3 * eps
+
+
+def _rot(a, b, c):
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    Rz = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1]])
+    Ry = np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+    Rx = np.array([[1, 0, 0], [0, cc, -sc], [0, sc, cc]])
+    return Rz @ Ry @ Rx
+
+
+def test_rigid_motion_zero_force_and_momentum():
+    x, vol, nl, dl = _lattice()
+    m = O.weighted_volume(x, vol, nl, dl)
+    bd = np.zeros(O.num_bonds(nl, len(vol)))
+    y = x @ _rot(0.3, -0.2, 0.1).T + np.array([1.0, 2.0, -0.5])
+    th, f = O.compute_force(x, y, m, vol, bd, nl, 130e9, 50e9, dl)
+    rng = np.random.default_rng(3)
+    y2 = x + 1e-6 * rng.standard_normal(x.shape)
+    th2, f2 = O.compute_force(x, y2, m, vol, bd, nl, 130e9, 50e9, dl)
+    scale = np.abs(f2).max()
+    assert np.abs(th).max() <= 1e-9
+    assert np.abs(f).max() <= 1e-6 * scale
+    # linear momentum: sum_i f_i V_i = 0
+    assert np.abs((f2 * vol[:, None]).sum(0)).max() <= 1e-9 * scale * vol.sum()
+    # angular momentum: sum_i y_i x f_i V_i = 0
+    L = np.cross(y2, f2 * vol[:, None]).sum(0)
+    assert np.abs(L).max() <= 1e-9 * scale * vol.sum() * np.abs(y2).max()
+
+
+def _lps_energy(x, y, m, vol, nl, K, mu, bd):
+    rp, col = O.list_to_csr(nl, len(vol))
+    E = 0.0
+    for i in range(len(vol)):
+        js = col[rp[i]:rp[i + 1]]
+        d = 1.0 - bd[rp[i]:rp[i + 1]]
+        X = x[js] - x[i]; Y = y[js] - y[i]
+        z = np.linalg.norm(X, axis=1); e = np.linalg.norm(Y, axis=1) - z
+        th = 3.0 / m[i] * np.sum(d * z * e * vol[js])
+        ed = e - th * z / 3.0
+        alpha = 15 * mu / m[i]
+        E += vol[i] * (0.5 * K * th * th + 0.5 * alpha * np.sum(d * d * ed * ed * vol[js]))
+    return E
+
+
+def test_force_is_energy_gradient():
+    # f_i = -(1/V_i) dE/dy_i for the LPS strain energy with K != 5 mu/3, so the
+    # dilatation coupling term c1 is exercised (it vanishes when 9K == 15 mu).
+    rng = np.random.default_rng(11)
+    x, vol, nl = O.quickgrid(4, 3, 3, [0, 0, 0], [4, 3, 3], 2.2)
+    x = x + 0.05 * rng.standard_normal(x.shape)
+    vol = vol * rng.uniform(0.8, 1.2, size=vol.shape)
+    K, mu = 130e9, 30e9
+    m = O.weighted_volume(x, vol, nl, 2.2)
+    bd = np.zeros(O.num_bonds(nl, len(vol)))
+    y = x + 1e-3 * rng.standard_normal(x.shape)
+    _, f = O.compute_force(x, y, m, vol, bd, nl, K, mu, 2.2)
+    hstep = 1e-7
+    for i in (0, 5, 17, 35):
+        for c in range(3):
+            yp = y.copy(); yp[i, c] += hstep
+            ym = y.copy(); ym[i, c] -= hstep
+            g = (_lps_energy(x, yp, m, vol, nl, K, mu, bd) - _lps_energy(x, ym, m, vol, nl, K, mu, bd)) / (2 * hstep)
+            assert abs(-g / vol[i] - f[i, c]) <= 1e-5 * np.abs(f).max()
+
+
+def test_damage_threshold_strict_and_monotone():
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    nl = np.array([2, 1, 2, 1, 0, 1, 0], np.int32)
+    sc = 0.02
+    y = x.copy(); y[1, 0] = 1.0 + sc  # stretch exactly sc (in FP64) -> not broken
+    s = O.bond_stretch(x, y, nl)
+    bd, dmg, _ = O.compute_damage(x, y, np.zeros(4), nl, sc)
+    assert (s[0] > sc) == (bd[0] == 1.0)
+    y[1, 0] = 1.0 + 1.5 * sc
+    bd, dmg, _ = O.compute_damage(x, y, np.zeros(4), nl, sc)
+    assert bd.tolist() == [1.0, 0.0, 1.0, 0.0] and dmg.tolist() == [0.5, 1.0, 0.0]
+    # irreversible: compressing again keeps the bond broken
+    bd2, _, _ = O.compute_damage(x, x, bd, nl, sc)
+    assert bd2.tolist() == bd.tolist()
+
+
+def test_directional_derivative_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    x, vol, nl = O.quickgrid(4, 4, 3, [0, 0, 0], [4, 4, 3], 2.2)
+    K, mu = 130e9, 40e9
+    m = O.weighted_volume(x, vol, nl, 2.2)
+    bd = np.zeros(O.num_bonds(nl, len(vol)))
+    u = 1e-3 * rng.standard_normal(x.shape)
+    w = rng.standard_normal(x.shape)
+    f, jw = O.force_directional(x, u, w, m, vol, bd, nl, K, mu, 2.2)
+    f0 = O.force_at(x, u, m, vol, bd, nl, K, mu, 2.2)
+    assert np.array_equal(f, f0)
+    eps = 1e-7
+    fp = O.force_at(x, u + eps * w, m, vol, bd, nl, K, mu, 2.2)
+    fm = O.force_at(x, u - eps * w, m, vol, bd, nl, K, mu, 2.2)
+    fd = (fp - fm) / (2 * eps)
+    assert np.abs(fd - jw).max() <= 1e-6 * np.abs(jw).max()
+
+
+def test_gnn_backward_matches_finite_differences():
+    rng = np.random.default_rng(2)
+    x, vol, nl = O.quickgrid(4, 3, 2, [0, 0, 0], [4, 3, 2], 1.5)
+    rp, col = O.list_to_csr(nl, len(vol))
+    ew = rng.uniform(0.1, 1.0, size=col.size)
+    dims = [5, 8, 8, 3]
+    npar = sum(2 * a * b + b for a, b in zip(dims[:-1], dims[1:]))
+    p = 0.3 * rng.standard_normal(npar)
+    feat = rng.standard_normal((len(vol), dims[0]))
+    gout = rng.standard_normal((len(vol), dims[-1]))
+    out, acts = O.gnn_forward(rp, col, ew, dims, p, feat)
+    dp = O.gnn_backward(rp, col, ew, dims, p, acts, gout)
+    eps = 1e-6
+    for k in rng.choice(npar, size=25, replace=False):
+        pp = p.copy(); pp[k] += eps
+        pm = p.copy(); pm[k] -= eps
+        lp = np.sum(O.gnn_forward(rp, col, ew, dims, pp, feat)[0] * gout)
+        lm = np.sum(O.gnn_forward(rp, col, ew, dims, pm, feat)[0] * gout)
+        assert abs((lp - lm) / (2 * eps) - dp[k]) <= 1e-6 * max(1.0, abs(dp[k]))
