@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Explicit-step throughput of the LPS + critical-stretch force path.
+
+Workload (default): the notched plate, 400x400x4 lattice (640k nodes),
+h = 0.5 mm, horizon 3.015h, critical stretch 5e-4, edge notch 40h (bonds
+crossing it removed), top/bottom 3-node layers pulled at +-2 m/s; synthetic
+inputs built on the device. A step is one velocity-Verlet step: the
+damage+dilatation pass and the force+Verlet pass over every directed bond.
+
+Prints ONE JSON line: value = directed-bond evaluations per second
+(B bonds per step; each bond is evaluated from both ends, as the CSR holds
+both directions) with all inputs resident in HBM. `roofline` reports the
+dominant kernel (the force+Verlet pass) from HIP events recorded on the
+stream it runs on; `cpu_baseline` times the FP64 CPU oracle (a scalar
+restatement of the reference loops, one core) on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plate|bar|cylinder]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def bytes_pass1(B, N):
+    """Algorithmic HBM bytes of the damage+dilatation pass: the int32 CSR
+    index (4B), the bond-break bits read and written (B/8 each) and per node
+    the FP32 reference position and displacement (12 + 12) and the
+    dilatation written (4)."""
+    return 4 * B + B / 8 + B / 8 + 28 * N
+
+
+def bytes_pass2(B, N):
+    """Algorithmic HBM bytes of the force+Verlet pass: the CSR index (4B),
+    the bond-break bits (B/8), and per node 192 bytes: reference position and
+    displacement (12 + 12), dilatation, weighted volume and volume (4 + 4 + 4),
+    FP64 displacement, velocity and acceleration read and written (3 x 48)
+    and the FP32 displacement written (12)."""
+    return 4 * B + B / 8 + 192 * N
+
+
+def build(workload):
+    from peridigm_amd import problems
+    if workload == "plate":
+        p = problems.plate()
+    elif workload == "bar":
+        p = problems.bar()
+    elif workload == "cylinder":
+        p = problems.cylinder()
+    else:
+        raise SystemExit(f"unknown workload {workload}")
+    nl = problems.neighbours(p)
+    return p, nl
+
+
+def cpu_baseline(steps=20):
+    """FP64 oracle on a 200x200x4 sample of the plate workload (13.7M bonds),
+    one core: per-step time from a run of `steps` steps minus a run of zero
+    steps (both include the initial evaluation and setup)."""
+    import oracle as O
+    from peridigm_amd import problems
+    p = problems.plate(nx=200, ny=200, nz=4, notch_cells=20)
+    nl = O.filter_planes(p["x"], O.radius_search(p["x"], p["horizon"]), p["planes"])
+    B = nl.size - len(p["x"])
+    args = (p["x"], p["volume"], p["density"], nl, p["K"], p["mu"], p["horizon"],
+            p["critical_stretch"], p["dt"])
+    t0 = time.perf_counter()
+    O.explicit_run(*args, 0, bcs=p["bcs"])
+    t1 = time.perf_counter()
+    O.explicit_run(*args, steps, bcs=p["bcs"])
+    t2 = time.perf_counter()
+    per_step = ((t2 - t1) - (t1 - t0)) / steps
+    return dict(value=B / per_step, unit="bond_evals/s", cores=1, kind="port",
+                sample=f"plate 200x200x4 (N={len(p['x'])}, B={B}), {steps} FP64 Verlet steps "
+                       f"(damage, dilatation, scatter force), {per_step*1e3:.1f} ms/step")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--workload", default="plate")
+    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from peridigm_amd import Context
+
+    p, nl = build(args.workload)
+    N = len(p["volume"])
+    ctx = Context(0, lanes=args.lanes)
+    ctx.set_topology(nl, N)
+    B = ctx.B
+    ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
+    ctx.set_density(np.full(N, p["density"]))
+    ctx.set_bcs(p["bcs"])
+    ctx.upload_state(v=p["v0"])
+    dt = p["dt"]
+
+    # warmup: initial force evaluation + W steps
+    ctx.verlet_run(args.warmup, dt, t0=0.0, initial_eval=True)
+    t_start = args.warmup * dt
+
+    # timed region: exactly K steps, device-resident state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.verlet_run(args.steps, dt, t0=t_start, initial_eval=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    t_start += args.steps * dt
+
+    # per-kernel durations from HIP events on the library's stream
+    ctx.set_per_pass_timing(True)
+    ksteps = max(50, args.steps // 4)
+    ctx.verlet_run(ksteps, dt, t0=t_start, initial_eval=False)
+    _, p1_ms, p2_ms = ctx.last_timing()
+    p1_avg = p1_ms / ksteps * 1e-3
+    p2_avg = p2_ms / ksteps * 1e-3
+    st = ctx.download_state()
+    ctx.close()
+
+    ms_per_step = elapsed / args.steps * 1e3
+    value = B * args.steps / elapsed
+    a2 = bytes_pass2(B, N) / p2_avg / 1e9
+    a1 = bytes_pass1(B, N) / p1_avg / 1e9
+    out = {
+        "metric": "bonds/s per explicit step",
+        "value": value,
+        "unit": "bond_evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": {
+                "plate": "notched plate 400x400x4 LPS + critical stretch 5e-4, h=0.5mm, "
+                         "horizon 3.015h, +-2 m/s pull, fixed dt 2.5e-8 s",
+                "bar": "LPS elastic bar 100x10x10, h=1mm, horizon 3.015h, dt 1e-8 s",
+                "cylinder": "expanding hollow cylinder 40h-50h x 200h, h=1mm, perturbed, sc 1.5e-3",
+            }[args.workload],
+            "nodes": N, "bonds": B, "lanes_per_node": args.lanes,
+            "bond_arithmetic": "fp32 compensated", "integration": "fp64",
+            "broken_bonds_end": int(st["bond_damage"].sum()),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": a2, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": a2 / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_pass2 (force + Verlet)", "kernel_avg_us": p2_avg * 1e6,
+            "pass1_achieved": a1, "pass1_avg_us": p1_avg * 1e6,
+            "step_frac": (bytes_pass1(B, N) + bytes_pass2(B, N)) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

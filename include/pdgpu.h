@@ -1,0 +1,150 @@
+/*
+ * pdgpu — MI355X (gfx950) peridynamic force path behind a plain C ABI.
+ *
+ * The linear peridynamic solid ("Elastic") material with critical-stretch bond
+ * damage, explicit velocity-Verlet integration, the matrix-free tangent of the
+ * quasi-static solve and its preconditioners, all computed by hand-written HIP
+ * kernels. Every array crossing this boundary is a host pointer in Peridigm's
+ * own layout:
+ *   - node vectors: FP64, xyz-interleaved, length 3*N, owned points 0..N-1;
+ *   - node scalars: FP64, length N;
+ *   - the neighbourhood list: int32 [n_0, j.., n_1, j.., ...] of local ids;
+ *   - bond fields (bond damage): FP64, length B, in neighbourhood-list order.
+ * Device mirrors are owned by the context. Every call returns 0 on success or
+ * a negative PDG_E* code; pdg_last_error() gives the message. Calls on one
+ * context must be serialised; use one context per host thread.
+ *
+ * Reference interfaces replaced (paths relative to the Peridigm source root):
+ *   Material::initialize / computeForce   src/materials/Peridigm_Material.hpp:120-141
+ *   ElasticMaterial::initialize/computeForce src/materials/Peridigm_ElasticMaterial.cpp:124-172
+ *   DamageModel::computeDamage            src/damage/Peridigm_DamageModel.hpp:106-111
+ *   CriticalStretchDamageModel            src/damage/Peridigm_CriticalStretchDamageModel.cpp:83-184
+ *   NeighborhoodData layout               src/core/Peridigm_NeighborhoodData.hpp:56-148
+ *   Peridigm::executeExplicit (Verlet)    src/core/Peridigm.cpp:1290-1593
+ *   ComputeCriticalTimeStep               src/core/Peridigm_CriticalTimeStep.cpp:54-126
+ *   QuickGrid / radius search / filters   src/io/mesh_input/quick_grid/QuickGrid.cxx:510-647,
+ *                                         src/io/Peridigm_ZoltanSearchTree.cpp:214-247,
+ *                                         src/io/pdneigh/BondFilter.cxx:59-153
+ *   QS residual / 3x3 block preconditioner src/core/Peridigm.cpp:4176-4317, :1603-1647
+ */
+#ifndef PDGPU_H
+#define PDGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDG_OK 0
+#define PDG_E_ARG -1       /* invalid argument or call order */
+#define PDG_E_HIP -2       /* HIP runtime error */
+#define PDG_E_UNSUPPORTED -3 /* input outside the supported model */
+#define PDG_E_NONFINITE -4 /* NaN/Inf in a force evaluation */
+#define PDG_E_NOCONV -5    /* iterative solve did not converge */
+
+/* influence functions (src/core/Peridigm_InfluenceFunction.hpp:69-88) */
+#define PDG_INFLUENCE_ONE 0
+#define PDG_INFLUENCE_PARABOLIC_DECAY 1
+#define PDG_INFLUENCE_GAUSSIAN 2
+
+typedef struct pdg_ctx pdg_ctx;
+
+/* Context on HIP device `device` (one stream, default priority). */
+int pdg_ctx_create(int device, pdg_ctx** out);
+int pdg_ctx_destroy(pdg_ctx* ctx);
+const char* pdg_last_error(const pdg_ctx* ctx);
+
+/* Lanes per node used by the bond kernels (16, 32 or 64; default 16). */
+int pdg_set_lanes_per_node(pdg_ctx* ctx, int lanes);
+
+/* ---- topology and constants -------------------------------------------
+ * pdg_set_topology: the block's NeighborhoodData (numOwnedPoints, ownedIDs,
+ * neighborhoodList of listSize ints). All neighbour ids must be owned points
+ * (single device, no ghosts) and the list must be symmetric (j in N(i) iff
+ * i in N(j)): forces are gathered from both bond ends without atomics.
+ * pdg_set_constants: Model_Coordinates x (3N), Volume (N), material and
+ * damage constants (s_crit <= 0: no damage model). Builds the Morton node
+ * order, the CSR bond arrays, the bond-damage bit masks and the device state.
+ */
+int pdg_set_topology(pdg_ctx* ctx, int numOwnedPoints, const int* ownedIDs,
+                     const int* neighborhoodList, long long listSize);
+int pdg_set_constants(pdg_ctx* ctx, const double* x, const double* volume,
+                      double bulkModulus, double shearModulus, double horizon,
+                      int influenceKind, double criticalStretch);
+/* Density per node (N) for the explicit integrator. */
+int pdg_set_density(pdg_ctx* ctx, const double* density);
+
+/* Number of bonds B and nodes N after pdg_set_topology. */
+int pdg_get_sizes(const pdg_ctx* ctx, int* numNodes, long long* numBonds);
+
+/* ---- drop-in material / damage calls ----------------------------------- */
+/* ElasticMaterial::initialize: Weighted_Volume (N), FP64, reference order of
+ * summation (bit-identical to the CPU loop for the "One" influence function). */
+int pdg_initialize(pdg_ctx* ctx, double* weightedVolume);
+
+/* CriticalStretchDamageModel::computeDamage: Coordinates NP1 y (3N),
+ * Bond_Damage N (B) -> Bond_Damage NP1 (B) and Damage (N). nearCritical
+ * (optional) receives the number of intact bonds whose FP32 stretch lies
+ * within 1e-6*s_crit of s_crit. */
+int pdg_compute_damage(pdg_ctx* ctx, const double* y, const double* bondDamageN,
+                       double* bondDamageNP1, double* damage, long long* nearCritical);
+
+/* ElasticMaterial::computeForce: y (3N), Bond_Damage NP1 (B, values 0 or 1,
+ * symmetric) -> Dilatation (N) and Force_Density (3N). */
+int pdg_compute_force(pdg_ctx* ctx, double dt, const double* y, const double* bondDamageNP1,
+                      double* dilatation, double* forceDensity);
+
+/* ---- explicit dynamics ------------------------------------------------- */
+/* Prescribed-displacement boundary conditions u_c(t) = c1*t + c0 on a node
+ * set, applied to the velocity after the half kick as DirichletIncrementBC
+ * does (src/core/Peridigm_BoundaryCondition.cpp:202-241). */
+typedef struct {
+  const int* nodes; /* node ids */
+  int numNodes;
+  int coord;        /* 0,1,2 */
+  double c0, c1;
+} pdg_bc;
+
+/* State upload/download (reference layout). u, v, a: 3N; bondDamage: B. */
+int pdg_state_upload(pdg_ctx* ctx, const double* u, const double* v, const double* a,
+                     const double* bondDamage);
+int pdg_state_download(pdg_ctx* ctx, double* u, double* v, double* a, double* bondDamage,
+                       double* damage, double* dilatation, double* forceDensity);
+int pdg_set_bcs(pdg_ctx* ctx, const pdg_bc* bcs, int numBcs);
+
+/* Peridigm::executeExplicit: if `initialEval`, evaluates damage and force in
+ * the current configuration first (:1388-1424); then runs nsteps Verlet steps
+ * from time t0 with step dt on the device-resident state. */
+int pdg_verlet_run(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEval);
+
+/* Stable time step (ComputeCriticalTimeStep, constant horizon) for density rho. */
+int pdg_critical_time_step(pdg_ctx* ctx, double density, double* dtOut);
+
+/* Per-kernel timing of the last pdg_verlet_run (ms, HIP events on the
+ * context's stream): total, damage+dilatation pass, force+Verlet pass. */
+int pdg_last_timing(const pdg_ctx* ctx, double* totalMs, double* pass1Ms, double* pass2Ms);
+
+/* 1: pdg_verlet_run records per-pass HIP events and synchronises after each
+ * step so pdg_last_timing reports per-pass times; 0 (default): no per-step
+ * synchronisation, only the total time is recorded. */
+int pdg_set_per_pass_timing(pdg_ctx* ctx, int on);
+
+/* Count of intact bonds flagged near-critical during the last run. */
+int pdg_near_critical_count(const pdg_ctx* ctx, long long* count);
+
+/* ---- neighbour search (integer results, bit-exact predicates) ----------- */
+/* QuickGrid tensor-product lattice: node coordinates (3N), volumes (N) and
+ * the flattened neighbourhood list (ascending global id, self excluded).
+ * Call with nlist == NULL to get *listSize, then again with capacity. */
+int pdg_quickgrid(int nx, int ny, int nz, const double origin[3], const double length[3],
+                  double horizon, int spherical, double* x, double* volume, int* nlist,
+                  long long* listSize);
+/* Fixed-radius search |x_j - x_i|^2 <= R^2 (rows sorted by id, self excluded),
+ * then removal of bonds cut by any of the finite planes (11 doubles each:
+ * normal, lower-left corner, bottom unit vector, bottom length, side length). */
+int pdg_radius_search(int N, const double* x, double radius, int numPlanes,
+                      const double* planes, int* nlist, long long* listSize);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDGPU_H */

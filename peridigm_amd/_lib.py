@@ -1,0 +1,87 @@
+"""ctypes binding of libpdgpu.so (the C ABI declared in include/pdgpu.h).
+
+The library must have been built (``python -c "import __graft_entry__ as g;
+g.build()"`` or ``make -C peridigm_amd``). There is no CPU fallback: if the
+library is missing or no GPU is present, loading fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdgpu.so")
+
+PDG_OK = 0
+PDG_E_ARG = -1
+PDG_E_HIP = -2
+PDG_E_UNSUPPORTED = -3
+PDG_E_NONFINITE = -4
+PDG_E_NOCONV = -5
+
+INFLUENCE = {"One": 0, "Parabolic Decay": 1, "Gaussian": 2}
+
+
+class PdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdgpu error {code}: {msg}")
+        self.code = code
+
+
+class pdg_bc(C.Structure):
+    _fields_ = [("nodes", C.POINTER(C.c_int)), ("numNodes", C.c_int), ("coord", C.c_int),
+                ("c0", C.c_double), ("c1", C.c_double)]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_ll = C.c_longlong
+_d = C.c_double
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "pdg_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "pdg_ctx_destroy": (_i, [_vp]),
+    "pdg_last_error": (C.c_char_p, [_vp]),
+    "pdg_set_lanes_per_node": (_i, [_vp, _i]),
+    "pdg_set_topology": (_i, [_vp, _i, _vp, _vp, _ll]),
+    "pdg_set_constants": (_i, [_vp, _vp, _vp, _d, _d, _d, _i, _d]),
+    "pdg_set_density": (_i, [_vp, _vp]),
+    "pdg_get_sizes": (_i, [_vp, C.POINTER(_i), C.POINTER(_ll)]),
+    "pdg_initialize": (_i, [_vp, _vp]),
+    "pdg_compute_damage": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(_ll)]),
+    "pdg_compute_force": (_i, [_vp, _d, _vp, _vp, _vp, _vp]),
+    "pdg_state_upload": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "pdg_state_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pdg_set_bcs": (_i, [_vp, C.POINTER(pdg_bc), _i]),
+    "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),
+    "pdg_critical_time_step": (_i, [_vp, _d, C.POINTER(_d)]),
+    "pdg_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
+    "pdg_set_per_pass_timing": (_i, [_vp, _i]),
+    "pdg_near_critical_count": (_i, [_vp, C.POINTER(_ll)]),
+    "pdg_quickgrid": (_i, [_i, _i, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, C.POINTER(_ll)]),
+    "pdg_radius_search": (_i, [_i, _vp, _d, _i, _vp, _vp, C.POINTER(_ll)]),
+}
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libpdgpu.so and bind every declared symbol (raises if absent)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise OSError(f"libpdgpu.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def exported_symbols(path: str = LIB_PATH):
+    lib = C.CDLL(path)
+    return {name for name in SIGNATURES if hasattr(lib, name)}

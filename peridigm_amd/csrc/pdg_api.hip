@@ -1,0 +1,717 @@
+// C-ABI of the MI355X peridynamic force path: context, topology, constants,
+// drop-in computeForce / computeDamage, and the device-resident explicit
+// velocity-Verlet driver.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pdg_internal.hpp"
+#include "pdg_ctx.hpp"
+
+using namespace pdg;
+
+namespace {
+
+__global__ void k_set_u(int n, const double* __restrict__ y, const int* __restrict__ perm,
+                        NodeRec* __restrict__ rec, double L) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 3 * n) return;
+  const int k = q / 3, c = q - 3 * (q / 3);
+  const int p = perm[k];
+  const double x = rec[k].x[c] * L;
+  rec[k].u[c] = (y[3 * p + c] - x) * (1.0 / L);
+}
+
+__global__ void k_set_u_disp(int n, const double* __restrict__ u, const int* __restrict__ perm,
+                             NodeRec* __restrict__ rec, double L) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 3 * n) return;
+  const int k = q / 3, c = q - 3 * (q / 3);
+  rec[k].u[c] = u[3 * perm[k] + c] * (1.0 / L);
+}
+
+uint64_t spread3(uint64_t v) {
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffull;
+  v = (v | v << 16) & 0x1f0000ff0000ffull;
+  v = (v | v << 8) & 0x100f00f00f00f00full;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+
+}  // namespace
+
+namespace pdg {
+
+int Ctx::fail(int code, const std::string& msg) {
+  err = msg;
+  return code;
+}
+
+int Ctx::hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PDG_OK;
+  err = std::string(what) + ": " + hipGetErrorString(e);
+  return PDG_E_HIP;
+}
+
+void Ctx::free_extra() {
+  for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+  tev.clear();
+}
+
+void Ctx::free_device() {
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(d_row); F(d_col); F(d_mw); F(d_mask); F(d_nc); F(d_rec[0]); F(d_rec[1]); F(d_theta);
+  F(d_nbroken); F(d_v); F(d_a); F(d_f); F(d_rho); F(d_bcslot); F(d_bcs); F(d_err); F(d_near);
+  F(d_perm); F(d_fout); F(d_y);
+  d_row = d_col = d_mw = d_nbroken = d_bcslot = d_err = d_perm = nullptr;
+  d_mask = d_near = nullptr;
+  d_nc = nullptr; d_rec[0] = d_rec[1] = nullptr; d_theta = nullptr;
+  d_v = d_a = d_f = d_rho = d_fout = d_y = nullptr;
+  d_bcs = nullptr;
+}
+
+BondArgs Ctx::bond_args() const {
+  BondArgs a;
+  a.n = N; a.row = d_row; a.col = d_col; a.mw = d_mw; a.mask = d_mask; a.nc = d_nc;
+  a.near_rel = 1e-6f; a.near_ctr = d_near; a.influence = infl;
+  return a;
+}
+
+// Host list (reference order) -> device mask words (internal order).
+int Ctx::upload_mask(const double* bd, bool check_symmetric) {
+  std::vector<unsigned long long> words(mw.back(), 0ull);
+  for (int i = 0; i < N; i++) {
+    const int rs = row[i], deg = row[i + 1] - rs;
+    for (int k = 0; k < deg; k++) {
+      const double d = bd ? bd[src[rs + k]] : 0.0;
+      if (d != 0.0 && d != 1.0)
+        return fail(PDG_E_UNSUPPORTED, "bond damage values must be 0 or 1");
+      if (d == 1.0) words[mw[i] + (k >> 6)] |= 1ull << (k & 63);
+      if (check_symmetric && bd && bd[src[rev[rs + k]]] != d)
+        return fail(PDG_E_UNSUPPORTED, "bond damage must be symmetric (d_ij == d_ji)");
+    }
+  }
+  return hip(hipMemcpyAsync(d_mask, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream),
+             "upload mask");
+}
+
+int Ctx::download_mask(double* bd) {
+  std::vector<unsigned long long> words(mw.back());
+  int rc = hip(hipMemcpyAsync(words.data(), d_mask, words.size() * 8, hipMemcpyDeviceToHost, stream),
+               "download mask");
+  if (rc) return rc;
+  rc = hip(hipStreamSynchronize(stream), "sync");
+  if (rc) return rc;
+  for (int i = 0; i < N; i++) {
+    const int rs = row[i], deg = row[i + 1] - rs;
+    for (int k = 0; k < deg; k++)
+      bd[src[rs + k]] = ((words[mw[i] + (k >> 6)] >> (k & 63)) & 1ull) ? 1.0 : 0.0;
+  }
+  return PDG_OK;
+}
+
+// internal-order 3-vectors (device) -> reference order (host)
+int Ctx::download_vec3(const double* dsrc, double* out) {
+  std::vector<double> tmp(3 * (size_t)N);
+  int rc = hip(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * 8, hipMemcpyDeviceToHost, stream), "d2h");
+  if (rc) return rc;
+  rc = hip(hipStreamSynchronize(stream), "sync");
+  if (rc) return rc;
+  for (int k = 0; k < N; k++)
+    for (int c = 0; c < 3; c++) out[3 * (size_t)perm[k] + c] = tmp[3 * (size_t)k + c];
+  return PDG_OK;
+}
+
+int Ctx::upload_vec3(const double* in, double* ddst) {
+  std::vector<double> tmp(3 * (size_t)N);
+  for (int k = 0; k < N; k++)
+    for (int c = 0; c < 3; c++) tmp[3 * (size_t)k + c] = in ? in[3 * (size_t)perm[k] + c] : 0.0;
+  int rc = hip(hipMemcpyAsync(ddst, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, stream), "h2d");
+  if (rc) return rc;
+  return hip(hipStreamSynchronize(stream), "sync");
+}
+
+}  // namespace pdg
+
+extern "C" {
+
+int pdg_ctx_create(int device, pdg_ctx** out) {
+  if (!out) return PDG_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PDG_E_HIP;
+  Ctx* c = new Ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return PDG_E_HIP;
+  }
+  for (int i = 0; i < 4; i++) (void)hipEventCreate(&c->ev[i]);
+  *out = reinterpret_cast<pdg_ctx*>(c);
+  return PDG_OK;
+}
+
+int pdg_ctx_destroy(pdg_ctx* h) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->free_device();
+  c->free_extra();
+  for (int i = 0; i < 4; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PDG_OK;
+}
+
+const char* pdg_last_error(const pdg_ctx* h) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  return c ? c->err.c_str() : "null context";
+}
+
+int pdg_set_lanes_per_node(pdg_ctx* h, int lanes) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (lanes != 16 && lanes != 32 && lanes != 64)
+    return c->fail(PDG_E_ARG, "lanes per node must be 16, 32 or 64");
+  c->lanes = lanes;
+  return PDG_OK;
+}
+
+int pdg_set_topology(pdg_ctx* h, int numOwned, const int* ownedIDs, const int* nlist,
+                     long long listSize) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (numOwned < 0 || (numOwned > 0 && (!nlist || listSize < numOwned)))
+    return c->fail(PDG_E_ARG, "invalid neighbourhood list");
+  c->have_topo = false;
+  c->have_consts = false;
+  c->N = numOwned;
+  if (ownedIDs)
+    for (int i = 0; i < numOwned; i++)
+      if (ownedIDs[i] != i)
+        return c->fail(PDG_E_UNSUPPORTED, "ownedIDs must be 0..numOwnedPoints-1 (single device)");
+  c->nlist.assign(nlist, nlist + listSize);
+  c->head.assign(numOwned, 0);
+  long long li = 0, B = 0;
+  for (int i = 0; i < numOwned; i++) {
+    if (li >= listSize) return c->fail(PDG_E_ARG, "neighbourhood list too short");
+    c->head[i] = li;
+    const int nn = nlist[li];
+    if (nn < 0 || li + 1 + nn > listSize) return c->fail(PDG_E_ARG, "neighbourhood list truncated");
+    for (int k = 0; k < nn; k++) {
+      const int j = nlist[li + 1 + k];
+      if (j < 0 || j >= numOwned)
+        return c->fail(PDG_E_UNSUPPORTED, "neighbour id outside the owned points (ghosts unsupported)");
+      if (j == i) return c->fail(PDG_E_UNSUPPORTED, "self bond in neighbourhood list");
+    }
+    li += nn + 1;
+    B += nn;
+  }
+  if (B >= (1ll << 31)) return c->fail(PDG_E_UNSUPPORTED, "more than 2^31-1 bonds");
+  c->B = B;
+  c->have_topo = true;
+  return PDG_OK;
+}
+
+int pdg_get_sizes(const pdg_ctx* h, int* n, long long* b) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c || !c->have_topo) return PDG_E_ARG;
+  if (n) *n = c->N;
+  if (b) *b = c->B;
+  return PDG_OK;
+}
+
+int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, double mu,
+                      double horizon, int infl, double sc) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (!c->have_topo) return c->fail(PDG_E_ARG, "pdg_set_topology must be called first");
+  if (!x || !vol || !(horizon > 0.0) || infl < 0 || infl > 2)
+    return c->fail(PDG_E_ARG, "invalid constants");
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(PDG_E_HIP, "hipSetDevice");
+  const int N = c->N;
+  c->K = K; c->mu = mu; c->horizon = horizon; c->infl = infl; c->sc = sc;
+  c->x.assign(x, x + 3 * (size_t)N);
+  c->vol.assign(vol, vol + N);
+
+  // Morton order of the nodes (cells of horizon/2), ties by original id.
+  double lo[3] = {1e300, 1e300, 1e300};
+  for (int i = 0; i < N; i++)
+    for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], x[3 * (size_t)i + d]);
+  const double cell = horizon * 0.5;
+  std::vector<uint64_t> key(N);
+  for (int i = 0; i < N; i++) {
+    uint64_t q[3];
+    for (int d = 0; d < 3; d++) {
+      double t = std::floor((x[3 * (size_t)i + d] - lo[d]) / cell);
+      q[d] = (uint64_t)std::min(std::max(t, 0.0), 2097151.0);
+    }
+    key[i] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+  }
+  c->perm.resize(N);
+  std::iota(c->perm.begin(), c->perm.end(), 0);
+  std::sort(c->perm.begin(), c->perm.end(),
+            [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+  c->iperm.resize(N);
+  for (int k = 0; k < N; k++) c->iperm[c->perm[k]] = k;
+
+  // Internal CSR, rows sorted by internal neighbour id; src = list-order bond index.
+  c->row.assign(N + 1, 0);
+  for (int k = 0; k < N; k++) c->row[k + 1] = c->row[k] + c->nlist[c->head[c->perm[k]]];
+  c->col.resize(c->B);
+  c->src.resize(c->B);
+  std::vector<long long> bond0(N);  // first list-order bond index of each original node
+  {
+    long long b = 0;
+    for (int p = 0; p < N; p++) { bond0[p] = b; b += c->nlist[c->head[p]]; }
+  }
+  std::vector<std::pair<int, int>> tmp;
+  for (int k = 0; k < N; k++) {
+    const int p = c->perm[k];
+    const long long h0 = c->head[p];
+    const int nn = c->nlist[h0];
+    tmp.resize(nn);
+    for (int q = 0; q < nn; q++) tmp[q] = {c->iperm[c->nlist[h0 + 1 + q]], (int)(bond0[p] + q)};
+    std::sort(tmp.begin(), tmp.end());
+    for (int q = 0; q < nn; q++) {
+      if (q > 0 && tmp[q].first == tmp[q - 1].first)
+        return c->fail(PDG_E_UNSUPPORTED, "duplicate neighbour in neighbourhood list");
+      c->col[c->row[k] + q] = tmp[q].first;
+      c->src[c->row[k] + q] = tmp[q].second;
+    }
+  }
+  // Reverse bonds; the gather-only force needs a symmetric list.
+  c->rev.resize(c->B);
+  for (int k = 0; k < N; k++)
+    for (int b = c->row[k]; b < c->row[k + 1]; b++) {
+      const int j = c->col[b];
+      const int* s = c->col.data() + c->row[j];
+      const int* e = c->col.data() + c->row[j + 1];
+      const int* it = std::lower_bound(s, e, k);
+      if (it == e || *it != k)
+        return c->fail(PDG_E_UNSUPPORTED, "neighbourhood list is not symmetric");
+      c->rev[b] = (int)(it - c->col.data());
+    }
+  c->mw.assign(N + 1, 0);
+  for (int k = 0; k < N; k++) c->mw[k + 1] = c->mw[k] + (c->row[k + 1] - c->row[k] + 63) / 64;
+
+  // Scales: L and S powers of two near the horizon and the mean volume.
+  double vsum = 0.0;
+  for (int i = 0; i < N; i++) vsum += vol[i];
+  const double vmean = N ? vsum / N : 1.0;
+  c->L = std::ldexp(1.0, (int)std::lround(std::log2(horizon)));
+  c->S = std::ldexp(1.0, vmean > 0 ? (int)std::lround(std::log2(vmean)) : 0);
+
+  c->free_device();
+  auto A = [&](void** p, size_t bytes, const char* what) {
+    return c->hip(hipMalloc(p, std::max<size_t>(bytes, 64)), what);
+  };
+  int rc = 0;
+  if ((rc = A((void**)&c->d_row, (N + 1) * 4, "alloc row"))) return rc;
+  if ((rc = A((void**)&c->d_col, c->B * 4, "alloc col"))) return rc;
+  if ((rc = A((void**)&c->d_mw, (N + 1) * 4, "alloc mw"))) return rc;
+  if ((rc = A((void**)&c->d_mask, (size_t)c->mw.back() * 8, "alloc mask"))) return rc;
+  if ((rc = A((void**)&c->d_nc, (size_t)N * sizeof(NodeConst), "alloc nc"))) return rc;
+  if ((rc = A((void**)&c->d_rec[0], (size_t)N * sizeof(NodeRec), "alloc rec0"))) return rc;
+  if ((rc = A((void**)&c->d_rec[1], (size_t)N * sizeof(NodeRec), "alloc rec1"))) return rc;
+  if ((rc = A((void**)&c->d_theta, (size_t)N * 4, "alloc theta"))) return rc;
+  if ((rc = A((void**)&c->d_nbroken, (size_t)N * 4, "alloc nbroken"))) return rc;
+  if ((rc = A((void**)&c->d_v, (size_t)N * 24, "alloc v"))) return rc;
+  if ((rc = A((void**)&c->d_a, (size_t)N * 24, "alloc a"))) return rc;
+  if ((rc = A((void**)&c->d_f, (size_t)N * 24, "alloc f"))) return rc;
+  if ((rc = A((void**)&c->d_fout, (size_t)N * 24, "alloc fout"))) return rc;
+  if ((rc = A((void**)&c->d_y, (size_t)N * 24, "alloc y"))) return rc;
+  if ((rc = A((void**)&c->d_rho, (size_t)N * 8, "alloc rho"))) return rc;
+  if ((rc = A((void**)&c->d_bcslot, (size_t)N * 4, "alloc bcslot"))) return rc;
+  if ((rc = A((void**)&c->d_bcs, sizeof(BcSlot), "alloc bcs"))) return rc;
+  if ((rc = A((void**)&c->d_err, 4, "alloc err"))) return rc;
+  if ((rc = A((void**)&c->d_near, 8, "alloc near"))) return rc;
+  if ((rc = A((void**)&c->d_perm, (size_t)N * 4, "alloc perm"))) return rc;
+
+  hipStream_t s = c->stream;
+  // Weighted volume on the device in the reference's summation order.
+  {
+    double *dx = nullptr, *dv = nullptr, *dm = nullptr;
+    int* dl = nullptr;
+    long long* dh = nullptr;
+    rc = A((void**)&dx, 24 * (size_t)N, "alloc x");
+    if (!rc) rc = A((void**)&dv, 8 * (size_t)N, "alloc vol");
+    if (!rc) rc = A((void**)&dm, 8 * (size_t)N, "alloc m");
+    if (!rc) rc = A((void**)&dl, 4 * c->nlist.size(), "alloc list");
+    if (!rc) rc = A((void**)&dh, 8 * (size_t)N, "alloc head");
+    if (!rc) rc = c->hip(hipMemcpyAsync(dx, x, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d x");
+    if (!rc) rc = c->hip(hipMemcpyAsync(dv, vol, 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d vol");
+    if (!rc) rc = c->hip(hipMemcpyAsync(dl, c->nlist.data(), 4 * c->nlist.size(), hipMemcpyHostToDevice, s), "h2d list");
+    if (!rc) rc = c->hip(hipMemcpyAsync(dh, c->head.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d head");
+    if (!rc) rc = c->hip(launch_weighted_volume(N, dx, dv, dl, dh, horizon, infl, dm, s), "weighted volume");
+    c->m.assign(N, 0.0);
+    if (!rc) rc = c->hip(hipMemcpyAsync(c->m.data(), dm, 8 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h m");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+    (void)hipFree(dx); (void)hipFree(dv); (void)hipFree(dm); (void)hipFree(dl); (void)hipFree(dh);
+    if (rc) return rc;
+  }
+
+  // Node records and constants (internal order).
+  std::vector<NodeRec> rec(N);
+  std::vector<NodeConst> nc(N);
+  const double L = c->L, S = c->S, invL = 1.0 / L;
+  for (int k = 0; k < N; k++) {
+    const int p = c->perm[k];
+    const double m = c->m[p];
+    NodeRec& r = rec[k];
+    for (int d = 0; d < 3; d++) { r.x[d] = x[3 * (size_t)p + d] * invL; r.u[d] = 0.0; }
+    r.vol = (float)(vol[p] / S);
+    r.p = 0.0f;
+    r.pad = 0.0f;
+    NodeConst& q = nc[k];
+    q.sc = sc > 0.0 ? (float)sc : 0.0f;
+    q.hhat = (float)(horizon * invL);
+    if (m > 0.0) {
+      const double alpha = 15.0 * mu / m;
+      const double beta = 3.0 * K / m - alpha / 3.0;
+      const double ct = 3.0 * L * L * S / m;
+      r.alpha = (float)(alpha * L * S);
+      q.ctheta = (float)ct;
+      q.kp = (float)(ct * beta * L * S);
+    } else {
+      r.alpha = 0.0f; q.ctheta = 0.0f; q.kp = 0.0f;
+    }
+  }
+  std::vector<int> noslot(N, -1);
+  std::vector<double> rho1(N, 1.0);
+  rc = c->hip(hipMemcpyAsync(c->d_row, c->row.data(), 4 * (N + 1), hipMemcpyHostToDevice, s), "h2d row");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_col, c->col.data(), 4 * c->B, hipMemcpyHostToDevice, s), "h2d col");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_mw, c->mw.data(), 4 * (N + 1), hipMemcpyHostToDevice, s), "h2d mw");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_mask, 0, (size_t)c->mw.back() * 8, s), "zero mask");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_nc, nc.data(), N * sizeof(NodeConst), hipMemcpyHostToDevice, s), "h2d nc");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_rec[0], rec.data(), N * sizeof(NodeRec), hipMemcpyHostToDevice, s), "h2d rec0");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_rec[1], rec.data(), N * sizeof(NodeRec), hipMemcpyHostToDevice, s), "h2d rec1");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_v, 0, 24 * (size_t)N, s), "zero v");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_a, 0, 24 * (size_t)N, s), "zero a");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_f, 0, 24 * (size_t)N, s), "zero f");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_theta, 0, 4 * (size_t)N, s), "zero theta");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_nbroken, 0, 4 * (size_t)N, s), "zero nbroken");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_bcslot, noslot.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d bcslot");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_rho, rho1.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d rho");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_perm, c->perm.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d perm");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (rc) return rc;
+  c->cur = 0;
+  c->have_consts = true;
+  return PDG_OK;
+}
+
+int pdg_set_density(pdg_ctx* h, const double* rho) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !rho) return PDG_E_ARG;
+  std::vector<double> r(c->N);
+  for (int k = 0; k < c->N; k++) {
+    r[k] = rho[c->perm[k]];
+    if (!(r[k] > 0.0)) return c->fail(PDG_E_ARG, "density must be positive");
+  }
+  int rc = c->hip(hipMemcpyAsync(c->d_rho, r.data(), 8 * (size_t)c->N, hipMemcpyHostToDevice, c->stream), "h2d rho");
+  if (rc) return rc;
+  return c->hip(hipStreamSynchronize(c->stream), "sync");
+}
+
+int pdg_initialize(pdg_ctx* h, double* m) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !m) return PDG_E_ARG;
+  std::copy(c->m.begin(), c->m.end(), m);
+  return PDG_OK;
+}
+
+int pdg_compute_damage(pdg_ctx* h, const double* y, const double* bdN, double* bdNP1,
+                       double* damage, long long* nearCritical) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !y || !bdNP1 || !damage) return PDG_E_ARG;
+  if (!(c->sc > 0.0)) return c->fail(PDG_E_ARG, "no damage model (critical stretch <= 0)");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const int N = c->N;
+  int rc = c->hip(hipMemcpyAsync(c->d_y, y, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d y");
+  if (!rc && N > 0)
+    hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y, c->d_perm,
+                       c->d_rec[c->cur], c->L);
+  if (!rc) rc = c->upload_mask(bdN, false);
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
+  if (!rc) rc = c->hip(launch_pass1(c->bond_args(), c->d_rec[c->cur], c->d_theta, c->d_nbroken,
+                                    true, nearCritical != nullptr, c->lanes, s), "pass1");
+  if (!rc) rc = c->download_mask(bdNP1);
+  if (rc) return rc;
+  std::vector<int> nb(N);
+  rc = c->hip(hipMemcpyAsync(nb.data(), c->d_nbroken, 4 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h nb");
+  unsigned long long near = 0;
+  if (!rc && nearCritical)
+    rc = c->hip(hipMemcpyAsync(&near, c->d_near, 8, hipMemcpyDeviceToHost, s), "d2h near");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (rc) return rc;
+  for (int k = 0; k < N; k++) {
+    const int deg = c->row[k + 1] - c->row[k];
+    double t = (double)nb[k];
+    damage[c->perm[k]] = deg > 0 ? t / deg : 0.0;
+  }
+  if (nearCritical) *nearCritical = (long long)near;
+  return PDG_OK;
+}
+
+int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP1,
+                      double* dilatation, double* force) {
+  (void)dt;
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !y || !force) return PDG_E_ARG;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const int N = c->N;
+  int rc = c->hip(hipMemcpyAsync(c->d_y, y, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d y");
+  if (!rc && N > 0)
+    hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y, c->d_perm,
+                       c->d_rec[c->cur], c->L);
+  if (!rc) rc = c->upload_mask(bdNP1, true);
+  if (!rc) rc = c->hip(launch_pass1(c->bond_args(), c->d_rec[c->cur], c->d_theta, nullptr, false,
+                                    false, c->lanes, s), "pass1");
+  if (!rc) rc = c->hip(launch_pass2_force(c->bond_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s),
+                       "pass2");
+  if (!rc) rc = c->download_vec3(c->d_fout, force);
+  if (rc) return rc;
+  if (dilatation) {
+    std::vector<float> th(N);
+    rc = c->hip(hipMemcpyAsync(th.data(), c->d_theta, 4 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h theta");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    for (int k = 0; k < N; k++) dilatation[c->perm[k]] = (double)th[k];
+  }
+  return PDG_OK;
+}
+
+int pdg_state_upload(pdg_ctx* h, const double* u, const double* v, const double* a,
+                     const double* bd) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts) return PDG_E_ARG;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const int N = c->N;
+  int rc = 0;
+  if (u) {
+    rc = c->hip(hipMemcpyAsync(c->d_y, u, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d u");
+    if (!rc && N > 0)
+      hipLaunchKernelGGL(k_set_u_disp, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y,
+                         c->d_perm, c->d_rec[c->cur], c->L);
+  } else if (N > 0) {
+    std::vector<double> z(3 * (size_t)N, 0.0);
+    rc = c->hip(hipMemcpyAsync(c->d_y, z.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d u");
+    if (!rc)
+      hipLaunchKernelGGL(k_set_u_disp, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y,
+                         c->d_perm, c->d_rec[c->cur], c->L);
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  }
+  if (!rc) rc = c->upload_vec3(v, c->d_v);
+  if (!rc) rc = c->upload_vec3(a, c->d_a);
+  if (!rc) rc = c->upload_mask(bd, true);
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  return rc;
+}
+
+int pdg_set_bcs(pdg_ctx* h, const pdg_bc* bcs, int nb) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || nb < 0 || (nb > 0 && !bcs)) return PDG_E_ARG;
+  std::vector<int> slot(c->N, -1);
+  std::vector<BcSlot> slots;
+  for (int b = 0; b < nb; b++) {
+    if (bcs[b].coord < 0 || bcs[b].coord > 2 || bcs[b].numNodes < 0)
+      return c->fail(PDG_E_ARG, "invalid boundary condition");
+    for (int q = 0; q < bcs[b].numNodes; q++) {
+      const int p = bcs[b].nodes[q];
+      if (p < 0 || p >= c->N) return c->fail(PDG_E_ARG, "boundary-condition node out of range");
+      const int k = c->iperm[p];
+      if (slot[k] < 0) {
+        slot[k] = (int)slots.size();
+        slots.push_back(BcSlot{{0, 0, 0}, {0, 0, 0}, 0, 0});
+      }
+      BcSlot& s = slots[slot[k]];
+      s.c0[bcs[b].coord] = bcs[b].c0;
+      s.c1[bcs[b].coord] = bcs[b].c1;
+      s.mask |= 1 << bcs[b].coord;
+    }
+  }
+  if (slots.empty()) slots.push_back(BcSlot{{0, 0, 0}, {0, 0, 0}, 0, 0});
+  if (c->d_bcs) (void)hipFree(c->d_bcs);
+  c->d_bcs = nullptr;
+  int rc = c->hip(hipMalloc((void**)&c->d_bcs, slots.size() * sizeof(BcSlot)), "alloc bcs");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_bcs, slots.data(), slots.size() * sizeof(BcSlot), hipMemcpyHostToDevice, c->stream), "h2d bcs");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_bcslot, slot.data(), 4 * (size_t)c->N, hipMemcpyHostToDevice, c->stream), "h2d bcslot");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
+int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const bool damage = c->sc > 0.0;
+  BondArgs ba = c->bond_args();
+  VerletArgs va{};
+  va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
+  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.err = c->d_err;
+  int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
+  if (rc) return rc;
+  c->t_pass1 = c->t_pass2 = 0.0;
+  rc = c->hip(hipEventRecord(c->ev[0], s), "event");
+  if (rc) return rc;
+  auto step_time = [&](int k) { return t0 + (k * dt); };
+  float ms = 0.0f;
+  // Per-pass timing: three events per force evaluation, recorded on the
+  // context's stream with no host synchronisation inside the run.
+  const size_t nev = c->per_pass_timing ? 3 * (size_t)(nsteps + 1) : 0;
+  while (c->tev.size() < nev) {
+    hipEvent_t e;
+    rc = c->hip(hipEventCreate(&e), "event create");
+    if (rc) return rc;
+    c->tev.push_back(e);
+  }
+  int nevals = 0;
+  // pass1 + pass2 of one force evaluation, finishing step `k` (k=0: initial)
+  auto eval = [&](int k, bool end_kick) -> int {
+    int r = 0;
+    if (nev) r = c->hip(hipEventRecord(c->tev[3 * nevals], s), "event");
+    if (!r) r = c->hip(launch_pass1(ba, c->d_rec[c->cur], c->d_theta, c->d_nbroken, damage, false,
+                                    c->lanes, s), "pass1");
+    if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 1], s), "event");
+    va.end_kick = end_kick ? 1 : 0;
+    va.start_step = k < nsteps ? 1 : 0;
+    va.t_cur = step_time(k + 1);
+    va.t_prev = step_time(k);
+    if (!r) r = c->hip(launch_pass2_verlet(ba, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va,
+                                           c->lanes, s), "pass2");
+    if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 2], s), "event");
+    nevals++;
+    c->cur ^= 1;
+    return r;
+  };
+  if (initialEval) {
+    rc = eval(0, false);
+  } else if (nsteps > 0) {
+    va.t_cur = step_time(1);
+    va.t_prev = step_time(0);
+    rc = c->hip(launch_verlet_prologue(c->N, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, s), "prologue");
+    c->cur ^= 1;
+  }
+  for (int k = 1; k <= nsteps && !rc; k++) rc = eval(k, true);
+  if (rc) return rc;
+  rc = c->hip(hipEventRecord(c->ev[3], s), "event");
+  if (!rc) rc = c->hip(hipEventSynchronize(c->ev[3]), "sync");
+  if (!rc) { (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[3]); c->t_total = ms; }
+  c->n_pass = nev ? nevals : 0;
+  for (int q = 0; q < (nev ? nevals : 0) && !rc; q++) {
+    (void)hipEventElapsedTime(&ms, c->tev[3 * q], c->tev[3 * q + 1]);
+    c->t_pass1 += ms;
+    (void)hipEventElapsedTime(&ms, c->tev[3 * q + 1], c->tev[3 * q + 2]);
+    c->t_pass2 += ms;
+  }
+  int err = 0;
+  if (!rc) rc = c->hip(hipMemcpy(&err, c->d_err, 4, hipMemcpyDeviceToHost), "d2h err");
+  if (rc) return rc;
+  if (err) return c->fail(PDG_E_NONFINITE, "NaN returned by force evaluation");
+  return PDG_OK;
+}
+
+int pdg_state_download(pdg_ctx* h, double* u, double* v, double* a, double* bd, double* damage,
+                       double* dilatation, double* force) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts) return PDG_E_ARG;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const int N = c->N;
+  int rc = 0;
+  if (u) {
+    std::vector<NodeRec> rec(N);
+    rc = c->hip(hipMemcpyAsync(rec.data(), c->d_rec[c->cur], N * sizeof(NodeRec), hipMemcpyDeviceToHost, s), "d2h rec");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    for (int k = 0; k < N; k++)
+      for (int d = 0; d < 3; d++) u[3 * (size_t)c->perm[k] + d] = rec[k].u[d] * c->L;
+  }
+  if (v && (rc = c->download_vec3(c->d_v, v))) return rc;
+  if (a && (rc = c->download_vec3(c->d_a, a))) return rc;
+  if (force && (rc = c->download_vec3(c->d_f, force))) return rc;
+  if (bd && (rc = c->download_mask(bd))) return rc;
+  if (damage || dilatation) {
+    std::vector<int> nb(N);
+    std::vector<float> th(N);
+    rc = c->hip(hipMemcpyAsync(nb.data(), c->d_nbroken, 4 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h nb");
+    if (!rc) rc = c->hip(hipMemcpyAsync(th.data(), c->d_theta, 4 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h th");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    for (int k = 0; k < N; k++) {
+      const int deg = c->row[k + 1] - c->row[k];
+      if (damage) damage[c->perm[k]] = (c->sc > 0.0 && deg > 0) ? (double)nb[k] / deg : 0.0;
+      if (dilatation) dilatation[c->perm[k]] = (double)th[k];
+    }
+  }
+  return PDG_OK;
+}
+
+int pdg_critical_time_step(pdg_ctx* h, double density, double* dtOut) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !dtOut) return PDG_E_ARG;
+  // src/core/Peridigm_CriticalTimeStep.cpp:54-126 (constant horizon)
+  const double pi = 3.14159265358979323846;
+  const double hz = c->horizon;
+  const double spring = 18.0 * c->K / (pi * hz * hz * hz * hz);
+  double best = 1.0e50;
+  for (int p = 0; p < c->N; p++) {
+    const long long h0 = c->head[p];
+    const int nn = c->nlist[h0];
+    const double* X = &c->x[3 * (size_t)p];
+    double den = 0.0;
+    for (int k = 0; k < nn; k++) {
+      const int j = c->nlist[h0 + 1 + k];
+      const double* Y = &c->x[3 * (size_t)j];
+      const double d = std::sqrt((X[0] - Y[0]) * (X[0] - Y[0]) + (X[1] - Y[1]) * (X[1] - Y[1]) +
+                                 (X[2] - Y[2]) * (X[2] - Y[2]));
+      den += c->vol[j] * spring / d;
+    }
+    double dt = 1.0e50;
+    if (nn > 0) dt = std::sqrt(2.0 * density / den);
+    if (dt < best) best = dt;
+  }
+  *dtOut = best;
+  return PDG_OK;
+}
+
+int pdg_last_timing(const pdg_ctx* h, double* total, double* p1, double* p2) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (total) *total = c->t_total;
+  if (p1) *p1 = c->t_pass1;
+  if (p2) *p2 = c->t_pass2;
+  return PDG_OK;
+}
+
+int pdg_set_per_pass_timing(pdg_ctx* h, int on) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  c->per_pass_timing = on != 0;
+  return PDG_OK;
+}
+
+int pdg_near_critical_count(const pdg_ctx* h, long long* count) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c || !count) return PDG_E_ARG;
+  unsigned long long v = 0;
+  if (hipMemcpy(&v, c->d_near, 8, hipMemcpyDeviceToHost) != hipSuccess) return PDG_E_HIP;
+  *count = (long long)v;
+  return PDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// The context behind the opaque pdg_ctx handle.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "pdg_internal.hpp"
+
+namespace pdg {
+
+struct QsState;   // quasi-static solver workspace (pdg_qs.hip)
+struct GnnState;  // learned preconditioner (pdg_gnn.hip)
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string err;
+  int lanes = 16;
+  bool per_pass_timing = false;
+  double t_total = 0.0, t_pass1 = 0.0, t_pass2 = 0.0;
+  int n_pass = 0;                 // force evaluations timed in the last run
+  std::vector<hipEvent_t> tev;    // per-pass timing events
+
+  // topology (host, reference order)
+  bool have_topo = false, have_consts = false;
+  int N = 0;
+  long long B = 0;
+  std::vector<int> nlist;
+  std::vector<long long> head;
+
+  // Morton order and internal CSR
+  std::vector<int> perm, iperm;  // perm[k]: original id of internal node k
+  std::vector<int> row, col;     // CSR over internal ids, rows sorted
+  std::vector<int> src;          // internal bond -> list-order bond index
+  std::vector<int> rev;          // internal bond (i->j) -> internal bond (j->i)
+  std::vector<int> mw;           // first mask word of each node
+
+  // constants
+  double K = 0, mu = 0, horizon = 0, sc = 0, L = 1, S = 1;
+  int infl = 0;
+  std::vector<double> x, vol, m;  // reference order
+
+  // device state
+  int cur = 0;
+  int* d_row = nullptr;
+  int* d_col = nullptr;
+  int* d_mw = nullptr;
+  unsigned long long* d_mask = nullptr;
+  NodeConst* d_nc = nullptr;
+  NodeRec* d_rec[2] = {nullptr, nullptr};
+  float* d_theta = nullptr;
+  int* d_nbroken = nullptr;
+  double* d_v = nullptr;
+  double* d_a = nullptr;
+  double* d_f = nullptr;
+  double* d_fout = nullptr;
+  double* d_y = nullptr;
+  double* d_rho = nullptr;
+  int* d_bcslot = nullptr;
+  BcSlot* d_bcs = nullptr;
+  int* d_err = nullptr;
+  unsigned long long* d_near = nullptr;
+  int* d_perm = nullptr;
+
+  QsState* qs = nullptr;
+  GnnState* gnn = nullptr;
+
+  int fail(int code, const std::string& msg);
+  int hip(hipError_t e, const char* what);
+  void free_device();
+  void free_extra();
+  BondArgs bond_args() const;
+  int upload_mask(const double* bd, bool check_symmetric);
+  int download_mask(double* bd);
+  int download_vec3(const double* dsrc, double* out);
+  int upload_vec3(const double* in, double* ddst);
+};
+
+}  // namespace pdg

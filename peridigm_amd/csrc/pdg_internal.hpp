@@ -1,0 +1,89 @@
+// Internal declarations shared by the pdgpu HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pdgpu.h"
+
+namespace pdg {
+
+// One node's bond-gather record, 64 bytes (half a 128-byte cache line):
+// everything a neighbour needs from node j in either bond pass, so a gather
+// touches exactly one aligned 64-byte segment.
+//   x: reference position / L (FP64), u: displacement / L (FP64),
+//   vol: V / S, alpha: 15 mu / m * L * S, p: theta * (3K/m - alpha/3) * L * S.
+// L and S are powers of two, so the scalings are exact.
+struct __align__(64) NodeRec {
+  double x[3];
+  double u[3];
+  float vol;
+  float alpha;
+  float p;
+  float pad;
+};
+static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
+
+// Per-node constants of the own-node side of both passes:
+//   ctheta = 3 L^2 S / m      (theta = ctheta * sum_j w (1-d) zeta e V)
+//   kp     = ctheta * (3K/m - alpha/3) * L * S   (p = kp * same sum)
+//   sc     = critical stretch (0: damage disabled for this node)
+struct __align__(16) NodeConst {
+  float ctheta;
+  float kp;
+  float sc;
+  float hhat;  // horizon / L (influence functions)
+};
+
+// Boundary-condition slot of a node: u_c(t) = c1[c] * t + c0[c] for the
+// coordinates whose bit is set in `mask`.
+struct BcSlot {
+  double c0[3];
+  double c1[3];
+  int mask;
+  int pad;
+};
+
+// Arguments shared by the two bond passes.
+struct BondArgs {
+  int n;                       // nodes
+  const int* row;              // CSR row offsets, n+1
+  const int* col;              // CSR neighbour ids (internal order), B
+  const int* mw;               // first 64-bit mask word of each node, n+1
+  unsigned long long* mask;    // bond-broken bits
+  const NodeConst* nc;         // per-node constants
+  float near_rel;              // near-critical band (relative)
+  unsigned long long* near_ctr;
+  int influence;               // PDG_INFLUENCE_*
+};
+
+struct VerletArgs {
+  const double* rho;  // n
+  double* v;          // 3n, FP64
+  double* a;          // 3n, FP64 (written on the last step only)
+  double* f;          // 3n, FP64 force density (written on the last step only)
+  const int* bc_slot; // n, -1 if none
+  const BcSlot* bcs;
+  double dt, t_cur, t_prev, L;
+  int end_kick;       // finish the current step: v += dt/2 a
+  int start_step;     // begin the next step: v += dt/2 a, BC, u += dt v
+  int* err;           // set to 1 on a non-finite force
+};
+
+// Launchers (pdg_bonds.hip). `lanes` = lanes per node (16, 32 or 64).
+hipError_t launch_pass1(const BondArgs& a, NodeRec* rec, float* theta, int* nbroken,
+                        bool damage, bool count_near, int lanes, hipStream_t s);
+hipError_t launch_pass2_force(const BondArgs& a, const NodeRec* rec, double* f, int lanes,
+                              hipStream_t s);
+hipError_t launch_pass2_verlet(const BondArgs& a, const NodeRec* rec, NodeRec* rec_next,
+                               const VerletArgs& va, int lanes, hipStream_t s);
+hipError_t launch_verlet_prologue(int n, NodeRec* rec, NodeRec* rec_next, const VerletArgs& va,
+                                  hipStream_t s);
+hipError_t launch_weighted_volume(int n, const double* x, const double* vol, const int* nlist,
+                                  const long long* head, double horizon, int influence, double* m,
+                                  hipStream_t s);
+
+}  // namespace pdg
