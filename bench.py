@@ -92,7 +92,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--workload", default="plate")
     ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (3x) to calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -102,15 +105,25 @@ def main():
     import torch
     from peridigm_amd import Context
 
+    if args.copy_calibration:
+        # an elementwise kernel that reads 256 MiB and writes 256 MiB
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+
     p, nl = build(args.workload)
     N = len(p["volume"])
-    ctx = Context(0, lanes=args.lanes)
+    ctx = Context(0, lanes=args.lanes, variant=args.variant)
     ctx.set_topology(nl, N)
     B = ctx.B
     ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
     ctx.set_density(np.full(N, p["density"]))
     ctx.set_bcs(p["bcs"])
     ctx.upload_state(v=p["v0"])
+    ntiles, halo_total = ctx.tiling()
     dt = p["dt"]
 
     # warmup: initial force evaluation + W steps
@@ -160,13 +173,15 @@ def main():
                 "cylinder": "expanding hollow cylinder 40h-50h x 200h, h=1mm, perturbed, sc 1.5e-3",
             }[args.workload],
             "nodes": N, "bonds": B, "lanes_per_node": args.lanes,
+            "kernels": "lds-tiled" if (args.variant == 0 and ntiles > 0) else "untiled",
+            "tiles": ntiles, "halo_per_node": halo_total / N,
             "bond_arithmetic": "fp32 compensated", "integration": "fp64",
             "broken_bonds_end": int(st["bond_damage"].sum()),
         },
         "roofline": {
             "bound": "hbm", "achieved": a2, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": a2 / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_pass2 (force + Verlet)", "kernel_avg_us": p2_avg * 1e6,
+            "kernel": "pass 2 (force + Verlet)", "kernel_avg_us": p2_avg * 1e6,
             "pass1_achieved": a1, "pass1_avg_us": p1_avg * 1e6,
             "step_frac": (bytes_pass1(B, N) + bytes_pass2(B, N)) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },

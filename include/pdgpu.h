@@ -56,6 +56,13 @@ const char* pdg_last_error(const pdg_ctx* ctx);
 /* Lanes per node used by the bond kernels (16, 32 or 64; default 16). */
 int pdg_set_lanes_per_node(pdg_ctx* ctx, int lanes);
 
+/* Bond-kernel variant: 0 (default) LDS-staged Morton tiles when every tile
+ * halo fits in LDS, else the untiled kernels; 1 forces the untiled kernels
+ * (one 64-byte neighbour-record gather per bond). */
+int pdg_set_kernel_variant(pdg_ctx* ctx, int variant);
+/* Number of Morton tiles (0: tiling unavailable) and total halo entries. */
+int pdg_get_tiling(const pdg_ctx* ctx, int* numTiles, long long* haloTotal);
+
 /* ---- topology and constants -------------------------------------------
  * pdg_set_topology: the block's NeighborhoodData (numOwnedPoints, ownedIDs,
  * neighborhoodList of listSize ints). All neighbour ids must be owned points
@@ -130,6 +137,32 @@ int pdg_set_per_pass_timing(pdg_ctx* ctx, int on);
 
 /* Count of intact bonds flagged near-critical during the last run. */
 int pdg_near_critical_count(const pdg_ctx* ctx, long long* count);
+
+/* ---- quasi-static (src/core/Peridigm.cpp:2470-3043, :4176-4317) --------
+ * FP64 throughout. Vectors are 3N, reference order. Kinematic (prescribed)
+ * DOFs are node*3+coord ids; their residual rows are zero and their tangent
+ * rows are identity. */
+#define PDG_PRECOND_NONE 0
+#define PDG_PRECOND_BLOCK_JACOBI 1 /* 3x3 diagonal blocks, :1603-1647 */
+#define PDG_PRECOND_LEARNED 2      /* message-passing network, flexible CG */
+int pdg_qs_set_fixed(pdg_ctx* ctx, const int* dofs, int numDofs);
+/* r = f(u) V with kinematic rows zeroed; *norm = ||r||_2 + 20 ||r||_inf (:4316). */
+int pdg_qs_residual(pdg_ctx* ctx, const double* u, double* residual, double* norm);
+/* K w = -V (df/du) w on free DOFs (forward-mode directional derivative of
+ * the force), w on kinematic DOFs. */
+int pdg_qs_tangent_apply(pdg_ctx* ctx, const double* u, const double* w, double* Kw);
+/* Inverses of the 3x3 diagonal blocks of K at u (9N, row-major per node). */
+int pdg_qs_block_diagonal(pdg_ctx* ctx, const double* u, double* blocksInv);
+/* Solve K(u) x = rhs (x: initial guess in, solution out) to ||r||/||rhs|| <=
+ * relTol with CG (block Jacobi) or flexible CG (learned preconditioner). */
+int pdg_qs_solve(pdg_ctx* ctx, const double* u, const double* rhs, double* x, int precond,
+                 double relTol, int maxIter, int* iters, double* relRes);
+/* Newton at fixed prescribed values (u holds them on kinematic DOFs): until
+ * ||r||_2 + 20||r||_inf <= tol * max(1, ||reactions||_2) (:2716-2746), each
+ * linear solve to linTol. */
+int pdg_qs_newton(pdg_ctx* ctx, double* u, double tol, int maxNewton, int precond,
+                  double linTol, int maxLin, int* newtonIters, int* linIters,
+                  double* residualNorm);
 
 /* ---- neighbour search (integer results, bit-exact predicates) ----------- */
 /* QuickGrid tensor-product lattice: node coordinates (3N), volumes (N) and

@@ -44,6 +44,8 @@ SIGNATURES = {
     "pdg_ctx_destroy": (_i, [_vp]),
     "pdg_last_error": (C.c_char_p, [_vp]),
     "pdg_set_lanes_per_node": (_i, [_vp, _i]),
+    "pdg_set_kernel_variant": (_i, [_vp, _i]),
+    "pdg_get_tiling": (_i, [_vp, C.POINTER(_i), C.POINTER(_ll)]),
     "pdg_set_topology": (_i, [_vp, _i, _vp, _vp, _ll]),
     "pdg_set_constants": (_i, [_vp, _vp, _vp, _d, _d, _d, _i, _d]),
     "pdg_set_density": (_i, [_vp, _vp]),
@@ -59,6 +61,13 @@ SIGNATURES = {
     "pdg_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
     "pdg_set_per_pass_timing": (_i, [_vp, _i]),
     "pdg_near_critical_count": (_i, [_vp, C.POINTER(_ll)]),
+    "pdg_qs_set_fixed": (_i, [_vp, _vp, _i]),
+    "pdg_qs_residual": (_i, [_vp, _vp, _vp, C.POINTER(_d)]),
+    "pdg_qs_tangent_apply": (_i, [_vp, _vp, _vp, _vp]),
+    "pdg_qs_block_diagonal": (_i, [_vp, _vp, _vp]),
+    "pdg_qs_solve": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, C.POINTER(_i), C.POINTER(_d)]),
+    "pdg_qs_newton": (_i, [_vp, _vp, _d, _i, _i, _d, _i, C.POINTER(_i), C.POINTER(_i),
+                           C.POINTER(_d)]),
     "pdg_quickgrid": (_i, [_i, _i, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, C.POINTER(_ll)]),
     "pdg_radius_search": (_i, [_i, _vp, _d, _i, _vp, _vp, C.POINTER(_ll)]),
 }

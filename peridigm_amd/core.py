@@ -32,7 +32,7 @@ def _p(a):
 class Context:
     """pdg_ctx on HIP device `device`."""
 
-    def __init__(self, device: int = 0, lanes: int = 16):
+    def __init__(self, device: int = 0, lanes: int = 16, variant: int = 0):
         self.lib = _lib.load()
         h = C.c_void_p()
         rc = self.lib.pdg_ctx_create(device, C.byref(h))
@@ -43,6 +43,7 @@ class Context:
         self.B = 0
         self._keep = []
         self.check(self.lib.pdg_set_lanes_per_node(self.h, lanes))
+        self.check(self.lib.pdg_set_kernel_variant(self.h, variant))
 
     # ------------------------------------------------------------ plumbing
     def check(self, rc):
@@ -145,6 +146,15 @@ class Context:
     def verlet_run(self, nsteps, dt, t0=0.0, initial_eval=True):
         self.check(self.lib.pdg_verlet_run(self.h, int(nsteps), float(dt), float(t0),
                                            int(bool(initial_eval))))
+
+    def set_kernel_variant(self, variant):
+        """0: LDS-staged Morton tiles (default), 1: untiled gathers."""
+        self.check(self.lib.pdg_set_kernel_variant(self.h, int(variant)))
+
+    def tiling(self):
+        n = C.c_int(); t = C.c_longlong()
+        self.check(self.lib.pdg_get_tiling(self.h, C.byref(n), C.byref(t)))
+        return n.value, t.value
 
     def set_per_pass_timing(self, on=True):
         self.check(self.lib.pdg_set_per_pass_timing(self.h, int(bool(on))))

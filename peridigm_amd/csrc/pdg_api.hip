@@ -59,16 +59,14 @@ int Ctx::hip(hipError_t e, const char* what) {
   return PDG_E_HIP;
 }
 
-void Ctx::free_extra() {
-  for (hipEvent_t e : tev) (void)hipEventDestroy(e);
-  tev.clear();
-}
-
 void Ctx::free_device() {
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(d_row); F(d_col); F(d_mw); F(d_mask); F(d_nc); F(d_rec[0]); F(d_rec[1]); F(d_theta);
   F(d_nbroken); F(d_v); F(d_a); F(d_f); F(d_rho); F(d_bcslot); F(d_bcs); F(d_err); F(d_near);
   F(d_perm); F(d_fout); F(d_y);
+  F(d_tile_start); F(d_halo_off); F(d_halo); F(d_own_off); F(d_lcol);
+  d_tile_start = d_halo_off = d_halo = d_own_off = nullptr;
+  d_lcol = nullptr;
   d_row = d_col = d_mw = d_nbroken = d_bcslot = d_err = d_perm = nullptr;
   d_mask = d_near = nullptr;
   d_nc = nullptr; d_rec[0] = d_rec[1] = nullptr; d_theta = nullptr;
@@ -76,11 +74,70 @@ void Ctx::free_device() {
   d_bcs = nullptr;
 }
 
+hipError_t Ctx::pass1(NodeRec* rec, int* nbroken, bool damage, bool count_near) {
+  if (tiled())
+    return launch_pass1_tiled(bond_args(), tile_args(), rec, d_theta, nbroken, damage, count_near,
+                              lanes, stream);
+  return launch_pass1(bond_args(), rec, d_theta, nbroken, damage, count_near, lanes, stream);
+}
+
 BondArgs Ctx::bond_args() const {
   BondArgs a;
   a.n = N; a.row = d_row; a.col = d_col; a.mw = d_mw; a.mask = d_mask; a.nc = d_nc;
   a.near_rel = 1e-6f; a.near_ctr = d_near; a.influence = infl;
   return a;
+}
+
+TileArgs Ctx::tile_args() const {
+  TileArgs t;
+  t.ntiles = ntiles; t.tile_start = d_tile_start; t.halo_off = d_halo_off; t.halo = d_halo;
+  t.own_off = d_own_off; t.lcol = d_lcol;
+  return t;
+}
+
+// Greedy Morton tiling: extend a tile node by node while its halo (own nodes
+// plus all their neighbours) stays within kTileHaloMax and it holds at most
+// kTileNodesMax nodes.
+int Ctx::build_tiles() {
+  ntiles = 0;
+  tile_start.assign(1, 0);
+  halo_off.assign(1, 0);
+  halo.clear();
+  own_off.clear();
+  lcol.assign(B, 0);
+  for (int i = 0; i < N; i++)
+    if (row[i + 1] - row[i] > 128) { ntiles = 0; tile_start.assign(1, 0); return 1; }
+  std::vector<int> mark(N, -1);
+  std::vector<int> H;
+  int n0 = 0;
+  while (n0 < N) {
+    const int tid = ntiles;
+    H.clear();
+    int i = n0;
+    while (i < N && i - n0 < kTileNodesMax) {
+      int add = mark[i] != tid ? 1 : 0;
+      for (int b = row[i]; b < row[i + 1]; b++) add += mark[col[b]] != tid ? 1 : 0;
+      if ((int)H.size() + add > kTileHaloMax) {
+        if (i == n0) { ntiles = 0; tile_start.assign(1, 0); return 1; }
+        break;
+      }
+      if (mark[i] != tid) { mark[i] = tid; H.push_back(i); }
+      for (int b = row[i]; b < row[i + 1]; b++)
+        if (mark[col[b]] != tid) { mark[col[b]] = tid; H.push_back(col[b]); }
+      i++;
+    }
+    std::sort(H.begin(), H.end());
+    own_off.push_back((int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin()));
+    for (int k = n0; k < i; k++)
+      for (int b = row[k]; b < row[k + 1]; b++)
+        lcol[b] = (unsigned short)(std::lower_bound(H.begin(), H.end(), col[b]) - H.begin());
+    halo.insert(halo.end(), H.begin(), H.end());
+    halo_off.push_back((int)halo.size());
+    tile_start.push_back(i);
+    ntiles++;
+    n0 = i;
+  }
+  return 0;
 }
 
 // Host list (reference order) -> device mask words (internal order).
@@ -181,6 +238,22 @@ int pdg_set_lanes_per_node(pdg_ctx* h, int lanes) {
   if (lanes != 16 && lanes != 32 && lanes != 64)
     return c->fail(PDG_E_ARG, "lanes per node must be 16, 32 or 64");
   c->lanes = lanes;
+  return PDG_OK;
+}
+
+int pdg_set_kernel_variant(pdg_ctx* h, int variant) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (variant != 0 && variant != 1) return c->fail(PDG_E_ARG, "variant must be 0 (tiled) or 1 (untiled)");
+  c->variant = variant;
+  return PDG_OK;
+}
+
+int pdg_get_tiling(const pdg_ctx* h, int* ntiles, long long* haloTotal) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c || !c->have_consts) return PDG_E_ARG;
+  if (ntiles) *ntiles = c->ntiles;
+  if (haloTotal) *haloTotal = (long long)c->halo.size();
   return PDG_OK;
 }
 
@@ -302,6 +375,8 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   c->mw.assign(N + 1, 0);
   for (int k = 0; k < N; k++) c->mw[k + 1] = c->mw[k] + (c->row[k + 1] - c->row[k] + 63) / 64;
 
+  c->build_tiles();
+
   // Scales: L and S powers of two near the horizon and the mean volume.
   double vsum = 0.0;
   for (int i = 0; i < N; i++) vsum += vol[i];
@@ -334,6 +409,13 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if ((rc = A((void**)&c->d_err, 4, "alloc err"))) return rc;
   if ((rc = A((void**)&c->d_near, 8, "alloc near"))) return rc;
   if ((rc = A((void**)&c->d_perm, (size_t)N * 4, "alloc perm"))) return rc;
+  if (c->ntiles > 0) {
+    if ((rc = A((void**)&c->d_tile_start, 4 * c->tile_start.size(), "alloc tiles"))) return rc;
+    if ((rc = A((void**)&c->d_halo_off, 4 * c->halo_off.size(), "alloc halo off"))) return rc;
+    if ((rc = A((void**)&c->d_halo, 4 * c->halo.size(), "alloc halo"))) return rc;
+    if ((rc = A((void**)&c->d_own_off, 4 * c->own_off.size(), "alloc own off"))) return rc;
+    if ((rc = A((void**)&c->d_lcol, 2 * (size_t)c->B, "alloc lcol"))) return rc;
+  }
 
   hipStream_t s = c->stream;
   // Weighted volume on the device in the reference's summation order.
@@ -403,6 +485,13 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_perm, c->perm.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d perm");
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
+  if (!rc && c->ntiles > 0) {
+    rc = c->hip(hipMemcpyAsync(c->d_tile_start, c->tile_start.data(), 4 * c->tile_start.size(), hipMemcpyHostToDevice, s), "h2d tiles");
+    if (!rc) rc = c->hip(hipMemcpyAsync(c->d_halo_off, c->halo_off.data(), 4 * c->halo_off.size(), hipMemcpyHostToDevice, s), "h2d halo off");
+    if (!rc) rc = c->hip(hipMemcpyAsync(c->d_halo, c->halo.data(), 4 * c->halo.size(), hipMemcpyHostToDevice, s), "h2d halo");
+    if (!rc) rc = c->hip(hipMemcpyAsync(c->d_own_off, c->own_off.data(), 4 * c->own_off.size(), hipMemcpyHostToDevice, s), "h2d own off");
+    if (!rc) rc = c->hip(hipMemcpyAsync(c->d_lcol, c->lcol.data(), 2 * (size_t)c->B, hipMemcpyHostToDevice, s), "h2d lcol");
+  }
   if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
   if (rc) return rc;
   c->cur = 0;
@@ -444,8 +533,7 @@ int pdg_compute_damage(pdg_ctx* h, const double* y, const double* bdN, double* b
                        c->d_rec[c->cur], c->L);
   if (!rc) rc = c->upload_mask(bdN, false);
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
-  if (!rc) rc = c->hip(launch_pass1(c->bond_args(), c->d_rec[c->cur], c->d_theta, c->d_nbroken,
-                                    true, nearCritical != nullptr, c->lanes, s), "pass1");
+  if (!rc) rc = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, true, nearCritical != nullptr), "pass1");
   if (!rc) rc = c->download_mask(bdNP1);
   if (rc) return rc;
   std::vector<int> nb(N);
@@ -477,9 +565,9 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
     hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y, c->d_perm,
                        c->d_rec[c->cur], c->L);
   if (!rc) rc = c->upload_mask(bdNP1, true);
-  if (!rc) rc = c->hip(launch_pass1(c->bond_args(), c->d_rec[c->cur], c->d_theta, nullptr, false,
-                                    false, c->lanes, s), "pass1");
-  if (!rc) rc = c->hip(launch_pass2_force(c->bond_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s),
+  if (!rc) rc = c->hip(c->pass1(c->d_rec[c->cur], nullptr, false, false), "pass1");
+  if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s)
+                                  : launch_pass2_force(c->bond_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s),
                        "pass2");
   if (!rc) rc = c->download_vec3(c->d_fout, force);
   if (rc) return rc;
@@ -584,15 +672,15 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   auto eval = [&](int k, bool end_kick) -> int {
     int r = 0;
     if (nev) r = c->hip(hipEventRecord(c->tev[3 * nevals], s), "event");
-    if (!r) r = c->hip(launch_pass1(ba, c->d_rec[c->cur], c->d_theta, c->d_nbroken, damage, false,
-                                    c->lanes, s), "pass1");
+    if (!r) r = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, damage, false), "pass1");
     if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 1], s), "event");
     va.end_kick = end_kick ? 1 : 0;
     va.start_step = k < nsteps ? 1 : 0;
     va.t_cur = step_time(k + 1);
     va.t_prev = step_time(k);
-    if (!r) r = c->hip(launch_pass2_verlet(ba, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va,
-                                           c->lanes, s), "pass2");
+    if (!r) r = c->hip(c->tiled() ? launch_pass2_tiled_verlet(ba, c->tile_args(), c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, s)
+                                  : launch_pass2_verlet(ba, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, s),
+                       "pass2");
     if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 2], s), "event");
     nevals++;
     c->cur ^= 1;

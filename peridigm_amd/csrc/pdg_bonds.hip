@@ -26,85 +26,13 @@
 // t_ij = w [theta_i (3K/m_i - alpha_i/3) zeta + alpha_i e] (1-d_ij); each
 // directed bond is evaluated once from each end.
 
-#include "pdg_internal.hpp"
+#include "pdg_device.hpp"
 
 namespace pdg {
 
 namespace {
 
-__device__ __forceinline__ float fsqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
-__device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
-__device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
-
-// Influence function in units of L (src/core/Peridigm_InfluenceFunction.hpp:69-88).
-__device__ __forceinline__ float omega_hat(int kind, float zeta, float hhat) {
-  if (kind == PDG_INFLUENCE_ONE) return 1.0f;
-  if (kind == PDG_INFLUENCE_PARABOLIC_DECAY) {
-    if (zeta > hhat) return 0.0f;
-    float s = zeta / hhat;
-    return s < 0.5f ? 1.0f : -4.0f * s * s + 4.0f * s;
-  }
-  float h2 = hhat * hhat * 0.16f;
-  return expf(-(zeta * zeta) / h2);
-}
-
-template <int G>
-__device__ __forceinline__ float group_sum(float v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-template <int G>
-__device__ __forceinline__ int group_sum_i(int v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
-}
-
-struct BondGeom {
-  float x0, x1, x2;  // xi (units of L)
-  float y0, y1, y2;  // Y = xi + eta
-  float zeta, ext;   // |xi|, |Y| - |xi|
-  float ry;          // 1/|Y|
-};
-
-// Compensated bond geometry from FP64 records.
-template <bool NEED_RY>
-__device__ __forceinline__ BondGeom bond_geom(const double* xi, const double* ui,
-                                              const NodeRec& r) {
-  BondGeom b;
-  b.x0 = (float)(r.x[0] - xi[0]);
-  b.x1 = (float)(r.x[1] - xi[1]);
-  b.x2 = (float)(r.x[2] - xi[2]);
-  const float e0 = (float)(r.u[0] - ui[0]);
-  const float e1 = (float)(r.u[1] - ui[1]);
-  const float e2 = (float)(r.u[2] - ui[2]);
-  b.y0 = b.x0 + e0;
-  b.y1 = b.x1 + e1;
-  b.y2 = b.x2 + e2;
-  const float z2 = fmaf(b.x2, b.x2, fmaf(b.x1, b.x1, b.x0 * b.x0));
-  const float yy = fmaf(b.y2, b.y2, fmaf(b.y1, b.y1, b.y0 * b.y0));
-  // |Y|^2 - |xi|^2 = eta . (xi + Y), free of cancellation
-  const float dot = fmaf(e2, b.x2 + b.y2, fmaf(e1, b.x1 + b.y1, e0 * (b.x0 + b.y0)));
-  b.zeta = fsqrt(z2);
-  float ylen;
-  if (NEED_RY) {
-    b.ry = frsq(yy);
-    ylen = yy * b.ry;
-  } else {
-    b.ry = 0.0f;
-    ylen = fsqrt(yy);
-  }
-  b.ext = dot * frcp(ylen + b.zeta);
-  return b;
-}
+using namespace dev;
 
 // ---------------------------------------------------------------------------
 // pass 1: damage + dilatation
@@ -146,11 +74,13 @@ __global__ __launch_bounds__(256) void k_pass1(BondArgs a, NodeRec* __restrict__
       const NodeRec r = rec[j];
       wold = a.mask[w0 + (k >> 6)];
       brk = ((wold >> (k & 63)) & 1ull) != 0ull;
-      const BondGeom b = bond_geom<false>(xi, ui, r);
+      const BondGeom b = bond_geom<false>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.x[0],
+                                           r.x[1], r.x[2], r.u[0], r.u[1], r.u[2]);
       if (DAMAGE) {
-        const float s = b.ext * frcp(b.zeta);
-        if (NEAR && !brk && fabsf(s - c.sc) <= a.near_rel * c.sc) nnear++;
-        brk = brk || (s > c.sc);
+        // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0)
+        const float lim = c.sc * b.zeta;
+        if (NEAR && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
+        brk = brk || (b.ext > lim);
       }
       const float w = omega_hat(a.influence, b.zeta, c.hhat);
       if (!brk) acc = fmaf(w * b.zeta * b.ext, r.vol, acc);
@@ -214,7 +144,8 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
     const NodeRec r = rec[j];
     const unsigned long long word = a.mask[w0 + (k >> 6)];
     const bool brk = ((word >> (k & 63)) & 1ull) != 0ull;
-    const BondGeom b = bond_geom<true>(xi, ui, r);
+    const BondGeom b = bond_geom<true>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.x[0], r.x[1],
+                                       r.x[2], r.u[0], r.u[1], r.u[2]);
     const float w = omega_hat(a.influence, b.zeta, hhat);
     float t = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
     t = brk ? 0.0f : w * t * r.vol * b.ry;
@@ -227,38 +158,11 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
   f2 = group_sum<G>(f2);
   if (!valid || l >= 3) return;
   const float fc = l == 0 ? f0 : (l == 1 ? f1 : f2);
-  const int q = 3 * node + l;
-  const double f = (double)fc;
   if (!VERLET) {
-    fout[q] = f;
+    fout[3 * node + l] = (double)fc;
     return;
   }
-  // Velocity-Verlet in FP64 (src/core/Peridigm.cpp:1480-1501, :1569-1577).
-  if (!isfinite(f)) *va.err = 1;
-  double acc = f;
-  acc += 0.0;  // external force (none)
-  acc /= va.rho[node];
-  double v = va.v[q];
-  double u = rec[node].u[l] * va.L;
-  const double dt2 = va.dt / 2.0;
-  if (va.end_kick) v = v + dt2 * acc;
-  if (va.start_step) {
-    v = v + dt2 * acc;
-    const int slot = va.bc_slot[node];
-    if (slot >= 0 && ((va.bcs[slot].mask >> l) & 1)) {
-      const double c0 = va.bcs[slot].c0[l], c1 = va.bcs[slot].c1[l];
-      const double cur = c1 * va.t_cur + c0;
-      double prev = c1 * va.t_prev + c0;
-      if (cur - prev == 0.0) prev = u;
-      v = 0.0 * (cur - prev) + 1.0 * (cur - prev) * (1.0 / (va.t_cur - va.t_prev));
-    }
-    u = u + va.dt * v;
-  } else {
-    va.a[q] = acc;
-    va.f[q] = f;
-  }
-  va.v[q] = v;
-  rec_next[node].u[l] = u * (1.0 / va.L);
+  verlet_epilogue(va, node, l, fc, rec[node].u[l], rec_next);
 }
 
 // Start of a step from a stored acceleration (continuing runs).

@@ -63,18 +63,40 @@ struct Ctx {
   unsigned long long* d_near = nullptr;
   int* d_perm = nullptr;
 
+  // LDS-staged Morton tiles (empty if a node's neighbourhood exceeds the
+  // halo capacity; the untiled kernels are used then)
+  int variant = 0;  // 0: tiled when available, 1: untiled
+  int ntiles = 0;
+  std::vector<int> tile_start, halo_off, halo, own_off;
+  std::vector<unsigned short> lcol;
+  int* d_tile_start = nullptr;
+  int* d_halo_off = nullptr;
+  int* d_halo = nullptr;
+  int* d_own_off = nullptr;
+  unsigned short* d_lcol = nullptr;
+  bool tiled() const { return variant == 0 && ntiles > 0; }
+  TileArgs tile_args() const;
+  int build_tiles();
+
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
 
+  void free_gnn();
   int fail(int code, const std::string& msg);
   int hip(hipError_t e, const char* what);
   void free_device();
   void free_extra();
   BondArgs bond_args() const;
+  hipError_t pass1(NodeRec* rec, int* nbroken, bool damage, bool count_near);
   int upload_mask(const double* bd, bool check_symmetric);
   int download_mask(double* bd);
   int download_vec3(const double* dsrc, double* out);
   int upload_vec3(const double* in, double* ddst);
 };
+
+// quasi-static solver (pdg_qs.hip) and learned preconditioner (pdg_gnn.hip)
+int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int maxit, int* iters,
+             double* rel);
+int gnn_apply(Ctx* c, const double* r, double* z);
 
 }  // namespace pdg

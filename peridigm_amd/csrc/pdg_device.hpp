@@ -1,0 +1,123 @@
+// Device helpers shared by the bond-pass kernels.
+#pragma once
+
+#include "pdg_internal.hpp"
+
+namespace pdg {
+namespace dev {
+
+__device__ __forceinline__ float fsqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
+
+// Influence function in units of L (src/core/Peridigm_InfluenceFunction.hpp:69-88).
+__device__ __forceinline__ float omega_hat(int kind, float zeta, float hhat) {
+  if (kind == PDG_INFLUENCE_ONE) return 1.0f;
+  if (kind == PDG_INFLUENCE_PARABOLIC_DECAY) {
+    if (zeta > hhat) return 0.0f;
+    float s = zeta / hhat;
+    return s < 0.5f ? 1.0f : -4.0f * s * s + 4.0f * s;
+  }
+  float h2 = hhat * hhat * 0.16f;
+  return expf(-(zeta * zeta) / h2);
+}
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_sum_i(int v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+struct BondGeom {
+  float x0, x1, x2;  // xi (units of L)
+  float y0, y1, y2;  // Y = xi + eta
+  float zeta, ext;   // |xi|, |Y| - |xi|
+  float ry;          // 1/|Y| (when requested)
+};
+
+// Compensated bond geometry from FP64 reference positions and displacements:
+// xi and eta are formed in FP64 and rounded once; the extension uses
+// e = eta . (xi + Y) / (|Y| + |xi|), which has no cancellation.
+template <bool NEED_RY>
+__device__ __forceinline__ BondGeom bond_geom(double xi0, double xi1, double xi2, double ui0,
+                                              double ui1, double ui2, double xj0, double xj1,
+                                              double xj2, double uj0, double uj1, double uj2) {
+  BondGeom b;
+  b.x0 = (float)(xj0 - xi0);
+  b.x1 = (float)(xj1 - xi1);
+  b.x2 = (float)(xj2 - xi2);
+  const float e0 = (float)(uj0 - ui0);
+  const float e1 = (float)(uj1 - ui1);
+  const float e2 = (float)(uj2 - ui2);
+  b.y0 = b.x0 + e0;
+  b.y1 = b.x1 + e1;
+  b.y2 = b.x2 + e2;
+  const float z2 = fmaf(b.x2, b.x2, fmaf(b.x1, b.x1, b.x0 * b.x0));
+  const float yy = fmaf(b.y2, b.y2, fmaf(b.y1, b.y1, b.y0 * b.y0));
+  const float dot = fmaf(e2, b.x2 + b.y2, fmaf(e1, b.x1 + b.y1, e0 * (b.x0 + b.y0)));
+  b.zeta = fsqrt(z2);
+  float ylen;
+  if (NEED_RY) {
+    b.ry = frsq(yy);
+    ylen = yy * b.ry;
+  } else {
+    b.ry = 0.0f;
+    ylen = fsqrt(yy);
+  }
+  b.ext = dot * frcp(ylen + b.zeta);
+  return b;
+}
+
+// Velocity-Verlet epilogue for one component l of node `node` (FP64,
+// src/core/Peridigm.cpp:1480-1501, :1569-1577): finishes the current step
+// (v += dt/2 a) and, unless this is the last force evaluation, starts the next
+// one (v += dt/2 a, prescribed-displacement velocity, u += dt v). u_scaled is
+// the node's displacement / L from the current record; the new one goes to
+// rec_next.
+__device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, int l, float fc,
+                                                double u_scaled, NodeRec* __restrict__ rec_next) {
+  const int q = 3 * node + l;
+  const double f = (double)fc;
+  if (!isfinite(f)) *va.err = 1;
+  double acc = f;
+  acc += 0.0;  // external force (none)
+  acc /= va.rho[node];
+  double v = va.v[q];
+  double u = u_scaled * va.L;
+  const double dt2 = va.dt / 2.0;
+  if (va.end_kick) v = v + dt2 * acc;
+  if (va.start_step) {
+    v = v + dt2 * acc;
+    const int slot = va.bc_slot[node];
+    if (slot >= 0 && ((va.bcs[slot].mask >> l) & 1)) {
+      const double c0 = va.bcs[slot].c0[l], c1 = va.bcs[slot].c1[l];
+      const double cur = c1 * va.t_cur + c0;
+      double prev = c1 * va.t_prev + c0;
+      if (cur - prev == 0.0) prev = u;
+      v = 0.0 * (cur - prev) + 1.0 * (cur - prev) * (1.0 / (va.t_cur - va.t_prev));
+    }
+    u = u + va.dt * v;
+  } else {
+    va.a[q] = acc;
+    va.f[q] = f;
+  }
+  va.v[q] = v;
+  rec_next[node].u[l] = u * (1.0 / va.L);
+}
+
+}  // namespace dev
+}  // namespace pdg

@@ -73,6 +73,31 @@ struct VerletArgs {
   int* err;           // set to 1 on a non-finite force
 };
 
+// Morton tiles of the LDS-staged kernels: a tile is a contiguous range of
+// internal nodes [tile_start[t], tile_start[t+1]); its halo (sorted internal
+// ids, own nodes included) is staged in LDS; lcol[b] is the position of bond
+// b's neighbour inside its tile's halo. Tiling needs every node degree <= 128.
+constexpr int kTileHaloMax = 1024;
+constexpr int kTileNodesMax = 128;
+
+struct TileArgs {
+  int ntiles;
+  const int* tile_start;      // ntiles + 1
+  const int* halo_off;        // ntiles + 1
+  const int* halo;            // halo node ids
+  const int* own_off;         // position of tile_start[t] in its halo
+  const unsigned short* lcol; // B
+};
+
+hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta,
+                              int* nbroken, bool damage, bool count_near, int lanes,
+                              hipStream_t s);
+hipError_t launch_pass2_tiled_force(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
+                                    double* f, int lanes, hipStream_t s);
+hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
+                                     NodeRec* rec_next, const VerletArgs& va, int lanes,
+                                     hipStream_t s);
+
 // Launchers (pdg_bonds.hip). `lanes` = lanes per node (16, 32 or 64).
 hipError_t launch_pass1(const BondArgs& a, NodeRec* rec, float* theta, int* nbroken,
                         bool damage, bool count_near, int lanes, hipStream_t s);

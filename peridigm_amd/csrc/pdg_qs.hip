@@ -1,0 +1,675 @@
+// Quasi-static path: FP64 residual, matrix-free tangent apply and 3x3
+// block-Jacobi preconditioner of the LPS model, device PCG / flexible CG, and
+// a Newton driver over load steps.
+//
+// Reference: computeQuasiStaticResidual (src/core/Peridigm.cpp:4176-4317):
+// r = (f(u) + f_ext) V with the rows of kinematic DOFs zeroed; the Newton
+// tangent (Sacado AD over the same templated force kernel,
+// src/materials/Peridigm_ElasticMaterial.cpp:258-383); the 3x3 block
+// preconditioner of the NOX path (src/core/Peridigm.cpp:1603-1647).
+//
+// Here the tangent is never assembled. K w = -V (df/du) w is the forward-mode
+// directional derivative of the both-ends gather force, evaluated with dual
+// numbers in FP64 in the same two bond passes as the force (dilatation first,
+// then pair forces); prescribed DOFs get identity rows. The diagonal 3x3
+// blocks of K are formed analytically from the same bond quantities.
+//
+// Dual-number derivatives of the LPS force, for bond i->j with n = Y/|Y|,
+// dY = w_j - w_i, de = n.dY:
+//   d theta_i = 3/m_i sum_j w (1-d) zeta de V_j
+//   T = t_ij + t_ji,  dT = w (1-d) [(dtheta_i b_i + dtheta_j b_j) zeta + (a_i + a_j) de]
+//   df_i = sum_j [dT n + T (dY - n (n.dY)) / |Y|] V_j
+// with a = 15 mu / m, b = 3K/m - a/3.
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pdg_ctx.hpp"
+#include "pdg_device.hpp"
+
+namespace pdg {
+
+struct QsState {
+  int n = 0;
+  double *x = nullptr, *u = nullptr, *w = nullptr, *vol = nullptr, *m = nullptr, *alpha = nullptr,
+         *beta = nullptr, *theta = nullptr, *dtheta = nullptr, *g = nullptr, *out = nullptr,
+         *dinv = nullptr;
+  signed char* fixed = nullptr;  // 3n, 1 = prescribed DOF
+  // solver vectors (internal order, 3n)
+  double *xs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *b = nullptr,
+         *part = nullptr, *scal = nullptr;
+  int nblk = 0;
+  std::vector<signed char> hfixed;
+};
+
+namespace {
+
+using namespace dev;
+
+constexpr int QG = 16;  // lanes per node
+constexpr int RB = 256; // reduction block
+
+__device__ __forceinline__ double infl64(int kind, double zeta, double horizon) {
+  if (kind == PDG_INFLUENCE_ONE) return 1.0;
+  if (kind == PDG_INFLUENCE_PARABOLIC_DECAY) {
+    if (zeta > horizon) return 0.0;
+    double s = zeta / horizon;
+    return s < 0.5 ? 1.0 : -4.0 * s * s + 4.0 * s;
+  }
+  double h2 = horizon * horizon * 0.4 * 0.4;
+  return exp(-(zeta * zeta) / h2);
+}
+
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+struct QsArgs {
+  int n;
+  const int* row;
+  const int* col;
+  const int* mw;
+  const unsigned long long* mask;
+  const double* x;
+  const double* u;
+  const double* w;
+  const double* vol;
+  const double* m;
+  const double* alpha;
+  const double* beta;
+  double* theta;
+  double* dtheta;
+  double* g;
+  const signed char* fixed;
+  double* out;
+  double* dinv;
+  double horizon;
+  int influence;
+};
+
+// dilatation (+ its directional derivative, + d theta_i / d u_i)
+template <bool DUAL, bool DIAG>
+__global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int gi = lane / QG, l = lane & (QG - 1);
+  const int node = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / QG) + gi;
+  const bool valid = node < a.n;
+  double th = 0.0, dth = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (valid) {
+    const int rs = a.row[node], deg = a.row[node + 1] - rs, w0 = a.mw[node];
+    const double* xi = a.x + 3 * node;
+    const double* ui = a.u + 3 * node;
+    for (int k = l; k < deg; k += QG) {
+      if ((a.mask[w0 + (k >> 6)] >> (k & 63)) & 1ull) continue;
+      const int j = a.col[rs + k];
+      const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
+      const double Y0 = X0 + (a.u[3 * j] - ui[0]);
+      const double Y1 = X1 + (a.u[3 * j + 1] - ui[1]);
+      const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
+      const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+      const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
+      const double e = yl - zeta;
+      const double c = infl64(a.influence, zeta, a.horizon) * zeta * a.vol[j];
+      th += c * e;
+      const double n0 = Y0 / yl, n1 = Y1 / yl, n2 = Y2 / yl;
+      if (DUAL) {
+        const double* wi = a.w + 3 * node;
+        const double de = n0 * (a.w[3 * j] - wi[0]) + n1 * (a.w[3 * j + 1] - wi[1]) +
+                          n2 * (a.w[3 * j + 2] - wi[2]);
+        dth += c * de;
+      }
+      if (DIAG) { g0 -= c * n0; g1 -= c * n1; g2 -= c * n2; }
+    }
+  }
+  th = gsum<QG>(th);
+  if (DUAL) dth = gsum<QG>(dth);
+  if (DIAG) { g0 = gsum<QG>(g0); g1 = gsum<QG>(g1); g2 = gsum<QG>(g2); }
+  if (valid && l == 0) {
+    const double s = a.m[node] > 0.0 ? 3.0 / a.m[node] : 0.0;
+    a.theta[node] = s * th;
+    if (DUAL) a.dtheta[node] = s * dth;
+    if (DIAG) { a.g[3 * node] = s * g0; a.g[3 * node + 1] = s * g1; a.g[3 * node + 2] = s * g2; }
+  }
+}
+
+// MODE 0: residual r = f V (0 on fixed DOFs); MODE 1: K w = -V df (w on fixed)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int gi = lane / QG, l = lane & (QG - 1);
+  const int node = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / QG) + gi;
+  const bool valid = node < a.n;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  if (valid) {
+    const int rs = a.row[node], deg = a.row[node + 1] - rs, w0 = a.mw[node];
+    const double* xi = a.x + 3 * node;
+    const double* ui = a.u + 3 * node;
+    const double ai = a.alpha[node], bi = a.beta[node], thi = a.theta[node];
+    const double dthi = MODE == 1 ? a.dtheta[node] : 0.0;
+    for (int k = l; k < deg; k += QG) {
+      if ((a.mask[w0 + (k >> 6)] >> (k & 63)) & 1ull) continue;
+      const int j = a.col[rs + k];
+      const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
+      const double Y0 = X0 + (a.u[3 * j] - ui[0]);
+      const double Y1 = X1 + (a.u[3 * j + 1] - ui[1]);
+      const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
+      const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+      const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
+      const double e = yl - zeta;
+      const double om = infl64(a.influence, zeta, a.horizon);
+      const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
+      const double n0 = Y0 / yl, n1 = Y1 / yl, n2 = Y2 / yl;
+      const double vj = a.vol[j];
+      if (MODE == 0) {
+        f0 += T * n0 * vj; f1 += T * n1 * vj; f2 += T * n2 * vj;
+      } else {
+        const double* wi = a.w + 3 * node;
+        const double d0 = a.w[3 * j] - wi[0], d1 = a.w[3 * j + 1] - wi[1], d2 = a.w[3 * j + 2] - wi[2];
+        const double de = n0 * d0 + n1 * d1 + n2 * d2;
+        const double dT = om * ((dthi * bi + a.dtheta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * de);
+        const double s = T / yl;
+        f0 += (dT * n0 + s * (d0 - n0 * de)) * vj;
+        f1 += (dT * n1 + s * (d1 - n1 * de)) * vj;
+        f2 += (dT * n2 + s * (d2 - n2 * de)) * vj;
+      }
+    }
+  }
+  f0 = gsum<QG>(f0); f1 = gsum<QG>(f1); f2 = gsum<QG>(f2);
+  if (!valid || l >= 3) return;
+  const int q = 3 * node + l;
+  const double f = l == 0 ? f0 : (l == 1 ? f1 : f2);
+  const double vi = a.vol[node];
+  if (MODE == 0) a.out[q] = a.fixed[q] ? 0.0 : f * vi;
+  else a.out[q] = a.fixed[q] ? a.w[q] : -f * vi;
+}
+
+// Inverse of the 3x3 diagonal block of K at each node.
+__global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int gi = lane / QG, l = lane & (QG - 1);
+  const int node = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / QG) + gi;
+  const bool valid = node < a.n;
+  double D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (valid) {
+    const int rs = a.row[node], deg = a.row[node + 1] - rs, w0 = a.mw[node];
+    const double* xi = a.x + 3 * node;
+    const double* ui = a.u + 3 * node;
+    const double ai = a.alpha[node], bi = a.beta[node], thi = a.theta[node];
+    const double gg[3] = {a.g[3 * node], a.g[3 * node + 1], a.g[3 * node + 2]};
+    const double vi = a.vol[node];
+    for (int k = l; k < deg; k += QG) {
+      if ((a.mask[w0 + (k >> 6)] >> (k & 63)) & 1ull) continue;
+      const int j = a.col[rs + k];
+      const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
+      const double Y0 = X0 + (a.u[3 * j] - ui[0]);
+      const double Y1 = X1 + (a.u[3 * j + 1] - ui[1]);
+      const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
+      const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+      const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
+      const double e = yl - zeta;
+      const double om = infl64(a.influence, zeta, a.horizon);
+      const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
+      const double n[3] = {Y0 / yl, Y1 / yl, Y2 / yl};
+      const double vj = a.vol[j];
+      // grad_{u_i}(t_ij + t_ji) = om [b_i zeta g_i - a_i n - b_j zeta^2 om V_i (3/m_j) n - a_j n]
+      const double cj = a.m[j] > 0.0 ? 3.0 / a.m[j] : 0.0;
+      const double sn = -ai - a.alpha[j] - a.beta[j] * zeta * zeta * om * vi * cj;
+      double gt[3];
+      for (int c = 0; c < 3; c++) gt[c] = om * (bi * zeta * gg[c] + sn * n[c]);
+      const double s = T / yl;
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+          D[3 * r + c] += vj * (n[r] * gt[c] - s * ((r == c ? 1.0 : 0.0) - n[r] * n[c]));
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 9; t++) D[t] = gsum<QG>(D[t]);
+  if (!valid || l != 0) return;
+  const double vi = a.vol[node];
+  double K[9];
+  for (int t = 0; t < 9; t++) K[t] = -vi * D[t];
+  for (int c = 0; c < 3; c++)
+    if (a.fixed[3 * node + c])
+      for (int t = 0; t < 3; t++) { K[3 * c + t] = c == t ? 1.0 : 0.0; K[3 * t + c] = c == t ? 1.0 : 0.0; }
+  // closed-form inverse by cofactors
+  const double c00 = K[4] * K[8] - K[5] * K[7], c01 = K[5] * K[6] - K[3] * K[8], c02 = K[3] * K[7] - K[4] * K[6];
+  const double det = K[0] * c00 + K[1] * c01 + K[2] * c02;
+  double* Di = a.dinv + 9 * node;
+  if (!(fabs(det) > 0.0)) {
+    for (int t = 0; t < 9; t++) Di[t] = (t % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  const double id = 1.0 / det;
+  Di[0] = c00 * id; Di[1] = (K[2] * K[7] - K[1] * K[8]) * id; Di[2] = (K[1] * K[5] - K[2] * K[4]) * id;
+  Di[3] = c01 * id; Di[4] = (K[0] * K[8] - K[2] * K[6]) * id; Di[5] = (K[2] * K[3] - K[0] * K[5]) * id;
+  Di[6] = c02 * id; Di[7] = (K[1] * K[6] - K[0] * K[7]) * id; Di[8] = (K[0] * K[4] - K[1] * K[3]) * id;
+}
+
+// ---- vector kernels (deterministic block reductions, no atomics) ----------
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += sh[k];
+  return t;
+}
+
+__global__ __launch_bounds__(RB) void k_dot(int n, const double* a, const double* b, double* part) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) s += a[i] * b[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RB) void k_final(int nb, const double* part, int stride, double* out) {
+  __shared__ double sh[RB / 64];
+  for (int c = 0; c < stride; c++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) s += part[c * nb + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[c] = s;
+  }
+}
+
+// z = Dinv r (block) on 3-vectors of nodes
+__global__ __launch_bounds__(RB) void k_block_apply(int n, const double* dinv, const double* r, double* z) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const double* D = dinv + 9 * i;
+  const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+  z[3 * i] = D[0] * r0 + D[1] * r1 + D[2] * r2;
+  z[3 * i + 1] = D[3] * r0 + D[4] * r1 + D[5] * r2;
+  z[3 * i + 2] = D[6] * r0 + D[7] * r1 + D[8] * r2;
+}
+
+// x += al p; r -= al q; partial sums of r.r
+__global__ __launch_bounds__(RB) void k_cg_update(int n, double al, const double* p, const double* q,
+                                                  double* x, double* r, double* part) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RB) void k_axpby(int n, double a, const double* x, double b, double* y) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+__global__ __launch_bounds__(RB) void k_sub(int n, const double* a, const double* b, double* c) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) c[i] = a[i] - b[i];
+}
+
+inline unsigned qblocks(int n) { return (unsigned)((n + 4 * (64 / QG) - 1) / (4 * (64 / QG))); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace qs {
+
+static QsArgs args(Ctx* c) {
+  QsState* S = c->qs;
+  QsArgs a;
+  a.n = c->N; a.row = c->d_row; a.col = c->d_col; a.mw = c->d_mw; a.mask = c->d_mask;
+  a.x = S->x; a.u = S->u; a.w = S->w; a.vol = S->vol; a.m = S->m; a.alpha = S->alpha;
+  a.beta = S->beta; a.theta = S->theta; a.dtheta = S->dtheta; a.g = S->g; a.fixed = S->fixed;
+  a.out = S->out; a.dinv = S->dinv; a.horizon = c->horizon; a.influence = c->infl;
+  return a;
+}
+
+static int ensure(Ctx* c) {
+  if (c->qs && c->qs->n == c->N) return PDG_OK;
+  if (c->qs) c->free_extra();
+  QsState* S = new QsState();
+  c->qs = S;
+  const int N = c->N;
+  S->n = N;
+  S->nblk = std::min(1024, std::max(1, (3 * N + RB - 1) / RB));
+  auto A = [&](void** p, size_t bytes) { return c->hip(hipMalloc(p, std::max<size_t>(bytes, 64)), "qs alloc"); };
+  int rc = 0;
+  double** vecs3[] = {&S->x, &S->u, &S->w, &S->g, &S->out, &S->xs, &S->r, &S->z, &S->p, &S->q, &S->b};
+  for (double** v : vecs3) if (!rc) rc = A((void**)v, 24 * (size_t)N);
+  double** vecs1[] = {&S->vol, &S->m, &S->alpha, &S->beta, &S->theta, &S->dtheta};
+  for (double** v : vecs1) if (!rc) rc = A((void**)v, 8 * (size_t)N);
+  if (!rc) rc = A((void**)&S->dinv, 72 * (size_t)N);
+  if (!rc) rc = A((void**)&S->part, 8 * (size_t)S->nblk * 2);
+  if (!rc) rc = A((void**)&S->scal, 64);
+  if (!rc) rc = A((void**)&S->fixed, 3 * (size_t)N);
+  if (rc) return rc;
+  // constants in internal order
+  std::vector<double> hx(3 * (size_t)N), hv(N), hm(N), ha(N), hb(N);
+  for (int k = 0; k < N; k++) {
+    const int p = c->perm[k];
+    for (int d = 0; d < 3; d++) hx[3 * (size_t)k + d] = c->x[3 * (size_t)p + d];
+    hv[k] = c->vol[p];
+    hm[k] = c->m[p];
+    if (hm[k] > 0.0) {
+      ha[k] = 15.0 * c->mu / hm[k];
+      hb[k] = 3.0 * c->K / hm[k] - ha[k] / 3.0;
+    } else {
+      ha[k] = hb[k] = 0.0;
+    }
+  }
+  hipStream_t s = c->stream;
+  rc = c->hip(hipMemcpyAsync(S->x, hx.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->vol, hv.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->m, hm.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->alpha, ha.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->beta, hb.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  S->hfixed.assign(3 * (size_t)N, 0);
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), 3 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  return rc;
+}
+
+// reference-order 3N host vector -> internal-order device vector
+static int to_dev(Ctx* c, const double* h, double* d) {
+  std::vector<double> t(3 * (size_t)c->N);
+  for (int k = 0; k < c->N; k++)
+    for (int q = 0; q < 3; q++) t[3 * (size_t)k + q] = h[3 * (size_t)c->perm[k] + q];
+  int rc = c->hip(hipMemcpyAsync(d, t.data(), t.size() * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
+static int to_host(Ctx* c, const double* d, double* h) { return c->download_vec3(d, h); }
+
+// device u -> residual in S->out (internal order)
+static int residual(Ctx* c) {
+  QsArgs a = args(c);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_qs_force<0>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  return c->hip(hipGetLastError(), "qs residual");
+}
+
+// K w for device w (S->w), output in `out`; needs theta at the current u
+static int tangent(Ctx* c, const double* w, double* out, bool theta_ready) {
+  QsArgs a = args(c);
+  a.w = w;
+  a.out = out;
+  hipStream_t s = c->stream;
+  (void)theta_ready;  // the dual pass recomputes theta with its derivative
+  hipLaunchKernelGGL((k_qs_theta<true, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_qs_force<1>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  return c->hip(hipGetLastError(), "qs tangent");
+}
+
+static int block_jacobi(Ctx* c) {
+  QsArgs a = args(c);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL((k_qs_theta<false, true>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_qs_diag, dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  return c->hip(hipGetLastError(), "qs block jacobi");
+}
+
+static int dot(Ctx* c, const double* x, const double* y, double* out) {
+  QsState* S = c->qs;
+  const int n = 3 * c->N;
+  hipLaunchKernelGGL(k_dot, dim3(S->nblk), dim3(RB), 0, c->stream, n, x, y, S->part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, c->stream, S->nblk, S->part, 1, S->scal);
+  int rc = c->hip(hipMemcpyAsync(out, S->scal, 8, hipMemcpyDeviceToHost, c->stream), "d2h");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
+}  // namespace qs
+
+void Ctx::free_extra() {
+  for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+  tev.clear();
+  if (qs) {
+    QsState* S = qs;
+    double* ptrs[] = {S->x, S->u, S->w, S->vol, S->m, S->alpha, S->beta, S->theta, S->dtheta, S->g,
+                      S->out, S->dinv, S->xs, S->r, S->z, S->p, S->q, S->b, S->part, S->scal};
+    for (double* p : ptrs) if (p) (void)hipFree(p);
+    if (S->fixed) (void)hipFree(S->fixed);
+    delete S;
+    qs = nullptr;
+  }
+  free_gnn();
+}
+
+// Preconditioned CG (precond 1: block Jacobi) or flexible CG with the learned
+// preconditioner (precond 2) on K x = b, device vectors in internal order.
+// x holds the initial guess. Returns iterations in *iters, final relative
+// residual in *rel.
+int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int maxit, int* iters,
+             double* rel) {
+  QsState* S = c->qs;
+  const int n3 = 3 * c->N;
+  hipStream_t s = c->stream;
+  const unsigned vb = (unsigned)((n3 + RB - 1) / RB);
+  const unsigned nb = (unsigned)((c->N + RB - 1) / RB);
+  int rc = 0;
+  double bb = 0.0;
+  if ((rc = qs::dot(c, b, b, &bb))) return rc;
+  const double bnorm = std::sqrt(bb);
+  *iters = 0;
+  if (bnorm == 0.0) {
+    rc = c->hip(hipMemsetAsync(x, 0, 8 * (size_t)n3, s), "zero x");
+    *rel = 0.0;
+    return rc;
+  }
+  // r = b - K x
+  if ((rc = qs::tangent(c, x, S->q, false))) return rc;
+  hipLaunchKernelGGL(k_sub, dim3(vb), dim3(RB), 0, s, n3, b, S->q, S->r);
+  auto precond_apply = [&](const double* r, double* z) -> int {
+    if (precond == 2) return gnn_apply(c, r, z);
+    if (precond == 1) {
+      hipLaunchKernelGGL(k_block_apply, dim3(nb), dim3(RB), 0, s, c->N, S->dinv, r, z);
+      return c->hip(hipGetLastError(), "block apply");
+    }
+    return c->hip(hipMemcpyAsync(z, r, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
+  };
+  if ((rc = precond_apply(S->r, S->z))) return rc;
+  rc = c->hip(hipMemcpyAsync(S->p, S->z, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
+  double rz = 0.0, rr = 0.0;
+  if (!rc) rc = qs::dot(c, S->r, S->z, &rz);
+  if (!rc) rc = qs::dot(c, S->r, S->r, &rr);
+  if (rc) return rc;
+  double relres = std::sqrt(rr) / bnorm;
+  int it = 0;
+  while (relres > tol && it < maxit) {
+    if ((rc = qs::tangent(c, S->p, S->q, true))) return rc;
+    double pq = 0.0;
+    if ((rc = qs::dot(c, S->p, S->q, &pq))) return rc;
+    if (!(pq > 0.0)) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
+    const double al = rz / pq;
+    hipLaunchKernelGGL(k_cg_update, dim3(S->nblk), dim3(RB), 0, s, n3, al, S->p, S->q, x, S->r, S->part);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, s, S->nblk, S->part, 1, S->scal);
+    rc = c->hip(hipMemcpyAsync(&rr, S->scal, 8, hipMemcpyDeviceToHost, s), "d2h");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    it++;
+    relres = std::sqrt(rr) / bnorm;
+    if (relres <= tol) break;
+    if ((rc = precond_apply(S->r, S->z))) return rc;
+    double rz_new = 0.0, beta = 0.0;
+    if ((rc = qs::dot(c, S->r, S->z, &rz_new))) return rc;
+    if (precond == 2) {
+      // flexible CG (one-term truncation): beta = -(z . q) / (p . q)
+      double zq = 0.0;
+      if ((rc = qs::dot(c, S->z, S->q, &zq))) return rc;
+      beta = -zq / pq;
+    } else {
+      beta = rz_new / rz;
+    }
+    rz = rz_new;
+    hipLaunchKernelGGL(k_axpby, dim3(vb), dim3(RB), 0, s, n3, 1.0, S->z, beta, S->p);
+  }
+  *iters = it;
+  *rel = relres;
+  if (relres > tol) return c->fail(PDG_E_NOCONV, "linear solve did not converge");
+  return c->hip(hipGetLastError(), "cg");
+}
+
+}  // namespace pdg
+
+using namespace pdg;
+
+extern "C" {
+
+int pdg_qs_set_fixed(pdg_ctx* h, const int* dofs, int numDofs) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || numDofs < 0 || (numDofs > 0 && !dofs)) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  if (rc) return rc;
+  QsState* S = c->qs;
+  std::fill(S->hfixed.begin(), S->hfixed.end(), 0);
+  for (int k = 0; k < numDofs; k++) {
+    const int d = dofs[k];
+    if (d < 0 || d >= 3 * c->N) return c->fail(PDG_E_ARG, "fixed DOF out of range");
+    S->hfixed[3 * (size_t)c->iperm[d / 3] + d % 3] = 1;
+  }
+  rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), S->hfixed.size(), hipMemcpyHostToDevice, c->stream), "h2d");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
+int pdg_qs_residual(pdg_ctx* h, const double* u, double* residual, double* norm) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  if (!rc) rc = qs::to_dev(c, u, c->qs->u);
+  if (!rc) rc = qs::residual(c);
+  if (rc) return rc;
+  std::vector<double> r(3 * (size_t)c->N);
+  rc = qs::to_host(c, c->qs->out, r.data());
+  if (rc) return rc;
+  double n2 = 0.0, ninf = 0.0;
+  for (double v : r) { n2 += v * v; ninf = std::max(ninf, std::fabs(v)); }
+  if (residual) std::copy(r.begin(), r.end(), residual);
+  if (norm) *norm = std::sqrt(n2) + 20.0 * ninf;
+  return PDG_OK;
+}
+
+int pdg_qs_tangent_apply(pdg_ctx* h, const double* u, const double* w, double* Kw) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u || !w || !Kw) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  if (!rc) rc = qs::to_dev(c, u, c->qs->u);
+  if (!rc) rc = qs::to_dev(c, w, c->qs->w);
+  if (!rc) rc = qs::tangent(c, c->qs->w, c->qs->out, false);
+  if (!rc) rc = qs::to_host(c, c->qs->out, Kw);
+  return rc;
+}
+
+int pdg_qs_block_diagonal(pdg_ctx* h, const double* u, double* blocksInv) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u || !blocksInv) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  if (!rc) rc = qs::to_dev(c, u, c->qs->u);
+  if (!rc) rc = qs::block_jacobi(c);
+  if (rc) return rc;
+  std::vector<double> t(9 * (size_t)c->N);
+  rc = c->hip(hipMemcpyAsync(t.data(), c->qs->dinv, t.size() * 8, hipMemcpyDeviceToHost, c->stream), "d2h");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  for (int k = 0; k < c->N; k++)
+    for (int q = 0; q < 9; q++) blocksInv[9 * (size_t)c->perm[k] + q] = t[9 * (size_t)k + q];
+  return PDG_OK;
+}
+
+int pdg_qs_solve(pdg_ctx* h, const double* u, const double* rhs, double* x, int precond,
+                 double relTol, int maxIter, int* iters, double* relRes) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u || !rhs || !x || precond < 0 || precond > 2) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  QsState* S = c->qs;
+  if (!rc) rc = qs::to_dev(c, u, S->u);
+  if (!rc && precond == 1) rc = qs::block_jacobi(c);
+  if (!rc) rc = qs::to_dev(c, rhs, S->b);
+  if (!rc) rc = qs::to_dev(c, x, S->xs);
+  if (rc) return rc;
+  int it = 0;
+  double rel = 0.0;
+  rc = qs_solve(c, S->xs, S->b, precond, relTol, maxIter, &it, &rel);
+  if (iters) *iters = it;
+  if (relRes) *relRes = rel;
+  int rc2 = qs::to_host(c, S->xs, x);
+  return rc ? rc : rc2;
+}
+
+// Newton iterations at fixed prescribed values: u (reference order) holds
+// the prescribed values on fixed DOFs and the initial guess elsewhere.
+// Converged when ||r||_2 + 20 ||r||_inf <= tol * max(1, ||reactions||_2)
+// (src/core/Peridigm.cpp:2716-2746). Each linear solve runs to linTol.
+int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond, double linTol,
+                  int maxLin, int* newtonIters, int* linIters, double* residualNorm) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u || precond < 0 || precond > 2) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  QsState* S = c->qs;
+  if (!rc) rc = qs::to_dev(c, u, S->u);
+  if (rc) return rc;
+  const int n3 = 3 * c->N;
+  const unsigned vb = (unsigned)((n3 + RB - 1) / RB);
+  int nit = 0, lit = 0;
+  bool converged = false;
+  double rnorm = 0.0;
+  std::vector<double> hr(n3), hreac(n3);
+  for (;; nit++) {
+    if ((rc = qs::residual(c))) return rc;
+    // residual norm and reactions (force on fixed DOFs) for the tolerance
+    rc = c->hip(hipMemcpyAsync(hr.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
+    if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    double n2 = 0.0, ninf = 0.0;
+    for (double v : hr) { n2 += v * v; ninf = std::max(ninf, std::fabs(v)); }
+    rnorm = std::sqrt(n2) + 20.0 * ninf;
+    // reactions: f V on fixed DOFs (recomputed with fixed rows unzeroed)
+    double reac = 0.0;
+    {
+      std::vector<signed char> none(n3, 0);
+      rc = c->hip(hipMemcpyAsync(S->fixed, none.data(), n3, hipMemcpyHostToDevice, c->stream), "h2d");
+      if (!rc) rc = qs::residual(c);
+      if (!rc) rc = c->hip(hipMemcpyAsync(hreac.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
+      if (!rc) rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), n3, hipMemcpyHostToDevice, c->stream), "h2d");
+      if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+      if (rc) return rc;
+      for (int q = 0; q < n3; q++) if (S->hfixed[q]) reac += hreac[q] * hreac[q];
+      reac = std::sqrt(reac);
+    }
+    if (rnorm <= tol * std::max(1.0, reac)) { converged = true; break; }
+    if (nit == maxNewton) break;
+    if (precond == 1 && (rc = qs::block_jacobi(c))) return rc;
+    if (precond == 2 && (rc = qs::block_jacobi(c))) return rc;  // learned preconditioner uses D too
+    rc = c->hip(hipMemcpyAsync(S->b, S->out, 8 * (size_t)n3, hipMemcpyDeviceToDevice, c->stream), "copy");
+    if (!rc) rc = c->hip(hipMemsetAsync(S->xs, 0, 8 * (size_t)n3, c->stream), "zero");
+    if (rc) return rc;
+    int it = 0;
+    double rel = 0.0;
+    rc = qs_solve(c, S->xs, S->b, precond, linTol, maxLin, &it, &rel);
+    lit += it;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_axpby, dim3(vb), dim3(RB), 0, c->stream, n3, 1.0, S->xs, 1.0, S->u);
+  }
+  if (newtonIters) *newtonIters = nit;
+  if (linIters) *linIters = lit;
+  if (residualNorm) *residualNorm = rnorm;
+  rc = qs::to_host(c, S->u, u);
+  if (rc) return rc;
+  if (!converged) return c->fail(PDG_E_NOCONV, "Newton did not converge");
+  return PDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+// LDS-staged bond passes over Morton tiles.
+//
+// One workgroup (8 waves) owns a tile: a contiguous range of at most 128
+// Morton-ordered nodes. It first stages the FP64 reference positions, FP64
+// displacements and per-node scalars of the tile's halo (the union of the
+// tile's nodes and all their neighbours, at most 1024 nodes) in LDS with one
+// 64-byte record load per halo node, then runs group-of-G-lanes bond loops
+// that read neighbours from LDS through 16-bit local indices. Per bond, HBM
+// and L2 see a 2-byte index and one bit of the damage mask instead of a
+// 4-byte index plus a 64-byte record gather.
+//
+// A node's damage-mask words (at most 2, degree <= 128) are held in registers
+// for the whole bond loop, the lane's 16-bit neighbour indices are prefetched
+// before it, and the loop is unrolled so every register index is static;
+// pass 1 rebuilds the words from per-iteration ballots and writes back only
+// the words that changed. The influence function is a template parameter.
+//
+// Workgroup -> tile mapping is XCD-aware: consecutive workgroup ids are
+// dispatched to the 8 XCDs in turn, so each XCD gets a contiguous run of
+// tiles and the halos shared by neighbouring tiles stay in its L2.
+//
+// The bond arithmetic and the Verlet epilogue are those of pdg_bonds.hip
+// (same helpers), so results match the untiled kernels up to FP32 summation
+// order.
+
+#include "pdg_device.hpp"
+
+namespace pdg {
+
+namespace {
+
+using namespace dev;
+
+constexpr int HMAX = kTileHaloMax;
+constexpr int WG = 512;
+constexpr int MAXDEG = 128;
+constexpr int NWORDS = MAXDEG / 64;
+
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+  const int per = (ntiles + 7) >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+inline unsigned tile_grid(int ntiles) { return 8u * (unsigned)((ntiles + 7) >> 3); }
+
+template <int INFL>
+__device__ __forceinline__ float omega_t(float zeta, float hhat) {
+  return omega_hat(INFL, zeta, hhat);
+}
+
+// Loads a node's mask words into registers.
+__device__ __forceinline__ void load_words(const unsigned long long* __restrict__ mask, int w0,
+                                           int deg, unsigned long long (&m)[NWORDS]) {
+  const int nw = (deg + 63) >> 6;
+#pragma unroll
+  for (int w = 0; w < NWORDS; w++) m[w] = w < nw ? mask[(unsigned)(w0 + w)] : 0ull;
+}
+
+// Prefetches a lane's local neighbour indices for the whole row.
+template <int G>
+__device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lcol, int rs, int deg,
+                                          int l, int (&lj)[MAXDEG / G]) {
+#pragma unroll
+  for (int it = 0; it < MAXDEG / G; it++) {
+    const int k = it * G + l;
+    lj[it] = k < deg ? (int)lcol[(unsigned)(rs + k)] : 0;
+  }
+}
+
+template <int G, bool DAMAGE, bool NEAR, int INFL>
+__global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
+                                                    NodeRec* __restrict__ rec,
+                                                    float* __restrict__ theta,
+                                                    int* __restrict__ nbroken) {
+  __shared__ double sx[3][HMAX];
+  __shared__ double su[3][HMAX];
+  __shared__ float svol[HMAX];
+  const int tile = xcd_tile(blockIdx.x, t.ntiles);
+  if (tile >= t.ntiles) return;
+  const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
+  const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
+  for (int h = threadIdx.x; h < hn; h += WG) {
+    const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
+    sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
+    su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
+    svol[h] = r.vol;
+  }
+  __syncthreads();
+  const int own = t.own_off[tile];
+  constexpr int NG = WG / G;
+  const int grp = threadIdx.x / G;
+  const int g = (threadIdx.x & 63) / G;
+  const int l = threadIdx.x & (G - 1);
+  for (int base = n0; base < n1; base += NG) {
+    const int node = base + grp;
+    const bool valid = node < n1;
+    int rs = 0, deg = 0, w0 = 0;
+    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
+    NodeConst c{0.0f, 0.0f, 0.0f, 1.0f};
+    if (valid) {
+      rs = a.row[node];
+      deg = a.row[node + 1] - rs;
+      w0 = a.mw[node];
+      const int li = own + (node - n0);
+      xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
+      ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
+      c = a.nc[node];
+    }
+    unsigned long long mo[NWORDS], mn[NWORDS];
+#pragma unroll
+    for (int w = 0; w < NWORDS; w++) mn[w] = 0ull;
+    load_words(a.mask, w0, deg, mo);
+    int lj[MAXDEG / G];
+    load_lcol<G>(t.lcol, rs, deg, l, lj);
+    const float scz = c.sc;
+    const int nmax = wave_max((deg + G - 1) / G);
+    float acc = 0.0f;
+    int nnear = 0;
+#pragma unroll
+    for (int it = 0; it < MAXDEG / G; ++it) {
+      if (it < nmax) {
+        const int k = it * G + l;
+        const bool act = k < deg;
+        bool brk = act && (((mo[(it * G) >> 6] >> (k & 63)) & 1ull) != 0ull);
+        if (act) {
+          const int j = lj[it];
+          const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, sx[0][j], sx[1][j],
+                                               sx[2][j], su[0][j], su[1][j], su[2][j]);
+          if (DAMAGE) {
+            // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0)
+            const float lim = scz * b.zeta;
+            if (NEAR && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
+            brk = brk || (b.ext > lim);
+          }
+          const float w = omega_t<INFL>(b.zeta, c.hhat);
+          if (!brk) acc = fmaf(w * b.zeta * b.ext, svol[j], acc);
+        }
+        if (DAMAGE) {
+          const unsigned long long bal = __ballot(brk);
+          const unsigned long long gb =
+              G == 64 ? bal : ((bal >> (g * G)) & ((1ull << (G & 63)) - 1ull));
+          mn[(it * G) >> 6] |= gb << ((it * G) & 63);
+        }
+      }
+    }
+    acc = group_sum<G>(acc);
+    if (NEAR) nnear = group_sum_i<G>(nnear);
+    if (valid && l == 0) {
+      theta[node] = c.ctheta * acc;
+      rec[node].p = c.kp * acc;
+      if (DAMAGE) {
+        const int nw = (deg + 63) >> 6;
+        int nb = 0;
+#pragma unroll
+        for (int w = 0; w < NWORDS; w++) {
+          if (w < nw && mn[w] != mo[w]) a.mask[(unsigned)(w0 + w)] = mn[w];
+          nb += __popcll(mn[w]);
+        }
+        if (nbroken) nbroken[node] = nb;
+      }
+      if (NEAR && nnear) atomicAdd(a.near_ctr, (unsigned long long)nnear);
+    }
+  }
+}
+
+template <int G, bool VERLET, int INFL>
+__global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
+                                                    const NodeRec* __restrict__ rec,
+                                                    NodeRec* __restrict__ rec_next,
+                                                    double* __restrict__ fout, VerletArgs va) {
+  __shared__ double sx[3][HMAX];
+  __shared__ double su[3][HMAX];
+  __shared__ float svol[HMAX];
+  __shared__ float salpha[HMAX];
+  __shared__ float sp[HMAX];
+  const int tile = xcd_tile(blockIdx.x, t.ntiles);
+  if (tile >= t.ntiles) return;
+  const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
+  const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
+  for (int h = threadIdx.x; h < hn; h += WG) {
+    const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
+    sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
+    su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
+    svol[h] = r.vol;
+    salpha[h] = r.alpha;
+    sp[h] = r.p;
+  }
+  __syncthreads();
+  const int own = t.own_off[tile];
+  constexpr int NG = WG / G;
+  const int grp = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  for (int base = n0; base < n1; base += NG) {
+    const int node = base + grp;
+    const bool valid = node < n1;
+    int rs = 0, deg = 0, w0 = 0, li = 0;
+    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
+    float alpha_i = 0.0f, p_i = 0.0f, hhat = 1.0f;
+    if (valid) {
+      rs = a.row[node];
+      deg = a.row[node + 1] - rs;
+      w0 = a.mw[node];
+      li = own + (node - n0);
+      xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
+      ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
+      alpha_i = salpha[li];
+      p_i = sp[li];
+      hhat = a.nc[node].hhat;
+    }
+    unsigned long long mo[NWORDS];
+    load_words(a.mask, w0, deg, mo);
+    int lj[MAXDEG / G];
+    load_lcol<G>(t.lcol, rs, deg, l, lj);
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+#pragma unroll
+    for (int it = 0; it < MAXDEG / G; ++it) {
+      const int k = it * G + l;
+      if (k < deg) {
+        const bool brk = ((mo[(it * G) >> 6] >> (k & 63)) & 1ull) != 0ull;
+        const int j = lj[it];
+        const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, sx[0][j], sx[1][j],
+                                           sx[2][j], su[0][j], su[1][j], su[2][j]);
+        const float w = omega_t<INFL>(b.zeta, hhat);
+        float tt = fmaf(b.zeta, p_i + sp[j], b.ext * (alpha_i + salpha[j]));
+        tt = brk ? 0.0f : w * tt * svol[j] * b.ry;
+        f0 = fmaf(tt, b.y0, f0);
+        f1 = fmaf(tt, b.y1, f1);
+        f2 = fmaf(tt, b.y2, f2);
+      }
+    }
+    f0 = group_sum<G>(f0);
+    f1 = group_sum<G>(f1);
+    f2 = group_sum<G>(f2);
+    if (valid && l < 3) {
+      const float fc = l == 0 ? f0 : (l == 1 ? f1 : f2);
+      if (!VERLET) fout[3 * node + l] = (double)fc;
+      else verlet_epilogue(va, node, l, fc, su[l][li], rec_next);
+    }
+  }
+}
+
+template <int G, int INFL>
+void p1_launch(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, int* nbroken,
+               bool damage, bool count_near, hipStream_t s) {
+  const dim3 grid(tile_grid(t.ntiles)), block(WG);
+  if (damage && count_near)
+    hipLaunchKernelGGL((k_pass1_tiled<G, true, true, INFL>), grid, block, 0, s, a, t, rec, theta, nbroken);
+  else if (damage)
+    hipLaunchKernelGGL((k_pass1_tiled<G, true, false, INFL>), grid, block, 0, s, a, t, rec, theta, nbroken);
+  else
+    hipLaunchKernelGGL((k_pass1_tiled<G, false, false, INFL>), grid, block, 0, s, a, t, rec, theta, nbroken);
+}
+
+template <int G, bool VERLET, int INFL>
+void p2_launch(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* rec_next,
+               double* f, const VerletArgs& va, hipStream_t s) {
+  const dim3 grid(tile_grid(t.ntiles)), block(WG);
+  hipLaunchKernelGGL((k_pass2_tiled<G, VERLET, INFL>), grid, block, 0, s, a, t, rec, rec_next, f, va);
+}
+
+template <int G>
+void p1_infl(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, int* nbroken,
+             bool damage, bool count_near, hipStream_t s) {
+  if (a.influence == PDG_INFLUENCE_ONE) p1_launch<G, 0>(a, t, rec, theta, nbroken, damage, count_near, s);
+  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p1_launch<G, 1>(a, t, rec, theta, nbroken, damage, count_near, s);
+  else p1_launch<G, 2>(a, t, rec, theta, nbroken, damage, count_near, s);
+}
+
+template <int G, bool VERLET>
+void p2_infl(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* rec_next, double* f,
+             const VerletArgs& va, hipStream_t s) {
+  if (a.influence == PDG_INFLUENCE_ONE) p2_launch<G, VERLET, 0>(a, t, rec, rec_next, f, va, s);
+  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p2_launch<G, VERLET, 1>(a, t, rec, rec_next, f, va, s);
+  else p2_launch<G, VERLET, 2>(a, t, rec, rec_next, f, va, s);
+}
+
+}  // namespace
+
+hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta,
+                              int* nbroken, bool damage, bool count_near, int lanes,
+                              hipStream_t s) {
+  if (t.ntiles <= 0) return hipSuccess;
+  if (lanes == 16) p1_infl<16>(a, t, rec, theta, nbroken, damage, count_near, s);
+  else if (lanes == 32) p1_infl<32>(a, t, rec, theta, nbroken, damage, count_near, s);
+  else p1_infl<64>(a, t, rec, theta, nbroken, damage, count_near, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_pass2_tiled_force(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
+                                    double* f, int lanes, hipStream_t s) {
+  if (t.ntiles <= 0) return hipSuccess;
+  VerletArgs va{};
+  if (lanes == 16) p2_infl<16, false>(a, t, rec, nullptr, f, va, s);
+  else if (lanes == 32) p2_infl<32, false>(a, t, rec, nullptr, f, va, s);
+  else p2_infl<64, false>(a, t, rec, nullptr, f, va, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
+                                     NodeRec* rec_next, const VerletArgs& va, int lanes,
+                                     hipStream_t s) {
+  if (t.ntiles <= 0) return hipSuccess;
+  if (lanes == 16) p2_infl<16, true>(a, t, rec, rec_next, nullptr, va, s);
+  else if (lanes == 32) p2_infl<32, true>(a, t, rec, rec_next, nullptr, va, s);
+  else p2_infl<64, true>(a, t, rec, rec_next, nullptr, va, s);
+  return hipGetLastError();
+}
+
+}  // namespace pdg

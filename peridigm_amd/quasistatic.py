@@ -1,0 +1,87 @@
+"""Quasi-static Newton solve on the device (Peridigm::executeQuasiStatic).
+
+Load stepping as in src/core/Peridigm.cpp:2578-2877: at each load step the
+prescribed displacements are applied, the Newton loop solves K(u) du = r(u)
+with the matrix-free FP64 tangent, and each linear solve runs to the given
+relative residual with block-Jacobi CG or the learned preconditioner inside
+flexible CG.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from .core import Context, _f64, _i32, _p
+
+PRECOND = {"none": 0, "block_jacobi": 1, "learned": 2}
+
+
+def set_fixed(ctx: Context, dofs):
+    d = _i32(dofs)
+    ctx.check(ctx.lib.pdg_qs_set_fixed(ctx.h, _p(d), d.size))
+
+
+def residual(ctx: Context, u):
+    u = _f64(u, (ctx.N, 3)); r = np.zeros((ctx.N, 3)); n = C.c_double()
+    ctx.check(ctx.lib.pdg_qs_residual(ctx.h, _p(u), _p(r), C.byref(n)))
+    return r, n.value
+
+
+def tangent_apply(ctx: Context, u, w):
+    u = _f64(u, (ctx.N, 3)); w = _f64(w, (ctx.N, 3)); out = np.zeros((ctx.N, 3))
+    ctx.check(ctx.lib.pdg_qs_tangent_apply(ctx.h, _p(u), _p(w), _p(out)))
+    return out
+
+
+def block_diagonal_inverse(ctx: Context, u):
+    u = _f64(u, (ctx.N, 3)); out = np.zeros((ctx.N, 3, 3))
+    ctx.check(ctx.lib.pdg_qs_block_diagonal(ctx.h, _p(u), _p(out)))
+    return out
+
+
+def solve(ctx: Context, u, rhs, x0=None, precond="block_jacobi", rel_tol=1e-8, max_iter=20000):
+    u = _f64(u, (ctx.N, 3)); b = _f64(rhs, (ctx.N, 3))
+    x = np.zeros((ctx.N, 3)) if x0 is None else _f64(x0, (ctx.N, 3)).copy()
+    it = C.c_int(); rel = C.c_double()
+    ctx.check(ctx.lib.pdg_qs_solve(ctx.h, _p(u), _p(b), _p(x), PRECOND[precond], float(rel_tol),
+                                   int(max_iter), C.byref(it), C.byref(rel)))
+    return x, it.value, rel.value
+
+
+def newton(ctx: Context, u, tol=1e-6, max_newton=20, precond="block_jacobi", lin_tol=1e-8,
+           max_lin=20000):
+    u = _f64(u, (ctx.N, 3)).copy()
+    ni = C.c_int(); li = C.c_int(); rn = C.c_double()
+    ctx.check(ctx.lib.pdg_qs_newton(ctx.h, _p(u), float(tol), int(max_newton), PRECOND[precond],
+                                    float(lin_tol), int(max_lin), C.byref(ni), C.byref(li),
+                                    C.byref(rn)))
+    return u, dict(newton=ni.value, linear=li.value, residual=rn.value)
+
+
+def end_dofs(left, right):
+    """All three components of the end-layer nodes are kinematic."""
+    nodes = np.concatenate([left, right])
+    return (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+
+
+def run_tension(ctx: Context, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8):
+    """Tensile load stepping of the qs_bar problem: end layers displaced by
+    -/+ strain*Lx/2 * s (y, z held) for s = k/load_steps."""
+    N = ctx.N
+    set_fixed(ctx, end_dofs(p["left"], p["right"]))
+    Lx = p["length"][0]
+    u = np.zeros((N, 3))
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(1, p["load_steps"] + 1):
+        s = k / p["load_steps"]
+        u[p["left"], :] = 0.0
+        u[p["right"], :] = 0.0
+        u[p["left"], 0] = -0.5 * p["strain"] * Lx * s
+        u[p["right"], 0] = 0.5 * p["strain"] * Lx * s
+        u, st = newton(ctx, u, tol=tol, precond=precond, lin_tol=lin_tol)
+        stats.append(st)
+    wall = time.perf_counter() - t0
+    return u, stats, wall

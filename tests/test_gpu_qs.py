@@ -1,0 +1,127 @@
+"""Quasi-static path on the GPU against the FP64 oracle.
+
+FP64 kernels, so tolerances are near round-off: residual and K w within
+1e-10 relative (max norm), diagonal blocks within 1e-9, and every linear solve
+is re-checked with the oracle's own tangent to ||K x - b|| / ||b|| <= 1e-7.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n=(8, 6, 5), h=1e-3, mu=60e9):
+    from peridigm_amd import Context
+    x, vol, nl = O.quickgrid(*n, [0, 0, 0], [k * h for k in n], 3.015 * h)
+    ctx = Context(0)
+    ctx.set_topology(nl, len(vol))
+    ctx.set_constants(x, vol, 130e9, mu, 3.015 * h, 0, 0.0)
+    return ctx, x, vol, nl, 3.015 * h, mu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _fixed_ends(x, h, nx):
+    i = np.rint(x[:, 0] / h - 0.5).astype(int)
+    left = np.nonzero(i <= 1)[0]
+    right = np.nonzero(i >= nx - 2)[0]
+    return left, right
+
+
+def test_residual_and_tangent_match_oracle():
+    from peridigm_amd import quasistatic as Q
+    ctx, x, vol, nl, hz, mu = _setup()
+    with ctx:
+        rng = np.random.default_rng(3)
+        u = 1e-6 * rng.standard_normal(x.shape) + 1e-3 * x
+        w = rng.standard_normal(x.shape)
+        left, right = _fixed_ends(x, 1e-3, 8)
+        fixed = Q.end_dofs(left, right)
+        Q.set_fixed(ctx, fixed)
+        m = ctx.initialize()
+        bd = np.zeros(O.num_bonds(nl, len(vol)))
+        f, jw = O.force_directional(x, u, w, m, vol, bd, nl, 130e9, mu, hz)
+        r_ref = (f * vol[:, None]).ravel()
+        kw_ref = (-jw * vol[:, None]).ravel()
+        r_ref[fixed] = 0.0
+        kw_ref[fixed] = w.ravel()[fixed]
+        r, nrm = Q.residual(ctx, u)
+        assert _rel(r.ravel(), r_ref) <= 1e-10
+        assert abs(nrm - (np.linalg.norm(r_ref) + 20 * np.abs(r_ref).max())) <= 1e-10 * nrm
+        kw = Q.tangent_apply(ctx, u, w)
+        assert _rel(kw.ravel(), kw_ref) <= 1e-10
+
+
+def test_block_diagonal_matches_dense_oracle_jacobian():
+    from peridigm_amd import quasistatic as Q
+    ctx, x, vol, nl, hz, mu = _setup(n=(5, 4, 4))
+    with ctx:
+        rng = np.random.default_rng(5)
+        u = 1e-6 * rng.standard_normal(x.shape) + 5e-4 * x
+        m = ctx.initialize()
+        bd = np.zeros(O.num_bonds(nl, len(vol)))
+        N = len(vol)
+        Q.set_fixed(ctx, [3 * 0 + 1])  # node 0, y held
+        dinv = Q.block_diagonal_inverse(ctx, u)
+        for i in (0, 7, 21, N - 1):
+            D = np.zeros((3, 3))
+            for c in range(3):
+                w = np.zeros((N, 3)); w[i, c] = 1.0
+                _, jw = O.force_directional(x, u, w, m, vol, bd, nl, 130e9, mu, hz)
+                D[:, c] = -vol[i] * jw[i]
+            if i == 0:
+                D[1, :] = 0.0; D[:, 1] = 0.0; D[1, 1] = 1.0
+            assert _rel(dinv[i], np.linalg.inv(D)) <= 1e-9
+
+
+@pytest.mark.parametrize("precond", ["none", "block_jacobi"])
+def test_linear_solve_reaches_1e8(precond):
+    from peridigm_amd import quasistatic as Q
+    ctx, x, vol, nl, hz, mu = _setup(n=(16, 6, 6))
+    with ctx:
+        left, right = _fixed_ends(x, 1e-3, 16)
+        fixed = Q.end_dofs(left, right)
+        Q.set_fixed(ctx, fixed)
+        rng = np.random.default_rng(8)
+        u = 2e-4 * x
+        b = rng.standard_normal(x.shape).ravel()
+        b[fixed] = 0.0
+        b = b.reshape(-1, 3)
+        xs, it, rel = Q.solve(ctx, u, b, precond=precond, rel_tol=1e-8)
+        assert rel <= 1e-8 and it > 0
+        m = ctx.initialize()
+        bd = np.zeros(O.num_bonds(nl, len(vol)))
+        _, jw = O.force_directional(x, u, xs, m, vol, bd, nl, 130e9, mu, hz)
+        kx = (-jw * vol[:, None]).ravel()
+        kx[fixed] = xs.ravel()[fixed]
+        assert np.linalg.norm(kx - b.ravel()) / np.linalg.norm(b) <= 1e-7
+
+
+def test_newton_tension_bar_converges():
+    from peridigm_amd import Context
+    from peridigm_amd import quasistatic as Q
+    from peridigm_amd.problems import qs_bar
+    p = qs_bar(nx=20, ny=8, nz=8, load_steps=3)
+    xo, vo, nl = O.quickgrid(*p["n"], p["origin"], p["length"], p["horizon"])
+    with Context(0) as ctx:
+        ctx.set_topology(nl, len(vo))
+        ctx.set_constants(xo, vo, p["K"], p["mu"], p["horizon"], 0, 0.0)
+        u, stats, wall = Q.run_tension(ctx, p, precond="block_jacobi", tol=1e-6)
+        m = ctx.initialize()
+    f = O.force_at(xo, u, m, vo, np.zeros(O.num_bonds(nl, len(vo))), nl, p["K"], p["mu"], p["horizon"])
+    r = (f * vo[:, None])
+    free = np.ones(len(vo), bool)
+    free[p["left"]] = False
+    free[p["right"]] = False
+    reac = np.linalg.norm(r[~free])
+    assert np.linalg.norm(r[free]) + 20 * np.abs(r[free]).max() <= 1e-6 * max(1.0, reac)
+    # uniform tension: mid-bar axial strain close to the applied one
+    mid = (np.abs(xo[:, 1] - xo[:, 1].mean()) < 1.1e-3) & (np.abs(xo[:, 2] - xo[:, 2].mean()) < 1.1e-3)
+    xs = xo[mid, 0]
+    slope = np.polyfit(xs, u[mid, 0], 1)[0]
+    assert 0.5 * p["strain"] < slope < 2.0 * p["strain"]
+    assert all(s["newton"] >= 1 for s in stats)
