@@ -77,6 +77,10 @@ int pdg_set_topology(pdg_ctx* ctx, int numOwnedPoints, const int* ownedIDs,
 int pdg_set_constants(pdg_ctx* ctx, const double* x, const double* volume,
                       double bulkModulus, double shearModulus, double horizon,
                       int influenceKind, double criticalStretch);
+/* Critical stretch per node (N) for multi-specimen runs; the damage model
+ * must have been enabled by a positive criticalStretch in pdg_set_constants.
+ * A bond i->j uses node i's value. */
+int pdg_set_critical_stretch(pdg_ctx* ctx, const double* criticalStretch);
 /* Density per node (N) for the explicit integrator. */
 int pdg_set_density(pdg_ctx* ctx, const double* density);
 
@@ -163,6 +167,21 @@ int pdg_qs_solve(pdg_ctx* ctx, const double* u, const double* rhs, double* x, in
 int pdg_qs_newton(pdg_ctx* ctx, double* u, double tol, int maxNewton, int precond,
                   double linTol, int maxLin, int* newtonIters, int* linIters,
                   double* residualNorm);
+
+/* ---- learned preconditioner (no counterpart in the reference) -----------
+ * A 3-layer message-passing network (width 32, BF16 MFMA, FP32 accumulate)
+ * over the bond graph; used by PDG_PRECOND_LEARNED inside flexible CG. */
+int pdg_gnn_init(pdg_ctx* ctx, unsigned seed);
+int pdg_gnn_num_params(void);
+int pdg_gnn_get_params(pdg_ctx* ctx, float* params);
+int pdg_gnn_set_params(pdg_ctx* ctx, const float* params);
+/* Forward on node features (N x 8) -> out (N x 3); with gout (N x 3) also the
+ * gradient of sum(gout * out) w.r.t. the parameters. */
+int pdg_gnn_forward_backward(pdg_ctx* ctx, const float* features, const float* gout,
+                             float* out, float* dparams);
+/* Adam training on (r, e = K^-1 r) pairs (nsamples x 3N each) at tangent u. */
+int pdg_gnn_train(pdg_ctx* ctx, const double* u, int nsamples, const double* r,
+                  const double* e, int epochs, float lr, unsigned seed, float* lastLoss);
 
 /* ---- neighbour search (integer results, bit-exact predicates) ----------- */
 /* QuickGrid tensor-product lattice: node coordinates (3N), volumes (N) and

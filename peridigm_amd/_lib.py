@@ -49,6 +49,7 @@ SIGNATURES = {
     "pdg_set_topology": (_i, [_vp, _i, _vp, _vp, _ll]),
     "pdg_set_constants": (_i, [_vp, _vp, _vp, _d, _d, _d, _i, _d]),
     "pdg_set_density": (_i, [_vp, _vp]),
+    "pdg_set_critical_stretch": (_i, [_vp, _vp]),
     "pdg_get_sizes": (_i, [_vp, C.POINTER(_i), C.POINTER(_ll)]),
     "pdg_initialize": (_i, [_vp, _vp]),
     "pdg_compute_damage": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(_ll)]),
@@ -68,6 +69,12 @@ SIGNATURES = {
     "pdg_qs_solve": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, C.POINTER(_i), C.POINTER(_d)]),
     "pdg_qs_newton": (_i, [_vp, _vp, _d, _i, _i, _d, _i, C.POINTER(_i), C.POINTER(_i),
                            C.POINTER(_d)]),
+    "pdg_gnn_init": (_i, [_vp, C.c_uint]),
+    "pdg_gnn_num_params": (_i, []),
+    "pdg_gnn_get_params": (_i, [_vp, _vp]),
+    "pdg_gnn_set_params": (_i, [_vp, _vp]),
+    "pdg_gnn_forward_backward": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "pdg_gnn_train": (_i, [_vp, _vp, _i, _vp, _vp, _i, C.c_float, C.c_uint, C.POINTER(C.c_float)]),
     "pdg_quickgrid": (_i, [_i, _i, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, C.POINTER(_ll)]),
     "pdg_radius_search": (_i, [_i, _vp, _d, _i, _vp, _vp, C.POINTER(_ll)]),
 }

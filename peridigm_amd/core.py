@@ -88,6 +88,11 @@ class Context:
                                               float(horizon), int(influence),
                                               float(critical_stretch)))
 
+    def set_critical_stretch(self, sc):
+        """Per-node critical stretch (multi-specimen runs)."""
+        v = _f64(np.broadcast_to(np.asarray(sc, float), (self.N,)))
+        self.check(self.lib.pdg_set_critical_stretch(self.h, _p(v)))
+
     def set_density(self, rho):
         r = _f64(np.broadcast_to(np.asarray(rho, float), (self.N,)))
         self.check(self.lib.pdg_set_density(self.h, _p(r)))

@@ -499,6 +499,24 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   return PDG_OK;
 }
 
+int pdg_set_critical_stretch(pdg_ctx* h, const double* sc) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !sc) return PDG_E_ARG;
+  if (!(c->sc > 0.0)) return c->fail(PDG_E_ARG, "damage model disabled (critical stretch <= 0 at set-up)");
+  std::vector<NodeConst> nc(c->N);
+  int rc = c->hip(hipMemcpyAsync(nc.data(), c->d_nc, c->N * sizeof(NodeConst), hipMemcpyDeviceToHost, c->stream), "d2h nc");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  for (int k = 0; k < c->N; k++) {
+    const double v = sc[c->perm[k]];
+    if (!(v > 0.0)) return c->fail(PDG_E_ARG, "critical stretch must be positive");
+    nc[k].sc = (float)v;
+  }
+  rc = c->hip(hipMemcpyAsync(c->d_nc, nc.data(), c->N * sizeof(NodeConst), hipMemcpyHostToDevice, c->stream), "h2d nc");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
 int pdg_set_density(pdg_ctx* h, const double* rho) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || !rho) return PDG_E_ARG;

@@ -82,6 +82,8 @@ struct Ctx {
   GnnState* gnn = nullptr;
 
   void free_gnn();
+  double* qs_dinv();
+  const signed char* qs_fixed();
   int fail(int code, const std::string& msg);
   int hip(hipError_t e, const char* what);
   void free_device();
@@ -98,5 +100,7 @@ struct Ctx {
 int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int maxit, int* iters,
              double* rel);
 int gnn_apply(Ctx* c, const double* r, double* z);
+// uploads u (reference order) and forms the block-Jacobi inverse at u
+int pdg_qs_prepare(Ctx* c, const double* u);
 
 }  // namespace pdg

@@ -433,6 +433,16 @@ static int dot(Ctx* c, const double* x, const double* y, double* out) {
 
 }  // namespace qs
 
+double* Ctx::qs_dinv() { return qs ? qs->dinv : nullptr; }
+const signed char* Ctx::qs_fixed() { return qs ? qs->fixed : nullptr; }
+
+int pdg_qs_prepare(Ctx* c, const double* u) {
+  int rc = qs::ensure(c);
+  if (!rc) rc = qs::to_dev(c, u, c->qs->u);
+  if (!rc) rc = qs::block_jacobi(c);
+  return rc;
+}
+
 void Ctx::free_extra() {
   for (hipEvent_t e : tev) (void)hipEventDestroy(e);
   tev.clear();
@@ -596,7 +606,7 @@ int pdg_qs_solve(pdg_ctx* h, const double* u, const double* rhs, double* x, int 
   int rc = qs::ensure(c);
   QsState* S = c->qs;
   if (!rc) rc = qs::to_dev(c, u, S->u);
-  if (!rc && precond == 1) rc = qs::block_jacobi(c);
+  if (!rc && precond >= 1) rc = qs::block_jacobi(c);
   if (!rc) rc = qs::to_dev(c, rhs, S->b);
   if (!rc) rc = qs::to_dev(c, x, S->xs);
   if (rc) return rc;

@@ -102,6 +102,49 @@ def qs_bar(nx=60, ny=20, nz=20, h=1.0e-3, strain=1.0e-3, load_steps=10):
                 strain=strain, load_steps=load_steps, search="quickgrid", planes=None, **STEEL)
 
 
+def batched_plates(nspec=64, seed=2024, h=5.0e-4, nz=4, nodes=(5000, 60000), sc=(3e-4, 1e-3),
+                   notch=(10, 60), dt=2.5e-8, nsteps=1000, pull=2.0):
+    """Parameter sweep: nspec notched plates of uneven size laid side by side
+    (gap of 10h, so no bond crosses specimens) in one node/bond space.
+
+    Per specimen (seeded): nx x ny x nz lattice with N in [5k, 60k], critical
+    stretch U[3e-4, 1e-3], notch length U[10h, 60h] from the x = 0 edge on
+    the mid plane, top/bottom 3-node layers pulled at +-pull."""
+    rng = np.random.default_rng(seed)
+    xs, vols, scs, planes, bcs, specs = [], [], [], [], [], []
+    off = 0.0
+    base = 0
+    for s in range(nspec):
+        target = int(rng.integers(nodes[0], nodes[1] + 1))
+        per_layer = max(1, target // nz)
+        aspect = float(rng.uniform(0.5, 2.0))
+        ny = max(8, int(round(np.sqrt(per_layer / aspect))))
+        nx = max(8, int(round(per_layer / ny)))
+        n = (nx, ny, nz)
+        L = [nx * h, ny * h, nz * h]
+        x, vol = lattice_centres(n, [off, 0.0, 0.0], L)
+        j = np.rint(x[:, 1] / h - 0.5).astype(int)
+        top = base + np.nonzero(j >= ny - 3)[0]
+        bot = base + np.nonzero(j <= 2)[0]
+        bcs += [(top, 1, 0.0, pull), (bot, 1, 0.0, -pull)]
+        ncells = float(rng.uniform(notch[0], notch[1]))
+        ncells = min(ncells, 0.8 * nx)
+        pl = plate_notch_plane(h, nx, ny, nz, 0)
+        pl[3] = off - h                       # lower-left corner x
+        pl[9] = (ncells + 1.03) * h           # bottom length
+        planes.append(pl)
+        s_c = float(rng.uniform(sc[0], sc[1]))
+        xs.append(x); vols.append(vol); scs.append(np.full(len(vol), s_c))
+        specs.append(dict(n=n, nodes=len(vol), critical_stretch=s_c, notch_cells=ncells, first=base))
+        base += len(vol)
+        off += L[0] + 10 * h
+    x = np.concatenate(xs); vol = np.concatenate(vols)
+    return dict(name="batched", h=h, x=x, volume=vol, horizon=3.015 * h,
+                critical_stretch=np.concatenate(scs), dt=dt, nsteps=nsteps,
+                v0=np.zeros_like(x), bcs=bcs, search="radius", planes=np.array(planes),
+                specimens=specs, seed=seed, **STEEL)
+
+
 def neighbours(p):
     """Neighbour list of a problem, built on the device."""
     if p["search"] == "quickgrid":
