@@ -57,6 +57,8 @@ def build(workload):
         p = problems.bar()
     elif workload == "cylinder":
         p = problems.cylinder()
+    elif workload == "batched":
+        p = problems.batched_plates()
     else:
         raise SystemExit(f"unknown workload {workload}")
     nl = problems.neighbours(p)
@@ -85,6 +87,45 @@ def cpu_baseline(steps=20):
                        f"(damage, dilatation, scatter force), {per_step*1e3:.1f} ms/step")
 
 
+def qs_figure(nsamples=200, epochs=20):
+    """Quasi-static tensile bar 60x20x20 (24k nodes), 10 load steps to 1e-3
+    strain, every linear solve to 1e-8 relative residual: wall time with
+    block-Jacobi CG and with the learned preconditioner in flexible CG (trained
+    on the device on `nsamples` block-Jacobi solves under random end loads)."""
+    from peridigm_amd import Context
+    from peridigm_amd import learned as Lr
+    from peridigm_amd import quasistatic as Q
+    from peridigm_amd.problems import qs_bar, neighbours
+    p = qs_bar()
+    nl = neighbours(p)
+    N = len(p["volume"])
+    out = {}
+    with Context(0) as ctx:
+        ctx.set_topology(nl, N)
+        ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, 0.0)
+        u_j, st_j, wall_j = Q.run_tension(ctx, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8)
+        t0 = time.perf_counter()
+        R, E, it_gen = Lr.generate_pairs(ctx, p, nsamples=nsamples, seed=11)
+        t_gen = time.perf_counter() - t0
+        Lr.init(ctx, 5)
+        t0 = time.perf_counter()
+        loss = Lr.train(ctx, R, E, epochs=epochs, lr=2e-3, seed=3)
+        t_train = time.perf_counter() - t0
+        u_l, st_l, wall_l = Q.run_tension(ctx, p, precond="learned", tol=1e-6, lin_tol=1e-8)
+    out = {
+        "workload": "QS tensile bar 60x20x20 (N=%d), 10 load steps to 1e-3 strain, "
+                    "linear solves to 1e-8 relative residual" % N,
+        "jacobi_pcg_s": wall_j, "learned_fcg_s": wall_l, "speedup": wall_j / wall_l,
+        "jacobi_linear_iters": int(sum(s["linear"] for s in st_j)),
+        "learned_linear_iters": int(sum(s["linear"] for s in st_l)),
+        "newton_iters": [int(sum(s["newton"] for s in st_j)), int(sum(s["newton"] for s in st_l))],
+        "max_u_diff_rel": float(np.abs(u_l - u_j).max() / np.abs(u_j).max()),
+        "training": {"samples": nsamples, "epochs": epochs, "final_rel_loss": loss,
+                     "data_s": t_gen, "train_s": t_train},
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +135,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--qs", action="store_true",
+                    help="also measure the quasi-static solve (block-Jacobi CG vs learned FCG)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3x) to calibrate HBM counters")
     args = ap.parse_args()
@@ -119,7 +162,12 @@ def main():
     ctx = Context(0, lanes=args.lanes, variant=args.variant)
     ctx.set_topology(nl, N)
     B = ctx.B
-    ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
+    sc = p["critical_stretch"]
+    per_node_sc = np.ndim(sc) > 0
+    ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0,
+                      float(np.max(sc)) if per_node_sc else sc)
+    if per_node_sc:
+        ctx.set_critical_stretch(sc)
     ctx.set_density(np.full(N, p["density"]))
     ctx.set_bcs(p["bcs"])
     ctx.upload_state(v=p["v0"])
@@ -171,6 +219,8 @@ def main():
                          "horizon 3.015h, +-2 m/s pull, fixed dt 2.5e-8 s",
                 "bar": "LPS elastic bar 100x10x10, h=1mm, horizon 3.015h, dt 1e-8 s",
                 "cylinder": "expanding hollow cylinder 40h-50h x 200h, h=1mm, perturbed, sc 1.5e-3",
+                "batched": "64 notched plates 5k-60k nodes (seed 2024), sc U[3e-4,1e-3], "
+                           "notch U[10h,60h], one node/bond space",
             }[args.workload],
             "nodes": N, "bonds": B, "lanes_per_node": args.lanes,
             "kernels": "lds-tiled" if (args.variant == 0 and ntiles > 0) else "untiled",
@@ -186,6 +236,8 @@ def main():
             "step_frac": (bytes_pass1(B, N) + bytes_pass2(B, N)) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
     }
+    if args.qs:
+        out["quasi_static"] = qs_figure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

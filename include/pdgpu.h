@@ -127,6 +127,13 @@ int pdg_set_bcs(pdg_ctx* ctx, const pdg_bc* bcs, int numBcs);
  * from time t0 with step dt on the device-resident state. */
 int pdg_verlet_run(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEval);
 
+/* Same run as pdg_verlet_run in ONE persistent cooperative launch (grid-wide
+ * barriers between the two passes of every step), for batched multi-specimen
+ * sweeps: the specimens are concatenated into one node/bond space. Needs the
+ * tiled kernels and 16 lanes per node. *gridSize receives the workgroups used. */
+int pdg_verlet_run_persistent(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEval,
+                              int* gridSize);
+
 /* Stable time step (ComputeCriticalTimeStep, constant horizon) for density rho. */
 int pdg_critical_time_step(pdg_ctx* ctx, double density, double* dtOut);
 

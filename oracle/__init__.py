@@ -38,6 +38,7 @@ def lib():
         L.orc_filter_planes.restype = C.c_longlong
         L.orc_critical_time_step.restype = C.c_double
         L.orc_explicit_run.restype = C.c_int
+        L.orc_explicit_run_sc.restype = C.c_int
         L.orc_plane_cuts.restype = C.c_int
         _LIB = L
     return _LIB
@@ -183,7 +184,8 @@ def _bc_arrays(bcs):
 
 def explicit_run(x, vol, rho, nlist, K, mu, horizon, sc, dt, nsteps, u0=None, v0=None,
                  bond_damage=None, bcs=(), t0=0.0, infl=0):
-    """Velocity-Verlet run (Peridigm::executeExplicit). Returns dict of final state."""
+    """Velocity-Verlet run (Peridigm::executeExplicit). Returns dict of final state.
+    sc: critical stretch (scalar, <= 0: no damage) or one value per node."""
     x = _c(x, np.float64); vol = _c(vol, np.float64)
     N = x.shape[0]
     rho = _c(np.broadcast_to(rho, (N,)), np.float64)
@@ -194,9 +196,14 @@ def explicit_run(x, vol, rho, nlist, K, mu, horizon, sc, dt, nsteps, u0=None, v0
     bd = np.zeros(B) if bond_damage is None else _c(bond_damage, np.float64).copy()
     a = np.zeros((N, 3)); th = np.zeros(N); f = np.zeros((N, 3)); dmg = np.zeros(N); m = np.zeros(N)
     ptr, nodes, coord, c0, c1 = _bc_arrays(bcs)
-    rc = lib().orc_explicit_run(C.c_int(N), _p(x), _p(vol), _p(rho), _p(nlist), C.c_double(K),
+    sc_node = None
+    if np.ndim(sc) > 0:
+        sc_node = _c(sc, np.float64)
+        sc = float(sc_node.max())
+    rc = lib().orc_explicit_run_sc(C.c_int(N), _p(x), _p(vol), _p(rho), _p(nlist), C.c_double(K),
                                 C.c_double(mu), C.c_double(horizon), C.c_int(infl),
-                                C.c_double(sc), C.c_double(dt), C.c_double(t0), C.c_int(nsteps),
+                                C.c_double(sc), None if sc_node is None else _p(sc_node),
+                                C.c_double(dt), C.c_double(t0), C.c_int(nsteps),
                                 C.c_int(len(bcs)), _p(ptr), _p(nodes), _p(coord), _p(c0), _p(c1),
                                 _p(u), _p(v), _p(a), _p(bd), _p(dmg), _p(th), _p(f), _p(m))
     if rc != 0:

@@ -148,12 +148,13 @@ void force_t(int numOwned, const double* x, const T* y, const double* m,
 // (bondDamageNP1 must hold bondDamageN on entry, as at :131).
 void damage_impl(int numOwned, const double* x, const double* y,
                  double* bdNP1, double* damage, const int* nlist, double sc,
-                 long long* near_count, double near_rel) {
+                 long long* near_count, double near_rel, const double* sc_node = nullptr) {
   int li = 0;
   long long b = 0;
   for (int i = 0; i < numOwned; ++i) {
     double X0 = x[3 * i], X1 = x[3 * i + 1], X2 = x[3 * i + 2];
     double Y0 = y[3 * i], Y1 = y[3 * i + 1], Y2 = y[3 * i + 2];
+    const double sci = sc_node ? sc_node[i] : sc;
     int nn = nlist[li++];
     for (int k = 0; k < nn; ++k) {
       int j = nlist[li++];
@@ -165,8 +166,8 @@ void damage_impl(int numOwned, const double* x, const double* y,
                              (Y2 - y[3 * j + 2]) * (Y2 - y[3 * j + 2]));
       double rel = (cur - init) / init;
       double trial = 0.0;
-      if (rel > sc) trial = 1.0;
-      if (near_count && bdNP1[b] == 0.0 && std::fabs(rel - sc) <= near_rel * sc)
+      if (rel > sci) trial = 1.0;
+      if (near_count && bdNP1[b] == 0.0 && std::fabs(rel - sci) <= near_rel * sci)
         (*near_count)++;
       if (trial > bdNP1[b]) bdNP1[b] = trial;
       b++;
@@ -538,6 +539,14 @@ double orc_critical_time_step(int N, const double* x, const double* vol, const i
 // force (density), damage, m receive the final values. bond_damage (B, list
 // order) is the initial bond damage on entry and the final one on exit.
 // sc <= 0 disables the damage model. rho is per node.
+int orc_explicit_run_sc(int N, const double* x, const double* vol, const double* rho,
+                        const int* nlist, double K, double mu, double horizon, int infl,
+                        double sc, const double* sc_node, double dt, double t0, int nsteps,
+                        int nbc, const int* bc_ptr, const int* bc_nodes, const int* bc_coord,
+                        const double* bc_c0, const double* bc_c1, double* u, double* v,
+                        double* a, double* bond_damage, double* damage, double* theta,
+                        double* force, double* m);
+
 int orc_explicit_run(int N, const double* x, const double* vol, const double* rho,
                      const int* nlist, double K, double mu, double horizon, int infl,
                      double sc, double dt, double t0, int nsteps, int nbc,
@@ -545,13 +554,27 @@ int orc_explicit_run(int N, const double* x, const double* vol, const double* rh
                      const double* bc_c0, const double* bc_c1, double* u, double* v,
                      double* a, double* bond_damage, double* damage, double* theta,
                      double* force, double* m) {
+  return orc_explicit_run_sc(N, x, vol, rho, nlist, K, mu, horizon, infl, sc, nullptr, dt, t0,
+                             nsteps, nbc, bc_ptr, bc_nodes, bc_coord, bc_c0, bc_c1, u, v, a,
+                             bond_damage, damage, theta, force, m);
+}
+
+// Same run with a critical stretch per node (one damage-model block per
+// specimen when several specimens share one list); sc > 0 enables damage.
+int orc_explicit_run_sc(int N, const double* x, const double* vol, const double* rho,
+                        const int* nlist, double K, double mu, double horizon, int infl,
+                        double sc, const double* sc_node, double dt, double t0, int nsteps,
+                        int nbc, const int* bc_ptr, const int* bc_nodes, const int* bc_coord,
+                        const double* bc_c0, const double* bc_c1, double* u, double* v,
+                        double* a, double* bond_damage, double* damage, double* theta,
+                        double* force, double* m) {
   BCs bc{nbc, bc_ptr, bc_nodes, bc_coord, bc_c0, bc_c1};
   std::vector<double> y(3 * N);
   orc_weighted_volume(N, x, vol, nlist, horizon, infl, m);
   for (int i = 0; i < 3 * N; i++) y[i] = x[i] + u[i];
   for (int i = 0; i < N; i++) damage[i] = 0.0;
   auto eval = [&]() {
-    if (sc > 0.0) damage_impl(N, x, y.data(), bond_damage, damage, nlist, sc, nullptr, 0.0);
+    if (sc > 0.0) damage_impl(N, x, y.data(), bond_damage, damage, nlist, sc, nullptr, 0.0, sc_node);
     orc_compute_force(N, x, y.data(), m, vol, bond_damage, nlist, K, mu, horizon, infl, theta, force);
   };
   eval();

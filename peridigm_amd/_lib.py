@@ -58,6 +58,7 @@ SIGNATURES = {
     "pdg_state_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pdg_set_bcs": (_i, [_vp, C.POINTER(pdg_bc), _i]),
     "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),
+    "pdg_verlet_run_persistent": (_i, [_vp, _i, _d, _d, _i, C.POINTER(_i)]),
     "pdg_critical_time_step": (_i, [_vp, _d, C.POINTER(_d)]),
     "pdg_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
     "pdg_set_per_pass_timing": (_i, [_vp, _i]),

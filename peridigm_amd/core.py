@@ -152,6 +152,13 @@ class Context:
         self.check(self.lib.pdg_verlet_run(self.h, int(nsteps), float(dt), float(t0),
                                            int(bool(initial_eval))))
 
+    def verlet_run_persistent(self, nsteps, dt, t0=0.0, initial_eval=True):
+        """All steps in one persistent cooperative launch; returns the grid size."""
+        g = C.c_int()
+        self.check(self.lib.pdg_verlet_run_persistent(self.h, int(nsteps), float(dt), float(t0),
+                                                      int(bool(initial_eval)), C.byref(g)))
+        return g.value
+
     def set_kernel_variant(self, variant):
         """0: LDS-staged Morton tiles (default), 1: untiled gathers."""
         self.check(self.lib.pdg_set_kernel_variant(self.h, int(variant)))

@@ -731,6 +731,49 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   return PDG_OK;
 }
 
+int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval,
+                              int* gridSize) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
+  if (!c->tiled()) return c->fail(PDG_E_UNSUPPORTED, "persistent run needs the tiled kernels");
+  if (c->lanes != 16) return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 16 lanes per node");
+  if (!initialEval && nsteps == 0) return PDG_OK;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  VerletArgs va{};
+  va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
+  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.err = c->d_err;
+  int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
+  if (rc) return rc;
+  // a continuing run starts step 1 from the stored acceleration
+  if (!initialEval) {
+    va.t_cur = t0 + (1 * dt);
+    va.t_prev = t0 + (0 * dt);
+    rc = c->hip(launch_verlet_prologue(c->N, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, s), "prologue");
+    if (rc) return rc;
+    c->cur ^= 1;
+  }
+  NodeRec* r0 = c->d_rec[c->cur];
+  NodeRec* r1 = c->d_rec[c->cur ^ 1];
+  float ms = 0.0f;
+  rc = c->hip(hipEventRecord(c->ev[0], s), "event");
+  if (!rc) rc = c->hip(launch_persistent(c->bond_args(), c->tile_args(), r0, r1, c->d_theta,
+                                         c->d_nbroken, va, nsteps, t0, initialEval, c->sc > 0.0,
+                                         gridSize, s), "persistent launch");
+  if (!rc) rc = c->hip(hipEventRecord(c->ev[3], s), "event");
+  if (!rc) rc = c->hip(hipEventSynchronize(c->ev[3]), "sync");
+  if (rc) return rc;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[3]);
+  c->t_total = ms;
+  const int evals = nsteps + (initialEval ? 1 : 0);
+  if (evals & 1) c->cur ^= 1;
+  int err = 0;
+  rc = c->hip(hipMemcpy(&err, c->d_err, 4, hipMemcpyDeviceToHost), "d2h err");
+  if (rc) return rc;
+  if (err) return c->fail(PDG_E_NONFINITE, "NaN returned by force evaluation");
+  return PDG_OK;
+}
+
 int pdg_state_download(pdg_ctx* h, double* u, double* v, double* a, double* bd, double* damage,
                        double* dilatation, double* force) {
   Ctx* c = reinterpret_cast<Ctx*>(h);

@@ -98,6 +98,14 @@ hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const
                                      NodeRec* rec_next, const VerletArgs& va, int lanes,
                                      hipStream_t s);
 
+// Persistent cooperative explicit run (pdg_tiles.hip): steps 1..nsteps
+// (plus the initial evaluation if initial_eval) in one launch. Ends with the
+// state in rec[(evaluations) % 2].
+hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0, NodeRec* rec1,
+                             float* theta, int* nbroken, const VerletArgs& va, int nsteps,
+                             double t0, int initial_eval, bool damage, int* grid_out,
+                             hipStream_t s);
+
 // Launchers (pdg_bonds.hip). `lanes` = lanes per node (16, 32 or 64).
 hipError_t launch_pass1(const BondArgs& a, NodeRec* rec, float* theta, int* nbroken,
                         bool damage, bool count_near, int lanes, hipStream_t s);

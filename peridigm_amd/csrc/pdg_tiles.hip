@@ -23,7 +23,13 @@
 // (same helpers), so results match the untiled kernels up to FP32 summation
 // order.
 
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+
 #include "pdg_device.hpp"
+
+namespace cg = cooperative_groups;
 
 namespace pdg {
 
@@ -39,6 +45,17 @@ constexpr int NWORDS = MAXDEG / 64;
 __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   const int per = (ntiles + 7) >> 3;
   return (bid & 7) * per + (bid >> 3);
+}
+
+// Tile for the q-th work item of a grid of nb workgroups striding over
+// ntiles. In a full round with nb a multiple of 8, workgroup b (dispatched to
+// XCD b % 8) takes a contiguous run of tiles; a short last round maps
+// identically. Both are bijections onto the round's tiles.
+__device__ __forceinline__ int xcd_tile_strided(int q, int nb, int ntiles) {
+  const int b = q % nb, base = q - b;
+  const int span = min(nb, ntiles - base);
+  if (span == nb && (nb & 7) == 0) return base + (b & 7) * (nb >> 3) + (b >> 3);
+  return base + b;
 }
 
 inline unsigned tile_grid(int ntiles) { return 8u * (unsigned)((ntiles + 7) >> 3); }
@@ -67,16 +84,24 @@ __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lco
   }
 }
 
+// LDS staging arrays of one workgroup (shared by both passes).
+struct TileLds {
+  double sx[3][HMAX];
+  double su[3][HMAX];
+  float svol[HMAX];
+  float salpha[HMAX];
+  float sp[HMAX];
+};
+
+// Pass 1 (damage + dilatation) of one tile; every thread of the workgroup
+// calls it with the same tile.
 template <int G, bool DAMAGE, bool NEAR, int INFL>
-__global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
-                                                    NodeRec* __restrict__ rec,
-                                                    float* __restrict__ theta,
-                                                    int* __restrict__ nbroken) {
-  __shared__ double sx[3][HMAX];
-  __shared__ double su[3][HMAX];
-  __shared__ float svol[HMAX];
-  const int tile = xcd_tile(blockIdx.x, t.ntiles);
-  if (tile >= t.ntiles) return;
+__device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
+                                           NodeRec* __restrict__ rec, float* __restrict__ theta,
+                                           int* __restrict__ nbroken, int tile, TileLds& L) {
+  double (&sx)[3][HMAX] = L.sx;
+  double (&su)[3][HMAX] = L.su;
+  float (&svol)[HMAX] = L.svol;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
   for (int h = threadIdx.x; h < hn; h += WG) {
@@ -161,20 +186,32 @@ __global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
       if (NEAR && nnear) atomicAdd(a.near_ctr, (unsigned long long)nnear);
     }
   }
+  __syncthreads();  // LDS is reused by the next tile
 }
 
-template <int G, bool VERLET, int INFL>
-__global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
-                                                    const NodeRec* __restrict__ rec,
-                                                    NodeRec* __restrict__ rec_next,
-                                                    double* __restrict__ fout, VerletArgs va) {
-  __shared__ double sx[3][HMAX];
-  __shared__ double su[3][HMAX];
-  __shared__ float svol[HMAX];
-  __shared__ float salpha[HMAX];
-  __shared__ float sp[HMAX];
+template <int G, bool DAMAGE, bool NEAR, int INFL>
+__global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
+                                                    NodeRec* __restrict__ rec,
+                                                    float* __restrict__ theta,
+                                                    int* __restrict__ nbroken) {
+  __shared__ TileLds L;
   const int tile = xcd_tile(blockIdx.x, t.ntiles);
   if (tile >= t.ntiles) return;
+  tile_pass1<G, DAMAGE, NEAR, INFL>(a, t, rec, theta, nbroken, tile, L);
+}
+
+// Pass 2 (both-ends force + Verlet epilogue) of one tile.
+template <int G, bool VERLET, int INFL>
+__device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
+                                           const NodeRec* __restrict__ rec,
+                                           NodeRec* __restrict__ rec_next,
+                                           double* __restrict__ fout, const VerletArgs& va,
+                                           int tile, TileLds& L) {
+  double (&sx)[3][HMAX] = L.sx;
+  double (&su)[3][HMAX] = L.su;
+  float (&svol)[HMAX] = L.svol;
+  float (&salpha)[HMAX] = L.salpha;
+  float (&sp)[HMAX] = L.sp;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
   for (int h = threadIdx.x; h < hn; h += WG) {
@@ -237,6 +274,54 @@ __global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
       else verlet_epilogue(va, node, l, fc, su[l][li], rec_next);
     }
   }
+  __syncthreads();  // LDS is reused by the next tile
+}
+
+template <int G, bool VERLET, int INFL>
+__global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
+                                                    const NodeRec* __restrict__ rec,
+                                                    NodeRec* __restrict__ rec_next,
+                                                    double* __restrict__ fout, VerletArgs va) {
+  __shared__ TileLds L;
+  const int tile = xcd_tile(blockIdx.x, t.ntiles);
+  if (tile >= t.ntiles) return;
+  tile_pass2<G, VERLET, INFL>(a, t, rec, rec_next, fout, va, tile, L);
+}
+
+// Persistent explicit run: one cooperative launch advances `nsteps` Verlet
+// steps; each step is pass 1 over all tiles, a grid barrier, pass 2 over all
+// tiles, a grid barrier. Workgroups stride over tiles in XCD-contiguous
+// order. Co-residency of the whole grid is guaranteed by the cooperative
+// launch, so the barriers always complete.
+template <int G, bool DAMAGE, int INFL>
+__global__ __launch_bounds__(WG) void k_persistent(BondArgs a, TileArgs t, NodeRec* rec0,
+                                                   NodeRec* rec1, float* theta, int* nbroken,
+                                                   VerletArgs va, int nsteps, double t0,
+                                                   int initial_eval) {
+  __shared__ TileLds L;
+  cg::grid_group grid = cg::this_grid();
+  NodeRec* rec[2] = {rec0, rec1};
+  int cur = 0;
+  const int nb = gridDim.x;
+  const int first = initial_eval ? 0 : 1;
+  for (int k = first; k <= nsteps; k++) {
+    for (int q = blockIdx.x; q < t.ntiles; q += nb) {
+      const int tile = xcd_tile_strided(q, nb, t.ntiles);
+      tile_pass1<G, DAMAGE, false, INFL>(a, t, rec[cur], theta, nbroken, tile, L);
+    }
+    grid.sync();
+    VerletArgs v = va;
+    v.end_kick = k > 0 ? 1 : 0;
+    v.start_step = k < nsteps ? 1 : 0;
+    v.t_cur = t0 + ((k + 1) * va.dt);
+    v.t_prev = t0 + (k * va.dt);
+    for (int q = blockIdx.x; q < t.ntiles; q += nb) {
+      const int tile = xcd_tile_strided(q, nb, t.ntiles);
+      tile_pass2<G, true, INFL>(a, t, rec[cur], rec[cur ^ 1], nullptr, v, tile, L);
+    }
+    grid.sync();
+    cur ^= 1;
+  }
 }
 
 template <int G, int INFL>
@@ -275,6 +360,36 @@ void p2_infl(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* 
 }
 
 }  // namespace
+
+hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0, NodeRec* rec1,
+                             float* theta, int* nbroken, const VerletArgs& va, int nsteps,
+                             double t0, int initial_eval, bool damage, int* grid_out,
+                             hipStream_t s) {
+  if (t.ntiles <= 0) return hipErrorInvalidValue;
+  void* fn;
+  if (a.influence == PDG_INFLUENCE_ONE)
+    fn = damage ? (void*)k_persistent<16, true, 0> : (void*)k_persistent<16, false, 0>;
+  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY)
+    fn = damage ? (void*)k_persistent<16, true, 1> : (void*)k_persistent<16, false, 1>;
+  else
+    fn = damage ? (void*)k_persistent<16, true, 2> : (void*)k_persistent<16, false, 2>;
+  int dev = 0, ncu = 0, per_cu = 0, coop = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+  if (e == hipSuccess && !coop) return hipErrorNotSupported;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, 0);
+  if (e != hipSuccess) return e;
+  int nb = std::min(per_cu * ncu, t.ntiles);
+  if (nb >= 8) nb &= ~7;
+  if (nb <= 0) return hipErrorInvalidConfiguration;
+  if (grid_out) *grid_out = nb;
+  BondArgs aa = a;
+  TileArgs tt = t;
+  VerletArgs vv = va;
+  void* args[] = {&aa, &tt, &rec0, &rec1, &theta, &nbroken, &vv, &nsteps, &t0, &initial_eval};
+  return hipLaunchCooperativeKernel(fn, dim3(nb), dim3(WG), args, 0, s);
+}
 
 hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta,
                               int* nbroken, bool damage, bool count_near, int lanes,

@@ -290,3 +290,58 @@ def test_full_plate_invariants():
         fV = (f * p["volume"][:, None]).sum(0)
         assert np.abs(fV).max() <= 1e-5 * np.abs(f).max() * p["volume"][0] * np.sqrt(len(x))
 
+
+
+# ------------------------------------------------ batched persistent launch
+def test_persistent_launch_bitwise_equals_per_step_launches():
+    from peridigm_amd import Context
+    from peridigm_amd.problems import plate
+    p = plate(nx=44, ny=40, nz=4, notch_cells=12, pull=17.3)
+    nl = O.filter_planes(p["x"], O.radius_search(p["x"], p["horizon"]), p["planes"])
+    outs = []
+    for persistent in (False, True):
+        with Context(0) as ctx:
+            ctx.set_topology(nl, len(p["volume"]))
+            ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
+            ctx.set_density(np.full(len(p["volume"]), p["density"]))
+            ctx.set_bcs(p["bcs"])
+            ctx.upload_state()
+            if persistent:
+                g = ctx.verlet_run_persistent(40, p["dt"])
+                assert g >= 1
+                ctx.verlet_run_persistent(25, p["dt"], t0=40 * p["dt"], initial_eval=False)
+            else:
+                ctx.verlet_run(40, p["dt"])
+                ctx.verlet_run(25, p["dt"], t0=40 * p["dt"], initial_eval=False)
+            outs.append(ctx.download_state())
+    for k in ("u", "v", "a", "bond_damage", "theta", "force"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_batched_specimens_match_oracle():
+    from peridigm_amd import Context
+    from peridigm_amd.problems import batched_plates
+    p = batched_plates(nspec=5, seed=7, nodes=(1500, 5000), notch=(5, 15), pull=17.3)
+    nl = O.filter_planes(p["x"], O.radius_search(p["x"], p["horizon"]), p["planes"])
+    nsteps = 80
+    sc = p["critical_stretch"]
+    r = O.explicit_run(p["x"], p["volume"], p["density"], nl, p["K"], p["mu"], p["horizon"],
+                       sc, p["dt"], nsteps, bcs=p["bcs"])
+    N = len(p["volume"])
+    with Context(0) as ctx:
+        ctx.set_topology(nl, N)
+        ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, float(sc.max()))
+        ctx.set_critical_stretch(sc)
+        ctx.set_density(np.full(N, p["density"]))
+        ctx.set_bcs(p["bcs"])
+        ctx.upload_state()
+        ctx.verlet_run_persistent(nsteps, p["dt"])
+        st = ctx.download_state()
+    nb = r["bond_damage"].sum()
+    mism = np.count_nonzero(st["bond_damage"] != r["bond_damage"])
+    du = st["u"] - r["u"]
+    print(f"specimens={len(p['specimens'])} N={N} broken={int(nb)} mismatch={mism} "
+          f"u_err max={np.abs(du).max() / np.abs(r['u']).max():.2e}")
+    assert nb > 0
+    assert mism <= max(2, 1e-3 * nb)
+    assert np.abs(du).max() <= 1e-4 * np.abs(r["u"]).max()
