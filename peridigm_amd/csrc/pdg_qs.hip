@@ -638,15 +638,8 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
   double rnorm = 0.0;
   std::vector<double> hr(n3), hreac(n3);
   for (;; nit++) {
-    if ((rc = qs::residual(c))) return rc;
-    // residual norm and reactions (force on fixed DOFs) for the tolerance
-    rc = c->hip(hipMemcpyAsync(hr.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
-    if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
-    if (rc) return rc;
-    double n2 = 0.0, ninf = 0.0;
-    for (double v : hr) { n2 += v * v; ninf = std::max(ninf, std::fabs(v)); }
-    rnorm = std::sqrt(n2) + 20.0 * ninf;
-    // reactions: f V on fixed DOFs (recomputed with fixed rows unzeroed)
+    // reactions (f V on kinematic DOFs) for the tolerance: residual with no
+    // rows zeroed; then the residual proper, which stays in S->out
     double reac = 0.0;
     {
       std::vector<signed char> none(n3, 0);
@@ -659,6 +652,13 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
       for (int q = 0; q < n3; q++) if (S->hfixed[q]) reac += hreac[q] * hreac[q];
       reac = std::sqrt(reac);
     }
+    if ((rc = qs::residual(c))) return rc;
+    rc = c->hip(hipMemcpyAsync(hr.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
+    if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    double n2 = 0.0, ninf = 0.0;
+    for (double v : hr) { n2 += v * v; ninf = std::max(ninf, std::fabs(v)); }
+    rnorm = std::sqrt(n2) + 20.0 * ninf;
     if (rnorm <= tol * std::max(1.0, reac)) { converged = true; break; }
     if (nit == maxNewton) break;
     if (precond == 1 && (rc = qs::block_jacobi(c))) return rc;
