@@ -87,11 +87,12 @@ def cpu_baseline(steps=20):
                        f"(damage, dilatation, scatter force), {per_step*1e3:.1f} ms/step")
 
 
-def qs_figure(nsamples=200, epochs=20):
+def qs_figure(nsamples=100, epochs=20):
     """Quasi-static tensile bar 60x20x20 (24k nodes), 10 load steps to 1e-3
     strain, every linear solve to 1e-8 relative residual: wall time with
     block-Jacobi CG and with the learned preconditioner in flexible CG (trained
-    on the device on `nsamples` block-Jacobi solves under random end loads)."""
+    on the device on residual/correction pairs from `nsamples` block-Jacobi
+    solves under random end loads, two intermediate iterates each)."""
     from peridigm_amd import Context
     from peridigm_amd import learned as Lr
     from peridigm_amd import quasistatic as Q
@@ -112,15 +113,17 @@ def qs_figure(nsamples=200, epochs=20):
         loss = Lr.train(ctx, R, E, epochs=epochs, lr=2e-3, seed=3)
         t_train = time.perf_counter() - t0
         u_l, st_l, wall_l = Q.run_tension(ctx, p, precond="learned", tol=1e-6, lin_tol=1e-8)
+        fallbacks = Q.fallback_count(ctx)
     out = {
         "workload": "QS tensile bar 60x20x20 (N=%d), 10 load steps to 1e-3 strain, "
                     "linear solves to 1e-8 relative residual" % N,
         "jacobi_pcg_s": wall_j, "learned_fcg_s": wall_l, "speedup": wall_j / wall_l,
         "jacobi_linear_iters": int(sum(s["linear"] for s in st_j)),
         "learned_linear_iters": int(sum(s["linear"] for s in st_l)),
+        "learned_fallbacks_to_jacobi": fallbacks,
         "newton_iters": [int(sum(s["newton"] for s in st_j)), int(sum(s["newton"] for s in st_l))],
         "max_u_diff_rel": float(np.abs(u_l - u_j).max() / np.abs(u_j).max()),
-        "training": {"samples": nsamples, "epochs": epochs, "final_rel_loss": loss,
+        "training": {"load_samples": nsamples, "pairs": int(len(R)), "epochs": epochs, "final_rel_loss": loss,
                      "data_s": t_gen, "train_s": t_train},
     }
     return out
@@ -133,7 +136,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--workload", default="plate")
     ap.add_argument("--lanes", type=int, default=16)
-    ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled")
+    ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled, 2: tiles with plain loops")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qs", action="store_true",
                     help="also measure the quasi-static solve (block-Jacobi CG vs learned FCG)")
@@ -223,7 +226,8 @@ def main():
                            "notch U[10h,60h], one node/bond space",
             }[args.workload],
             "nodes": N, "bonds": B, "lanes_per_node": args.lanes,
-            "kernels": "lds-tiled" if (args.variant == 0 and ntiles > 0) else "untiled",
+            "kernels": ("untiled" if (args.variant == 1 or ntiles == 0) else
+                        "lds-tiled" + (", plain loops" if args.variant == 2 else ", pipelined loops")),
             "tiles": ntiles, "halo_per_node": halo_total / N,
             "bond_arithmetic": "fp32 compensated", "integration": "fp64",
             "broken_bonds_end": int(st["bond_damage"].sum()),

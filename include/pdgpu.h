@@ -58,7 +58,8 @@ int pdg_set_lanes_per_node(pdg_ctx* ctx, int lanes);
 
 /* Bond-kernel variant: 0 (default) LDS-staged Morton tiles when every tile
  * halo fits in LDS, else the untiled kernels; 1 forces the untiled kernels
- * (one 64-byte neighbour-record gather per bond). */
+ * (one 64-byte neighbour-record gather per bond); 2 is variant 0 with plain
+ * (branching, unpipelined) bond loops, same arithmetic. */
 int pdg_set_kernel_variant(pdg_ctx* ctx, int variant);
 /* Number of Morton tiles (0: tiling unavailable) and total halo entries. */
 int pdg_get_tiling(const pdg_ctx* ctx, int* numTiles, long long* haloTotal);
@@ -174,6 +175,9 @@ int pdg_qs_solve(pdg_ctx* ctx, const double* u, const double* rhs, double* x, in
 int pdg_qs_newton(pdg_ctx* ctx, double* u, double tol, int maxNewton, int precond,
                   double linTol, int maxLin, int* newtonIters, int* linIters,
                   double* residualNorm);
+/* Number of learned-preconditioner solves that stalled (residual not halved
+ * in 50 iterations) and were finished by block-Jacobi CG, the fallback. */
+int pdg_qs_fallback_count(pdg_ctx* ctx, int* count);
 
 /* ---- learned preconditioner (no counterpart in the reference) -----------
  * A 3-layer message-passing network (width 32, BF16 MFMA, FP32 accumulate)

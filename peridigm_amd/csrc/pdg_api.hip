@@ -77,7 +77,7 @@ void Ctx::free_device() {
 hipError_t Ctx::pass1(NodeRec* rec, int* nbroken, bool damage, bool count_near) {
   if (tiled())
     return launch_pass1_tiled(bond_args(), tile_args(), rec, d_theta, nbroken, damage, count_near,
-                              lanes, stream);
+                              lanes, pipe(), stream);
   return launch_pass1(bond_args(), rec, d_theta, nbroken, damage, count_near, lanes, stream);
 }
 
@@ -244,7 +244,8 @@ int pdg_set_lanes_per_node(pdg_ctx* h, int lanes) {
 int pdg_set_kernel_variant(pdg_ctx* h, int variant) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
-  if (variant != 0 && variant != 1) return c->fail(PDG_E_ARG, "variant must be 0 (tiled) or 1 (untiled)");
+  if (variant < 0 || variant > 2)
+    return c->fail(PDG_E_ARG, "variant must be 0 (tiled), 1 (untiled) or 2 (tiled, plain loops)");
   c->variant = variant;
   return PDG_OK;
 }
@@ -584,7 +585,7 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
                        c->d_rec[c->cur], c->L);
   if (!rc) rc = c->upload_mask(bdNP1, true);
   if (!rc) rc = c->hip(c->pass1(c->d_rec[c->cur], nullptr, false, false), "pass1");
-  if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s)
+  if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->lanes, c->pipe(), s)
                                   : launch_pass2_force(c->bond_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s),
                        "pass2");
   if (!rc) rc = c->download_vec3(c->d_fout, force);
@@ -668,7 +669,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   BondArgs ba = c->bond_args();
   VerletArgs va{};
   va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
-  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.err = c->d_err;
+  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.inv_L = 1.0 / c->L; va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
   if (rc) return rc;
   c->t_pass1 = c->t_pass2 = 0.0;
@@ -696,7 +697,8 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
     va.start_step = k < nsteps ? 1 : 0;
     va.t_cur = step_time(k + 1);
     va.t_prev = step_time(k);
-    if (!r) r = c->hip(c->tiled() ? launch_pass2_tiled_verlet(ba, c->tile_args(), c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, s)
+    va.inv_dtbc = 1.0 / (va.t_cur - va.t_prev);
+    if (!r) r = c->hip(c->tiled() ? launch_pass2_tiled_verlet(ba, c->tile_args(), c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, c->pipe(), s)
                                   : launch_pass2_verlet(ba, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, s),
                        "pass2");
     if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 2], s), "event");
@@ -709,6 +711,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   } else if (nsteps > 0) {
     va.t_cur = step_time(1);
     va.t_prev = step_time(0);
+    va.inv_dtbc = 1.0 / (va.t_cur - va.t_prev);
     rc = c->hip(launch_verlet_prologue(c->N, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, s), "prologue");
     c->cur ^= 1;
   }
@@ -742,13 +745,14 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   hipStream_t s = c->stream;
   VerletArgs va{};
   va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
-  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.err = c->d_err;
+  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.inv_L = 1.0 / c->L; va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
   if (rc) return rc;
   // a continuing run starts step 1 from the stored acceleration
   if (!initialEval) {
     va.t_cur = t0 + (1 * dt);
     va.t_prev = t0 + (0 * dt);
+    va.inv_dtbc = 1.0 / (va.t_cur - va.t_prev);
     rc = c->hip(launch_verlet_prologue(c->N, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, s), "prologue");
     if (rc) return rc;
     c->cur ^= 1;

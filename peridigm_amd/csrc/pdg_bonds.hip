@@ -182,11 +182,11 @@ __global__ __launch_bounds__(256) void k_verlet_prologue(int n, const NodeRec* _
     const double cur = c1 * va.t_cur + c0;
     double prev = c1 * va.t_prev + c0;
     if (cur - prev == 0.0) prev = u;
-    v = 0.0 * (cur - prev) + 1.0 * (cur - prev) * (1.0 / (va.t_cur - va.t_prev));
+    v = 0.0 * (cur - prev) + 1.0 * (cur - prev) * va.inv_dtbc;
   }
   u = u + va.dt * v;
   va.v[q] = v;
-  rec_next[node].u[l] = u * (1.0 / va.L);
+  rec_next[node].u[l] = u * va.inv_L;
 }
 
 // Weighted volume in the reference's own summation order (one thread per

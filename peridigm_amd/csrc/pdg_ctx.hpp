@@ -65,7 +65,7 @@ struct Ctx {
 
   // LDS-staged Morton tiles (empty if a node's neighbourhood exceeds the
   // halo capacity; the untiled kernels are used then)
-  int variant = 0;  // 0: tiled when available, 1: untiled
+  int variant = 0;  // 0: tiled (pipelined loops), 1: untiled, 2: tiled (plain loops)
   int ntiles = 0;
   std::vector<int> tile_start, halo_off, halo, own_off;
   std::vector<unsigned short> lcol;
@@ -74,7 +74,8 @@ struct Ctx {
   int* d_halo = nullptr;
   int* d_own_off = nullptr;
   unsigned short* d_lcol = nullptr;
-  bool tiled() const { return variant == 0 && ntiles > 0; }
+  bool tiled() const { return variant != 1 && ntiles > 0; }
+  bool pipe() const { return variant != 2; }
   TileArgs tile_args() const;
   int build_tiles();
 
@@ -100,6 +101,10 @@ struct Ctx {
 int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int maxit, int* iters,
              double* rel);
 int gnn_apply(Ctx* c, const double* r, double* z);
+// K w at the prepared tangent state (device vectors, internal order; kinematic
+// rows return w) and a device dot product read back to the host.
+int qs_tangent_apply(Ctx* c, const double* w, double* out);
+int qs_dot(Ctx* c, const double* x, const double* y, double* out);
 // uploads u (reference order) and forms the block-Jacobi inverse at u
 int pdg_qs_prepare(Ctx* c, const double* u);
 

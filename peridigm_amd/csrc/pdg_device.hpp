@@ -108,7 +108,7 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
       const double cur = c1 * va.t_cur + c0;
       double prev = c1 * va.t_prev + c0;
       if (cur - prev == 0.0) prev = u;
-      v = 0.0 * (cur - prev) + 1.0 * (cur - prev) * (1.0 / (va.t_cur - va.t_prev));
+      v = 0.0 * (cur - prev) + 1.0 * (cur - prev) * va.inv_dtbc;
     }
     u = u + va.dt * v;
   } else {
@@ -116,7 +116,7 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
     va.f[q] = f;
   }
   va.v[q] = v;
-  rec_next[node].u[l] = u * (1.0 / va.L);
+  rec_next[node].u[l] = u * va.inv_L;
 }
 
 }  // namespace dev

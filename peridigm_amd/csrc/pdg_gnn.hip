@@ -67,9 +67,10 @@ struct GnnState {
   float* gbuf[2] = {};        // N x 32 fp32 gradients
   float* gagg = nullptr;      // N x 32
   float* part = nullptr;      // per-tile partial dW/db
-  float* target = nullptr;    // N x 3
   float* scales = nullptr;    // [s_r, s_z, loss]
   double* z0 = nullptr;       // D^-1 r (3N)
+  double* dz = nullptr;       // z - e (3N), training
+  double* kdz = nullptr;      // K (z - e) (3N), training
   double* red = nullptr;      // reduction partials
   int nred = 0;
   int step = 0;
@@ -409,43 +410,49 @@ __global__ void k_output(int n3, const double* __restrict__ z0, const float* __r
 }
 
 // target t = (e - z0) / s_z and loss gradient g = 2 (o - t) / sum(t^2)
-__global__ void k_target(int n3, const double* __restrict__ e, const double* __restrict__ z0,
-                         const signed char* __restrict__ fixed, const float* __restrict__ scales,
-                         float* __restrict__ t) {
+// Training loss: squared energy-norm error of the preconditioned step,
+//   L = (z - e)^T K (z - e) / (e^T K e),  z = D^-1 r + s_z o,  e = K^-1 r,
+// so dL/do = 2 s_z K (z - e) / (e^T K e) on free DOFs (K symmetric).
+__global__ void k_step_error(int n3, const double* __restrict__ e, const double* __restrict__ z0,
+                             const float* __restrict__ o, const signed char* __restrict__ fixed,
+                             const float* __restrict__ scales, double* __restrict__ dz) {
   const int q = blockIdx.x * 256 + threadIdx.x;
   if (q >= n3) return;
-  const float iz = scales[1] > 0.0f ? 1.0f / scales[1] : 0.0f;
-  t[q] = fixed[q] ? 0.0f : (float)((e[q] - z0[q]) * (double)iz);
+  dz[q] = fixed[q] ? 0.0 : z0[q] + (double)scales[1] * (double)o[q] - e[q];
 }
 
-__global__ __launch_bounds__(256) void k_loss_grad(int n3, const float* __restrict__ o,
-                                                   const float* __restrict__ t,
+__global__ void k_masked_copy(int n3, const double* __restrict__ src,
+                              const signed char* __restrict__ fixed, double* __restrict__ dst) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < n3) dst[q] = fixed[q] ? 0.0 : src[q];
+}
+
+__global__ __launch_bounds__(256) void k_loss_grad(int n3, const double* __restrict__ dz,
+                                                   const double* __restrict__ kdz,
                                                    const signed char* __restrict__ fixed,
-                                                   float* __restrict__ g, double* red, int nred) {
-  __shared__ double sh[2][4];
-  double se = 0.0, st = 0.0;
+                                                   float* __restrict__ g, double* red) {
+  __shared__ double sh[4];
+  double se = 0.0;
   for (int q = blockIdx.x * 256 + threadIdx.x; q < n3; q += gridDim.x * 256) {
-    const float d = fixed[q] ? 0.0f : o[q] - t[q];
-    se += (double)d * d;
-    st += (double)t[q] * t[q];
-    g[q] = d;  // scaled below once sum(t^2) is known
+    const double k = fixed[q] ? 0.0 : kdz[q];
+    se += dz[q] * k;
+    g[q] = (float)k;  // scaled below once the loss normaliser is known
   }
-  for (int k = 32; k > 0; k >>= 1) { se += __shfl_xor(se, k); st += __shfl_xor(st, k); }
+  for (int k = 32; k > 0; k >>= 1) se += __shfl_xor(se, k);
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { sh[0][w] = se; sh[1][w] = st; }
+  if ((threadIdx.x & 63) == 0) sh[w] = se;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    red[blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
-    red[nred + blockIdx.x] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
-  }
+  if (threadIdx.x == 0) red[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-__global__ void k_loss_final(int nred, const double* __restrict__ red, float* scales) {
+// scales[2] = relative energy-norm error sqrt(L); scales[3] = 2 s_z / e^T K e
+__global__ void k_loss_final(int nred, const double* __restrict__ red, double eke, float* scales) {
   if (threadIdx.x != 0) return;
-  double a = 0.0, b = 0.0;
-  for (int k = 0; k < nred; k++) { a += red[k]; b += red[nred + k]; }
-  scales[2] = (float)(a / (b > 0.0 ? b : 1.0));
-  scales[3] = (float)(b > 0.0 ? 2.0 / b : 0.0);
+  double a = 0.0;
+  for (int k = 0; k < nred; k++) a += red[k];
+  const double inv = eke > 0.0 ? 1.0 / eke : 0.0;
+  scales[2] = (float)std::sqrt(std::max(a * inv, 0.0));
+  scales[3] = (float)(2.0 * (double)scales[1] * inv);
 }
 
 __global__ void k_scale_grad(int n3, float* __restrict__ g, const float* __restrict__ scales) {
@@ -493,9 +500,10 @@ static int ensure(Ctx* c, unsigned seed) {
   if (!rc) rc = A((void**)&G->gbuf[1], 4 * (size_t)N * 32);
   if (!rc) rc = A((void**)&G->gagg, 4 * (size_t)N * 32);
   if (!rc) rc = A((void**)&G->part, 4 * (size_t)G->ntiles * (2 * 32 * 32 + 32));
-  if (!rc) rc = A((void**)&G->target, 4 * (size_t)N * 3);
   if (!rc) rc = A((void**)&G->scales, 64);
   if (!rc) rc = A((void**)&G->z0, 24 * (size_t)N);
+  if (!rc) rc = A((void**)&G->dz, 24 * (size_t)N);
+  if (!rc) rc = A((void**)&G->kdz, 24 * (size_t)N);
   if (!rc) rc = A((void**)&G->red, 16 * (size_t)G->nred);
   if (rc) return rc;
   // Glorot-style initial weights; the last layer starts at zero so the
@@ -576,7 +584,7 @@ void Ctx::free_gnn() {
   if (!gnn) return;
   GnnState* G = gnn;
   void* ptrs[] = {G->params, G->grads, G->adam_m, G->adam_v, G->h[0], G->h[1], G->h[2], G->out,
-                  G->gbuf[0], G->gbuf[1], G->gagg, G->part, G->target, G->scales, G->z0, G->red};
+                  G->gbuf[0], G->gbuf[1], G->gagg, G->part, G->scales, G->z0, G->red, G->dz, G->kdz};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   for (int l = 0; l < gnn::NL; l++) {
     if (G->wt[l]) (void)hipFree(G->wt[l]);
@@ -704,6 +712,14 @@ int pdg_gnn_train(pdg_ctx* h, const double* u, int nsamples, const double* r, co
       }
     }
   }
+  // energy norms e^T K e of the targets (loss normalisers)
+  std::vector<double> eke(nsamples, 0.0);
+  for (int smp = 0; smp < nsamples && !rc; smp++) {
+    const double* es = de + (size_t)smp * n3;
+    hipLaunchKernelGGL(k_masked_copy, dim3(nblk(n3)), dim3(256), 0, s, n3, es, c->qs_fixed(), G->dz);
+    rc = qs_tangent_apply(c, G->dz, G->kdz);
+    if (!rc) rc = qs_dot(c, G->dz, G->kdz, &eke[smp]);
+  }
   std::mt19937 gen(seed);
   std::vector<int> order(nsamples);
   for (int k = 0; k < nsamples; k++) order[k] = k;
@@ -719,9 +735,11 @@ int pdg_gnn_train(pdg_ctx* h, const double* u, int nsamples, const double* r, co
       if (!rc) rc = gnnh::features(c, rs);
       if (!rc) rc = gnnh::forward(c);
       if (rc) break;
-      hipLaunchKernelGGL(k_target, dim3(nblk(n3)), dim3(256), 0, s, n3, es, G->z0, c->qs_fixed(), G->scales, G->target);
-      hipLaunchKernelGGL(k_loss_grad, dim3(G->nred), dim3(256), 0, s, n3, G->out, G->target, c->qs_fixed(), G->gbuf[1], G->red, G->nred);
-      hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, s, G->nred, G->red, G->scales);
+      hipLaunchKernelGGL(k_step_error, dim3(nblk(n3)), dim3(256), 0, s, n3, es, G->z0, G->out, c->qs_fixed(), G->scales, G->dz);
+      rc = qs_tangent_apply(c, G->dz, G->kdz);
+      if (rc) break;
+      hipLaunchKernelGGL(k_loss_grad, dim3(G->nred), dim3(256), 0, s, n3, G->dz, G->kdz, c->qs_fixed(), G->gbuf[1], G->red);
+      hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, s, G->nred, G->red, eke[order[q]], G->scales);
       hipLaunchKernelGGL(k_scale_grad, dim3(nblk(n3)), dim3(256), 0, s, n3, G->gbuf[1], G->scales);
       rc = gnnh::backward(c);
       if (rc) break;

@@ -68,6 +68,8 @@ struct VerletArgs {
   const int* bc_slot; // n, -1 if none
   const BcSlot* bcs;
   double dt, t_cur, t_prev, L;
+  double inv_dtbc;    // 1.0 / (t_cur - t_prev), rounded once like the reference's division
+  double inv_L;       // 1.0 / L
   int end_kick;       // finish the current step: v += dt/2 a
   int start_step;     // begin the next step: v += dt/2 a, BC, u += dt v
   int* err;           // set to 1 on a non-finite force
@@ -90,12 +92,12 @@ struct TileArgs {
 };
 
 hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta,
-                              int* nbroken, bool damage, bool count_near, int lanes,
+                              int* nbroken, bool damage, bool count_near, int lanes, bool pipe,
                               hipStream_t s);
 hipError_t launch_pass2_tiled_force(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
-                                    double* f, int lanes, hipStream_t s);
+                                    double* f, int lanes, bool pipe, hipStream_t s);
 hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
-                                     NodeRec* rec_next, const VerletArgs& va, int lanes,
+                                     NodeRec* rec_next, const VerletArgs& va, int lanes, bool pipe,
                                      hipStream_t s);
 
 // Persistent cooperative explicit run (pdg_tiles.hip): steps 1..nsteps

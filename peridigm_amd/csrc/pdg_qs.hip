@@ -40,6 +40,8 @@ struct QsState {
   double *xs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *b = nullptr,
          *part = nullptr, *scal = nullptr;
   int nblk = 0;
+  double* hscal = nullptr;  // pinned host copy of the solver scalars
+  int fallbacks = 0;  // learned-preconditioner solves handed over to block-Jacobi CG
   std::vector<signed char> hfixed;
 };
 
@@ -292,9 +294,18 @@ __global__ __launch_bounds__(RB) void k_block_apply(int n, const double* dinv, c
 }
 
 // x += al p; r -= al q; partial sums of r.r
-__global__ __launch_bounds__(RB) void k_cg_update(int n, double al, const double* p, const double* q,
-                                                  double* x, double* r, double* part) {
+// Solver scalars kept on the device (slots of `sc`), so a CG iteration needs
+// no host round trip: RZ = r.z, PQ = p.Kp, RR = r.r, RZN = new r.z, ZQ = z.Kp,
+// BETA, BRK = 1 after a breakdown (p.Kp <= 0).
+enum { SC_RZ = 0, SC_PQ = 1, SC_RR = 2, SC_RZN = 3, SC_ZQ = 4, SC_BETA = 5, SC_BRK = 6 };
+
+// x += al p, r -= al q, partial sums of r.r; al = RZ / PQ (0 after a breakdown)
+__global__ __launch_bounds__(RB) void k_cg_step(int n, const double* sc, const double* p,
+                                                const double* q, double* x, double* r,
+                                                double* part) {
   __shared__ double sh[RB / 64];
+  const double pq = sc[SC_PQ];
+  const double al = pq > 0.0 ? sc[SC_RZ] / pq : 0.0;
   double s = 0.0;
   for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
     x[i] += al * p[i];
@@ -304,6 +315,21 @@ __global__ __launch_bounds__(RB) void k_cg_update(int n, double al, const double
   }
   s = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// beta = RZN / RZ (CG) or -ZQ / PQ (flexible CG, one-term truncation); RZ = RZN
+__global__ void k_cg_beta(double* sc, int flexible) {
+  if (threadIdx.x != 0) return;
+  const double pq = sc[SC_PQ];
+  if (!(pq > 0.0)) sc[SC_BRK] = 1.0;
+  sc[SC_BETA] = flexible ? -sc[SC_ZQ] / pq : sc[SC_RZN] / sc[SC_RZ];
+  sc[SC_RZ] = sc[SC_RZN];
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(RB) void k_cg_dir(int n, const double* sc, const double* z, double* p) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) p[i] = z[i] + sc[SC_BETA] * p[i];
 }
 
 __global__ __launch_bounds__(RB) void k_axpby(int n, double a, const double* x, double b, double* y) {
@@ -352,6 +378,7 @@ static int ensure(Ctx* c) {
   if (!rc) rc = A((void**)&S->dinv, 72 * (size_t)N);
   if (!rc) rc = A((void**)&S->part, 8 * (size_t)S->nblk * 2);
   if (!rc) rc = A((void**)&S->scal, 64);
+  if (!rc) rc = c->hip(hipHostMalloc((void**)&S->hscal, 64, hipHostMallocDefault), "pinned alloc");
   if (!rc) rc = A((void**)&S->fixed, 3 * (size_t)N);
   if (rc) return rc;
   // constants in internal order
@@ -433,6 +460,9 @@ static int dot(Ctx* c, const double* x, const double* y, double* out) {
 
 }  // namespace qs
 
+int qs_tangent_apply(Ctx* c, const double* w, double* out) { return qs::tangent(c, w, out, true); }
+int qs_dot(Ctx* c, const double* x, const double* y, double* out) { return qs::dot(c, x, y, out); }
+
 double* Ctx::qs_dinv() { return qs ? qs->dinv : nullptr; }
 const signed char* Ctx::qs_fixed() { return qs ? qs->fixed : nullptr; }
 
@@ -452,6 +482,7 @@ void Ctx::free_extra() {
                       S->out, S->dinv, S->xs, S->r, S->z, S->p, S->q, S->b, S->part, S->scal};
     for (double* p : ptrs) if (p) (void)hipFree(p);
     if (S->fixed) (void)hipFree(S->fixed);
+    if (S->hscal) (void)hipHostFree(S->hscal);
     delete S;
     qs = nullptr;
   }
@@ -469,6 +500,7 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   hipStream_t s = c->stream;
   const unsigned vb = (unsigned)((n3 + RB - 1) / RB);
   const unsigned nb = (unsigned)((c->N + RB - 1) / RB);
+  const dim3 red(S->nblk), blk(RB);
   int rc = 0;
   double bb = 0.0;
   if ((rc = qs::dot(c, b, b, &bb))) return rc;
@@ -479,52 +511,77 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     *rel = 0.0;
     return rc;
   }
-  // r = b - K x
-  if ((rc = qs::tangent(c, x, S->q, false))) return rc;
-  hipLaunchKernelGGL(k_sub, dim3(vb), dim3(RB), 0, s, n3, b, S->q, S->r);
+  // device dot product into scalar slot k
+  auto ddot = [&](const double* u, const double* v, int k) {
+    hipLaunchKernelGGL(k_dot, red, blk, 0, s, n3, u, v, S->part);
+    hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, S->part, 1, S->scal + k);
+  };
   auto precond_apply = [&](const double* r, double* z) -> int {
     if (precond == 2) return gnn_apply(c, r, z);
     if (precond == 1) {
-      hipLaunchKernelGGL(k_block_apply, dim3(nb), dim3(RB), 0, s, c->N, S->dinv, r, z);
+      hipLaunchKernelGGL(k_block_apply, dim3(nb), blk, 0, s, c->N, S->dinv, r, z);
       return c->hip(hipGetLastError(), "block apply");
     }
     return c->hip(hipMemcpyAsync(z, r, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
   };
-  if ((rc = precond_apply(S->r, S->z))) return rc;
-  rc = c->hip(hipMemcpyAsync(S->p, S->z, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
-  double rz = 0.0, rr = 0.0;
-  if (!rc) rc = qs::dot(c, S->r, S->z, &rz);
-  if (!rc) rc = qs::dot(c, S->r, S->r, &rr);
+  // r = b - K x; z = M r; p = z
+  if ((rc = qs::tangent(c, x, S->q, false))) return rc;
+  hipLaunchKernelGGL(k_sub, dim3(vb), blk, 0, s, n3, b, S->q, S->r);
+  rc = c->hip(hipMemsetAsync(S->scal, 0, 64, s), "zero scalars");
+  if (!rc) rc = precond_apply(S->r, S->z);
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->p, S->z, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
   if (rc) return rc;
-  double relres = std::sqrt(rr) / bnorm;
+  ddot(S->r, S->z, SC_RZ);
+  ddot(S->r, S->r, SC_RR);
+  // host reads RR and BRK at checks only: every iteration for the first 8,
+  // then every 4 (a solve may run up to 3 iterations past the tolerance)
+  auto check = [&](double* relres, bool* brk) -> int {
+    int r2 = c->hip(hipMemcpyAsync(S->hscal, S->scal, 64, hipMemcpyDeviceToHost, s), "d2h");
+    if (!r2) r2 = c->hip(hipStreamSynchronize(s), "sync");
+    *relres = std::sqrt(S->hscal[SC_RR]) / bnorm;
+    *brk = S->hscal[SC_BRK] != 0.0;
+    return r2;
+  };
+  double relres = 0.0;
+  bool brk = false;
+  if ((rc = check(&relres, &brk))) return rc;
   int it = 0;
+  // stagnation watch for the learned preconditioner: every 48 iterations the
+  // residual must have halved, else flexible CG hands over to block-Jacobi CG
+  double watch = relres;
+  bool stalled = false;
   while (relres > tol && it < maxit) {
     if ((rc = qs::tangent(c, S->p, S->q, true))) return rc;
-    double pq = 0.0;
-    if ((rc = qs::dot(c, S->p, S->q, &pq))) return rc;
-    if (!(pq > 0.0)) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
-    const double al = rz / pq;
-    hipLaunchKernelGGL(k_cg_update, dim3(S->nblk), dim3(RB), 0, s, n3, al, S->p, S->q, x, S->r, S->part);
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, s, S->nblk, S->part, 1, S->scal);
-    rc = c->hip(hipMemcpyAsync(&rr, S->scal, 8, hipMemcpyDeviceToHost, s), "d2h");
-    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
-    if (rc) return rc;
+    ddot(S->p, S->q, SC_PQ);
+    hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->scal, S->p, S->q, x, S->r, S->part);
+    hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, S->part, 1, S->scal + SC_RR);
     it++;
-    relres = std::sqrt(rr) / bnorm;
-    if (relres <= tol) break;
-    if ((rc = precond_apply(S->r, S->z))) return rc;
-    double rz_new = 0.0, beta = 0.0;
-    if ((rc = qs::dot(c, S->r, S->z, &rz_new))) return rc;
-    if (precond == 2) {
-      // flexible CG (one-term truncation): beta = -(z . q) / (p . q)
-      double zq = 0.0;
-      if ((rc = qs::dot(c, S->z, S->q, &zq))) return rc;
-      beta = -zq / pq;
-    } else {
-      beta = rz_new / rz;
+    if (it <= 8 || it % 4 == 0 || it == maxit) {
+      if ((rc = check(&relres, &brk))) return rc;
+      if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
+      if (relres <= tol) break;
+      if (precond == 2 && it % 48 == 0) {
+        if (!(relres < 0.5 * watch)) { stalled = true; break; }
+        watch = relres;
+      }
     }
-    rz = rz_new;
-    hipLaunchKernelGGL(k_axpby, dim3(vb), dim3(RB), 0, s, n3, 1.0, S->z, beta, S->p);
+    if ((rc = precond_apply(S->r, S->z))) return rc;
+    ddot(S->r, S->z, SC_RZN);
+    if (precond == 2) ddot(S->z, S->q, SC_ZQ);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, S->scal, precond == 2 ? 1 : 0);
+    hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, S->p);
+  }
+  if (!stalled && relres > tol) {
+    if ((rc = check(&relres, &brk))) return rc;
+    if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
+  }
+  if (stalled) {
+    // fallback: block-Jacobi CG from the current iterate
+    c->qs->fallbacks++;
+    int it2 = 0;
+    rc = qs_solve(c, x, b, 1, tol, std::max(0, maxit - it), &it2, rel);
+    *iters = it + it2;
+    return rc;
   }
   *iters = it;
   *rel = relres;
@@ -569,6 +626,13 @@ int pdg_qs_residual(pdg_ctx* h, const double* u, double* residual, double* norm)
   for (double v : r) { n2 += v * v; ninf = std::max(ninf, std::fabs(v)); }
   if (residual) std::copy(r.begin(), r.end(), residual);
   if (norm) *norm = std::sqrt(n2) + 20.0 * ninf;
+  return PDG_OK;
+}
+
+int pdg_qs_fallback_count(pdg_ctx* h, int* count) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !count) return PDG_E_ARG;
+  *count = c->qs ? c->qs->fallbacks : 0;
   return PDG_OK;
 }
 

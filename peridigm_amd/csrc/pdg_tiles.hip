@@ -91,11 +91,31 @@ struct TileLds {
   float svol[HMAX];
   float salpha[HMAX];
   float sp[HMAX];
+  float sf[3 * kTileNodesMax];  // pass 2: the tile's forces, for the Verlet epilogue
 };
+
+// One neighbour's staged data, loaded one bond ahead of its use by the
+// pipelined bond loops so LDS latency overlaps the previous bond's math.
+template <bool FORCE>
+struct Nb {
+  double x0, x1, x2, u0, u1, u2;
+  float vol, alpha, p;
+};
+
+template <bool FORCE>
+__device__ __forceinline__ Nb<FORCE> nb_load(const TileLds& L, int j) {
+  Nb<FORCE> n;
+  n.x0 = L.sx[0][j]; n.x1 = L.sx[1][j]; n.x2 = L.sx[2][j];
+  n.u0 = L.su[0][j]; n.u1 = L.su[1][j]; n.u2 = L.su[2][j];
+  n.vol = L.svol[j];
+  if (FORCE) { n.alpha = L.salpha[j]; n.p = L.sp[j]; }
+  else { n.alpha = 0.0f; n.p = 0.0f; }
+  return n;
+}
 
 // Pass 1 (damage + dilatation) of one tile; every thread of the workgroup
 // calls it with the same tile.
-template <int G, bool DAMAGE, bool NEAR, int INFL>
+template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
 __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
                                            NodeRec* __restrict__ rec, float* __restrict__ theta,
                                            int* __restrict__ nbroken, int tile, TileLds& L) {
@@ -141,6 +161,39 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     const int nmax = wave_max((deg + G - 1) / G);
     float acc = 0.0f;
     int nnear = 0;
+    if (PIPE) {
+      // branch-free bond loop: every lane computes, inactive and broken
+      // bonds are selected away; the next bond's LDS data is in flight
+      Nb<false> nb[2];
+      nb[0] = nb_load<false>(L, lj[0]);
+#pragma unroll
+      for (int it = 0; it < MAXDEG / G; ++it) {
+        if (it < nmax) {
+          if (it + 1 < MAXDEG / G && it + 1 < nmax)
+            nb[(it + 1) & 1] = nb_load<false>(L, lj[(it + 1) % (MAXDEG / G)]);
+          const Nb<false>& cur = nb[it & 1];
+          const int k = it * G + l;
+          const bool act = k < deg;
+          bool brk = act && (((mo[(it * G) >> 6] >> (k & 63)) & 1ull) != 0ull);
+          const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
+                                               cur.x2, cur.u0, cur.u1, cur.u2);
+          if (DAMAGE) {
+            const float lim = scz * b.zeta;
+            if (NEAR && act && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
+            brk = brk || (act && b.ext > lim);
+          }
+          const float w = omega_t<INFL>(b.zeta, c.hhat);
+          const float nacc = fmaf(w * b.zeta * b.ext, cur.vol, acc);
+          acc = (act && !brk) ? nacc : acc;
+          if (DAMAGE) {
+            const unsigned long long bal = __ballot(brk);
+            const unsigned long long gb =
+                G == 64 ? bal : ((bal >> (g * G)) & ((1ull << (G & 63)) - 1ull));
+            mn[(it * G) >> 6] |= gb << ((it * G) & 63);
+          }
+        }
+      }
+    } else {
 #pragma unroll
     for (int it = 0; it < MAXDEG / G; ++it) {
       if (it < nmax) {
@@ -168,6 +221,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
         }
       }
     }
+    }
     acc = group_sum<G>(acc);
     if (NEAR) nnear = group_sum_i<G>(nnear);
     if (valid && l == 0) {
@@ -189,7 +243,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   __syncthreads();  // LDS is reused by the next tile
 }
 
-template <int G, bool DAMAGE, bool NEAR, int INFL>
+template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
 __global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
                                                     NodeRec* __restrict__ rec,
                                                     float* __restrict__ theta,
@@ -197,11 +251,11 @@ __global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
   __shared__ TileLds L;
   const int tile = xcd_tile(blockIdx.x, t.ntiles);
   if (tile >= t.ntiles) return;
-  tile_pass1<G, DAMAGE, NEAR, INFL>(a, t, rec, theta, nbroken, tile, L);
+  tile_pass1<G, DAMAGE, NEAR, INFL, PIPE>(a, t, rec, theta, nbroken, tile, L);
 }
 
 // Pass 2 (both-ends force + Verlet epilogue) of one tile.
-template <int G, bool VERLET, int INFL>
+template <int G, bool VERLET, int INFL, bool PIPE>
 __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
                                            const NodeRec* __restrict__ rec,
                                            NodeRec* __restrict__ rec_next,
@@ -249,6 +303,30 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     int lj[MAXDEG / G];
     load_lcol<G>(t.lcol, rs, deg, l, lj);
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    if (PIPE) {
+      const int nmax = wave_max((deg + G - 1) / G);
+      Nb<true> nb[2];
+      nb[0] = nb_load<true>(L, lj[0]);
+#pragma unroll
+      for (int it = 0; it < MAXDEG / G; ++it) {
+        if (it < nmax) {
+          if (it + 1 < MAXDEG / G && it + 1 < nmax)
+            nb[(it + 1) & 1] = nb_load<true>(L, lj[(it + 1) % (MAXDEG / G)]);
+          const Nb<true>& cur = nb[it & 1];
+          const int k = it * G + l;
+          const bool ok = k < deg && ((mo[(it * G) >> 6] >> (k & 63)) & 1ull) == 0ull;
+          const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
+                                             cur.x2, cur.u0, cur.u1, cur.u2);
+          const float w = omega_t<INFL>(b.zeta, hhat);
+          float tt = fmaf(b.zeta, p_i + cur.p, b.ext * (alpha_i + cur.alpha));
+          tt = w * tt * cur.vol * b.ry;
+          tt = ok ? tt : 0.0f;
+          f0 = fmaf(tt, b.y0, f0);
+          f1 = fmaf(tt, b.y1, f1);
+          f2 = fmaf(tt, b.y2, f2);
+        }
+      }
+    } else {
 #pragma unroll
     for (int it = 0; it < MAXDEG / G; ++it) {
       const int k = it * G + l;
@@ -265,19 +343,29 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
         f2 = fmaf(tt, b.y2, f2);
       }
     }
+    }
     f0 = group_sum<G>(f0);
     f1 = group_sum<G>(f1);
     f2 = group_sum<G>(f2);
     if (valid && l < 3) {
       const float fc = l == 0 ? f0 : (l == 1 ? f1 : f2);
       if (!VERLET) fout[3 * node + l] = (double)fc;
-      else verlet_epilogue(va, node, l, fc, su[l][li], rec_next);
+      else L.sf[3 * (node - n0) + l] = fc;
+    }
+  }
+  if (VERLET) {
+    // Verlet epilogue of the whole tile at once: one component per thread
+    // instead of 3 of every 16 lanes per node round
+    __syncthreads();
+    for (int q = threadIdx.x; q < 3 * (n1 - n0); q += WG) {
+      const int k = q / 3, l3 = q - 3 * k;
+      verlet_epilogue(va, n0 + k, l3, L.sf[q], su[l3][own + k], rec_next);
     }
   }
   __syncthreads();  // LDS is reused by the next tile
 }
 
-template <int G, bool VERLET, int INFL>
+template <int G, bool VERLET, int INFL, bool PIPE>
 __global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
                                                     const NodeRec* __restrict__ rec,
                                                     NodeRec* __restrict__ rec_next,
@@ -285,7 +373,7 @@ __global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
   __shared__ TileLds L;
   const int tile = xcd_tile(blockIdx.x, t.ntiles);
   if (tile >= t.ntiles) return;
-  tile_pass2<G, VERLET, INFL>(a, t, rec, rec_next, fout, va, tile, L);
+  tile_pass2<G, VERLET, INFL, PIPE>(a, t, rec, rec_next, fout, va, tile, L);
 }
 
 // Persistent explicit run: one cooperative launch advances `nsteps` Verlet
@@ -307,7 +395,7 @@ __global__ __launch_bounds__(WG) void k_persistent(BondArgs a, TileArgs t, NodeR
   for (int k = first; k <= nsteps; k++) {
     for (int q = blockIdx.x; q < t.ntiles; q += nb) {
       const int tile = xcd_tile_strided(q, nb, t.ntiles);
-      tile_pass1<G, DAMAGE, false, INFL>(a, t, rec[cur], theta, nbroken, tile, L);
+      tile_pass1<G, DAMAGE, false, INFL, true>(a, t, rec[cur], theta, nbroken, tile, L);
     }
     grid.sync();
     VerletArgs v = va;
@@ -315,48 +403,69 @@ __global__ __launch_bounds__(WG) void k_persistent(BondArgs a, TileArgs t, NodeR
     v.start_step = k < nsteps ? 1 : 0;
     v.t_cur = t0 + ((k + 1) * va.dt);
     v.t_prev = t0 + (k * va.dt);
+    v.inv_dtbc = 1.0 / (v.t_cur - v.t_prev);
     for (int q = blockIdx.x; q < t.ntiles; q += nb) {
       const int tile = xcd_tile_strided(q, nb, t.ntiles);
-      tile_pass2<G, true, INFL>(a, t, rec[cur], rec[cur ^ 1], nullptr, v, tile, L);
+      tile_pass2<G, true, INFL, true>(a, t, rec[cur], rec[cur ^ 1], nullptr, v, tile, L);
     }
     grid.sync();
     cur ^= 1;
   }
 }
 
-template <int G, int INFL>
+// pipe: the pipelined branch-free bond loops (default); the plain loops are
+// kept for G = 16 as a measured alternative.
+template <int G, int INFL, bool PIPE>
 void p1_launch(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, int* nbroken,
                bool damage, bool count_near, hipStream_t s) {
   const dim3 grid(tile_grid(t.ntiles)), block(WG);
   if (damage && count_near)
-    hipLaunchKernelGGL((k_pass1_tiled<G, true, true, INFL>), grid, block, 0, s, a, t, rec, theta, nbroken);
+    hipLaunchKernelGGL((k_pass1_tiled<G, true, true, INFL, PIPE>), grid, block, 0, s, a, t, rec, theta, nbroken);
   else if (damage)
-    hipLaunchKernelGGL((k_pass1_tiled<G, true, false, INFL>), grid, block, 0, s, a, t, rec, theta, nbroken);
+    hipLaunchKernelGGL((k_pass1_tiled<G, true, false, INFL, PIPE>), grid, block, 0, s, a, t, rec, theta, nbroken);
   else
-    hipLaunchKernelGGL((k_pass1_tiled<G, false, false, INFL>), grid, block, 0, s, a, t, rec, theta, nbroken);
+    hipLaunchKernelGGL((k_pass1_tiled<G, false, false, INFL, PIPE>), grid, block, 0, s, a, t, rec, theta, nbroken);
 }
 
-template <int G, bool VERLET, int INFL>
+template <int G, bool VERLET, int INFL, bool PIPE>
 void p2_launch(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* rec_next,
                double* f, const VerletArgs& va, hipStream_t s) {
   const dim3 grid(tile_grid(t.ntiles)), block(WG);
-  hipLaunchKernelGGL((k_pass2_tiled<G, VERLET, INFL>), grid, block, 0, s, a, t, rec, rec_next, f, va);
+  hipLaunchKernelGGL((k_pass2_tiled<G, VERLET, INFL, PIPE>), grid, block, 0, s, a, t, rec, rec_next, f, va);
+}
+
+template <int G, int INFL>
+void p1_pipe(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, int* nbroken,
+             bool damage, bool count_near, bool pipe, hipStream_t s) {
+  if constexpr (G == 16) {
+    if (!pipe) return p1_launch<G, INFL, false>(a, t, rec, theta, nbroken, damage, count_near, s);
+  }
+  p1_launch<G, INFL, true>(a, t, rec, theta, nbroken, damage, count_near, s);
+}
+
+template <int G, bool VERLET, int INFL>
+void p2_pipe(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* rec_next, double* f,
+             const VerletArgs& va, bool pipe, hipStream_t s) {
+  if constexpr (G == 16) {
+    if (!pipe) return p2_launch<G, VERLET, INFL, false>(a, t, rec, rec_next, f, va, s);
+  }
+  p2_launch<G, VERLET, INFL, true>(a, t, rec, rec_next, f, va, s);
 }
 
 template <int G>
 void p1_infl(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, int* nbroken,
-             bool damage, bool count_near, hipStream_t s) {
-  if (a.influence == PDG_INFLUENCE_ONE) p1_launch<G, 0>(a, t, rec, theta, nbroken, damage, count_near, s);
-  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p1_launch<G, 1>(a, t, rec, theta, nbroken, damage, count_near, s);
-  else p1_launch<G, 2>(a, t, rec, theta, nbroken, damage, count_near, s);
+             bool damage, bool count_near, bool pipe, hipStream_t s) {
+  if (a.influence == PDG_INFLUENCE_ONE) p1_pipe<G, 0>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p1_pipe<G, 1>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  else p1_pipe<G, 2>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
 }
 
 template <int G, bool VERLET>
 void p2_infl(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* rec_next, double* f,
-             const VerletArgs& va, hipStream_t s) {
-  if (a.influence == PDG_INFLUENCE_ONE) p2_launch<G, VERLET, 0>(a, t, rec, rec_next, f, va, s);
-  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p2_launch<G, VERLET, 1>(a, t, rec, rec_next, f, va, s);
-  else p2_launch<G, VERLET, 2>(a, t, rec, rec_next, f, va, s);
+             const VerletArgs& va, bool pipe, hipStream_t s) {
+  if (a.influence == PDG_INFLUENCE_ONE) p2_pipe<G, VERLET, 0>(a, t, rec, rec_next, f, va, pipe, s);
+  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p2_pipe<G, VERLET, 1>(a, t, rec, rec_next, f, va, pipe, s);
+  else p2_pipe<G, VERLET, 2>(a, t, rec, rec_next, f, va, pipe, s);
 }
 
 }  // namespace
@@ -392,32 +501,32 @@ hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0
 }
 
 hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta,
-                              int* nbroken, bool damage, bool count_near, int lanes,
+                              int* nbroken, bool damage, bool count_near, int lanes, bool pipe,
                               hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
-  if (lanes == 16) p1_infl<16>(a, t, rec, theta, nbroken, damage, count_near, s);
-  else if (lanes == 32) p1_infl<32>(a, t, rec, theta, nbroken, damage, count_near, s);
-  else p1_infl<64>(a, t, rec, theta, nbroken, damage, count_near, s);
+  if (lanes == 16) p1_infl<16>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  else if (lanes == 32) p1_infl<32>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  else p1_infl<64>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   return hipGetLastError();
 }
 
 hipError_t launch_pass2_tiled_force(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
-                                    double* f, int lanes, hipStream_t s) {
+                                    double* f, int lanes, bool pipe, hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
   VerletArgs va{};
-  if (lanes == 16) p2_infl<16, false>(a, t, rec, nullptr, f, va, s);
-  else if (lanes == 32) p2_infl<32, false>(a, t, rec, nullptr, f, va, s);
-  else p2_infl<64, false>(a, t, rec, nullptr, f, va, s);
+  if (lanes == 16) p2_infl<16, false>(a, t, rec, nullptr, f, va, pipe, s);
+  else if (lanes == 32) p2_infl<32, false>(a, t, rec, nullptr, f, va, pipe, s);
+  else p2_infl<64, false>(a, t, rec, nullptr, f, va, pipe, s);
   return hipGetLastError();
 }
 
 hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const NodeRec* rec,
-                                     NodeRec* rec_next, const VerletArgs& va, int lanes,
+                                     NodeRec* rec_next, const VerletArgs& va, int lanes, bool pipe,
                                      hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
-  if (lanes == 16) p2_infl<16, true>(a, t, rec, rec_next, nullptr, va, s);
-  else if (lanes == 32) p2_infl<32, true>(a, t, rec, rec_next, nullptr, va, s);
-  else p2_infl<64, true>(a, t, rec, rec_next, nullptr, va, s);
+  if (lanes == 16) p2_infl<16, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
+  else if (lanes == 32) p2_infl<32, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
+  else p2_infl<64, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   return hipGetLastError();
 }
 

@@ -60,6 +60,13 @@ def newton(ctx: Context, u, tol=1e-6, max_newton=20, precond="block_jacobi", lin
     return u, dict(newton=ni.value, linear=li.value, residual=rn.value)
 
 
+def fallback_count(ctx: Context):
+    """Learned-preconditioner solves finished by block-Jacobi CG so far."""
+    n = C.c_int()
+    ctx.check(ctx.lib.pdg_qs_fallback_count(ctx.h, C.byref(n)))
+    return n.value
+
+
 def end_dofs(left, right):
     """All three components of the end-layer nodes are kinematic."""
     nodes = np.concatenate([left, right])
