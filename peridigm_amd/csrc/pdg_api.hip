@@ -393,7 +393,10 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if ((rc = A((void**)&c->d_row, (N + 1) * 4, "alloc row"))) return rc;
   if ((rc = A((void**)&c->d_col, c->B * 4, "alloc col"))) return rc;
   if ((rc = A((void**)&c->d_mw, (N + 1) * 4, "alloc mw"))) return rc;
-  if ((rc = A((void**)&c->d_mask, (size_t)c->mw.back() * 8, "alloc mask"))) return rc;
+  // mask words and local indices carry kMaskPad / kLcolPad zeroed entries so
+  // the bond kernels may load past a row's end without a branch
+  if ((rc = A((void**)&c->d_mask, ((size_t)c->mw.back() + kMaskPad) * 8, "alloc mask"))) return rc;
+  if ((rc = c->hip(hipMemsetAsync(c->d_mask, 0, ((size_t)c->mw.back() + kMaskPad) * 8, c->stream), "zero mask"))) return rc;
   if ((rc = A((void**)&c->d_nc, (size_t)N * sizeof(NodeConst), "alloc nc"))) return rc;
   if ((rc = A((void**)&c->d_rec[0], (size_t)N * sizeof(NodeRec), "alloc rec0"))) return rc;
   if ((rc = A((void**)&c->d_rec[1], (size_t)N * sizeof(NodeRec), "alloc rec1"))) return rc;
@@ -415,7 +418,8 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
     if ((rc = A((void**)&c->d_halo_off, 4 * c->halo_off.size(), "alloc halo off"))) return rc;
     if ((rc = A((void**)&c->d_halo, 4 * c->halo.size(), "alloc halo"))) return rc;
     if ((rc = A((void**)&c->d_own_off, 4 * c->own_off.size(), "alloc own off"))) return rc;
-    if ((rc = A((void**)&c->d_lcol, 2 * (size_t)c->B, "alloc lcol"))) return rc;
+    if ((rc = A((void**)&c->d_lcol, 2 * ((size_t)c->B + kLcolPad), "alloc lcol"))) return rc;
+    if ((rc = c->hip(hipMemsetAsync(c->d_lcol, 0, 2 * ((size_t)c->B + kLcolPad), c->stream), "zero lcol"))) return rc;
   }
 
   hipStream_t s = c->stream;

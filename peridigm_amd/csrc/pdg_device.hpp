@@ -22,18 +22,56 @@ __device__ __forceinline__ float omega_hat(int kind, float zeta, float hhat) {
   return expf(-(zeta * zeta) / h2);
 }
 
+// Cross-lane moves inside 16-lane rows (DPP): quad_perm [1,0,3,2] and
+// [2,3,0,1], half-row mirror, row mirror.
+enum { DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E, DPP_HMIRROR = 0x141, DPP_MIRROR = 0x140 };
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+
+// Sum over aligned groups of G lanes (16, 32 or 64), result in every lane of
+// the group. Within a 16-lane row it takes four DPP adds (quad pairs, quads,
+// half rows, rows); wider groups add shuffles across rows. Call with all
+// lanes active.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  v += dpp_f<DPP_QXOR1>(v);
+  v += dpp_f<DPP_QXOR2>(v);
+  v += dpp_f<DPP_HMIRROR>(v);
+  v += dpp_f<DPP_MIRROR>(v);
+  if (G >= 32) v += __shfl_xor(v, 16);
+  if (G >= 64) v += __shfl_xor(v, 32);
   return v;
 }
 
 template <int G>
 __device__ __forceinline__ int group_sum_i(int v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  v += dpp_i<DPP_QXOR1>(v);
+  v += dpp_i<DPP_QXOR2>(v);
+  v += dpp_i<DPP_HMIRROR>(v);
+  v += dpp_i<DPP_MIRROR>(v);
+  if (G >= 32) v += __shfl_xor(v, 16);
+  if (G >= 64) v += __shfl_xor(v, 32);
   return v;
+}
+
+// Wave-wide maximum of a value that is uniform inside each group of G lanes
+// (read from the first lane of every group; returns a wave-uniform value).
+template <int G>
+__device__ __forceinline__ int groups_max(int v) {
+  int m = __builtin_amdgcn_readfirstlane(v);
+  if (G <= 32) m = max(m, __builtin_amdgcn_readlane(v, 32));
+  if (G <= 16) {
+    m = max(m, __builtin_amdgcn_readlane(v, 16));
+    m = max(m, __builtin_amdgcn_readlane(v, 48));
+  }
+  return m;
 }
 
 __device__ __forceinline__ int wave_max(int v) {

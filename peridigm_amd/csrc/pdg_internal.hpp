@@ -79,8 +79,10 @@ struct VerletArgs {
 // internal nodes [tile_start[t], tile_start[t+1]); its halo (sorted internal
 // ids, own nodes included) is staged in LDS; lcol[b] is the position of bond
 // b's neighbour inside its tile's halo. Tiling needs every node degree <= 128.
-constexpr int kTileHaloMax = 1024;
+constexpr int kTileHaloMax = 800;
 constexpr int kTileNodesMax = 128;
+constexpr int kMaskPad = 2;    // >= mask words per node in the tiled kernels (degree <= 128)
+constexpr int kLcolPad = 128;  // >= the tiled kernels' maximum degree
 
 struct TileArgs {
   int ntiles;

@@ -3,7 +3,7 @@
 // One workgroup (8 waves) owns a tile: a contiguous range of at most 128
 // Morton-ordered nodes. It first stages the FP64 reference positions, FP64
 // displacements and per-node scalars of the tile's halo (the union of the
-// tile's nodes and all their neighbours, at most 1024 nodes) in LDS with one
+// tile's nodes and all their neighbours, at most 800 nodes) in LDS with one
 // 64-byte record load per halo node, then runs group-of-G-lanes bond loops
 // that read neighbours from LDS through 16-bit local indices. Per bond, HBM
 // and L2 see a 2-byte index and one bit of the damage mask instead of a
@@ -39,6 +39,10 @@ using namespace dev;
 
 constexpr int HMAX = kTileHaloMax;
 constexpr int WG = 512;
+// Workgroups per CU the register allocation is tuned for: 49.5 KB of LDS per
+// workgroup (halo capacity 800) leaves room for three; measured faster than
+// the two that a 1024-node halo allows.
+constexpr int WG_PER_CU = 3;
 constexpr int MAXDEG = 128;
 constexpr int NWORDS = MAXDEG / 64;
 
@@ -65,12 +69,19 @@ __device__ __forceinline__ float omega_t(float zeta, float hhat) {
   return omega_hat(INFL, zeta, hhat);
 }
 
-// Loads a node's mask words into registers.
+// Loads a node's mask words into registers. The mask array is padded by
+// NWORDS words and the index array by MAXDEG entries, so the loads past a
+// row's end stay in bounds and need no branch; what they read is replaced by
+// selects (words by 0; indices by 0, a staged slot, so the masked-out bonds
+// of inactive lanes still read initialised LDS and stay finite).
 __device__ __forceinline__ void load_words(const unsigned long long* __restrict__ mask, int w0,
                                            int deg, unsigned long long (&m)[NWORDS]) {
   const int nw = (deg + 63) >> 6;
 #pragma unroll
-  for (int w = 0; w < NWORDS; w++) m[w] = w < nw ? mask[(unsigned)(w0 + w)] : 0ull;
+  for (int w = 0; w < NWORDS; w++) {
+    const unsigned long long v = mask[(unsigned)(w0 + w)];
+    m[w] = w < nw ? v : 0ull;
+  }
 }
 
 // Prefetches a lane's local neighbour indices for the whole row.
@@ -79,8 +90,8 @@ __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lco
                                           int l, int (&lj)[MAXDEG / G]) {
 #pragma unroll
   for (int it = 0; it < MAXDEG / G; it++) {
-    const int k = it * G + l;
-    lj[it] = k < deg ? (int)lcol[(unsigned)(rs + k)] : 0;
+    const int v = (int)lcol[(unsigned)(rs + it * G + l)];
+    lj[it] = it * G + l < deg ? v : 0;
   }
 }
 
@@ -158,12 +169,15 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     int lj[MAXDEG / G];
     load_lcol<G>(t.lcol, rs, deg, l, lj);
     const float scz = c.sc;
-    const int nmax = wave_max((deg + G - 1) / G);
+    const int nmax = groups_max<G>((deg + G - 1) / G);
     float acc = 0.0f;
     int nnear = 0;
     if (PIPE) {
       // branch-free bond loop: every lane computes, inactive and broken
-      // bonds are selected away; the next bond's LDS data is in flight
+      // bonds are selected away; the next bond's LDS data is in flight. The
+      // per-iteration break ballots stay wave-uniform (scalar registers) and
+      // are turned into the group's mask words after the loop.
+      unsigned long long balv[MAXDEG / G];
       Nb<false> nb[2];
       nb[0] = nb_load<false>(L, lj[0]);
 #pragma unroll
@@ -185,12 +199,17 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
           const float w = omega_t<INFL>(b.zeta, c.hhat);
           const float nacc = fmaf(w * b.zeta * b.ext, cur.vol, acc);
           acc = (act && !brk) ? nacc : acc;
-          if (DAMAGE) {
-            const unsigned long long bal = __ballot(brk);
-            const unsigned long long gb =
-                G == 64 ? bal : ((bal >> (g * G)) & ((1ull << (G & 63)) - 1ull));
-            mn[(it * G) >> 6] |= gb << ((it * G) & 63);
-          }
+          if (DAMAGE) balv[it] = __ballot(brk);
+        } else if (DAMAGE) {
+          balv[it] = 0ull;
+        }
+      }
+      if (DAMAGE) {
+#pragma unroll
+        for (int it = 0; it < MAXDEG / G; ++it) {
+          const unsigned long long gb =
+              G == 64 ? balv[it] : ((balv[it] >> (g * G)) & ((1ull << (G & 63)) - 1ull));
+          mn[(it * G) >> 6] |= gb << ((it * G) & 63);
         }
       }
     } else {
@@ -244,7 +263,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
 }
 
 template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
-__global__ __launch_bounds__(WG) void k_pass1_tiled(BondArgs a, TileArgs t,
+__global__ __launch_bounds__(WG, WG_PER_CU) void k_pass1_tiled(BondArgs a, TileArgs t,
                                                     NodeRec* __restrict__ rec,
                                                     float* __restrict__ theta,
                                                     int* __restrict__ nbroken) {
@@ -304,7 +323,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     load_lcol<G>(t.lcol, rs, deg, l, lj);
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
-      const int nmax = wave_max((deg + G - 1) / G);
+      const int nmax = groups_max<G>((deg + G - 1) / G);
       Nb<true> nb[2];
       nb[0] = nb_load<true>(L, lj[0]);
 #pragma unroll
@@ -366,7 +385,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
 }
 
 template <int G, bool VERLET, int INFL, bool PIPE>
-__global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
+__global__ __launch_bounds__(WG, WG_PER_CU) void k_pass2_tiled(BondArgs a, TileArgs t,
                                                     const NodeRec* __restrict__ rec,
                                                     NodeRec* __restrict__ rec_next,
                                                     double* __restrict__ fout, VerletArgs va) {
@@ -382,7 +401,7 @@ __global__ __launch_bounds__(WG) void k_pass2_tiled(BondArgs a, TileArgs t,
 // order. Co-residency of the whole grid is guaranteed by the cooperative
 // launch, so the barriers always complete.
 template <int G, bool DAMAGE, int INFL>
-__global__ __launch_bounds__(WG) void k_persistent(BondArgs a, TileArgs t, NodeRec* rec0,
+__global__ __launch_bounds__(WG, WG_PER_CU) void k_persistent(BondArgs a, TileArgs t, NodeRec* rec0,
                                                    NodeRec* rec1, float* theta, int* nbroken,
                                                    VerletArgs va, int nsteps, double t0,
                                                    int initial_eval) {
