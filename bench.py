@@ -31,6 +31,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
+# Measured HBM bytes per launch of the default configuration's kernels (plate
+# 400x400x4, 8 lanes per node, LDS-tiled pipelined loops): rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean of 73 dispatches), each
+# scaled by what the counter reads for a plain elementwise kernel that reads
+# and writes 256 MiB (FETCH_SIZE reads half: x2.0; WRITE_SIZE exact: x1.0).
+# Source: profiles/bond_passes_tiled_plate_400x400x4.md. Reported only for
+# that configuration.
+MEASURED_TRAFFIC = {
+    "pass1": (95251.3 * 2.0 + 25002.8 * 1.0) * 1024,
+    "pass2": (102299.3 * 2.0 + 56232.9 * 1.0) * 1024,
+}
+
 
 def bytes_pass1(B, N):
     """Algorithmic HBM bytes of the damage+dilatation pass: the int32 CSR
@@ -135,7 +147,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--workload", default="plate")
-    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled, 2: tiles with plain loops")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qs", action="store_true",
@@ -199,6 +211,8 @@ def main():
     st = ctx.download_state()
     ctx.close()
 
+    profiled = args.workload == "plate" and args.lanes == 8 and args.variant == 0
+    traffic2 = MEASURED_TRAFFIC["pass2"] if profiled else None
     ms_per_step = elapsed / args.steps * 1e3
     value = B * args.steps / elapsed
     a2 = bytes_pass2(B, N) / p2_avg / 1e9
@@ -234,9 +248,11 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": a2, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": a2 / HBM_PEAK_GBS, "traffic": None,
+            "frac": a2 / HBM_PEAK_GBS, "traffic": traffic2,
             "kernel": "pass 2 (force + Verlet)", "kernel_avg_us": p2_avg * 1e6,
             "pass1_achieved": a1, "pass1_avg_us": p1_avg * 1e6,
+            "pass1_traffic": MEASURED_TRAFFIC["pass1"] if profiled else None,
+            "algorithmic_bytes": bytes_pass2(B, N),
             "step_frac": (bytes_pass1(B, N) + bytes_pass2(B, N)) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
     }

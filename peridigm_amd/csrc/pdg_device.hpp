@@ -35,16 +35,16 @@ __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
 }
 
-// Sum over aligned groups of G lanes (16, 32 or 64), result in every lane of
-// the group. Within a 16-lane row it takes four DPP adds (quad pairs, quads,
-// half rows, rows); wider groups add shuffles across rows. Call with all
-// lanes active.
+// Sum over aligned groups of G lanes (8, 16, 32 or 64), result in every lane
+// of the group. Within a 16-lane row it takes up to four DPP adds (quad
+// pairs, quads, half rows, rows); wider groups add shuffles across rows. Call
+// with all lanes active.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
   v += dpp_f<DPP_QXOR1>(v);
   v += dpp_f<DPP_QXOR2>(v);
   v += dpp_f<DPP_HMIRROR>(v);
-  v += dpp_f<DPP_MIRROR>(v);
+  if (G >= 16) v += dpp_f<DPP_MIRROR>(v);
   if (G >= 32) v += __shfl_xor(v, 16);
   if (G >= 64) v += __shfl_xor(v, 32);
   return v;
@@ -55,7 +55,7 @@ __device__ __forceinline__ int group_sum_i(int v) {
   v += dpp_i<DPP_QXOR1>(v);
   v += dpp_i<DPP_QXOR2>(v);
   v += dpp_i<DPP_HMIRROR>(v);
-  v += dpp_i<DPP_MIRROR>(v);
+  if (G >= 16) v += dpp_i<DPP_MIRROR>(v);
   if (G >= 32) v += __shfl_xor(v, 16);
   if (G >= 64) v += __shfl_xor(v, 32);
   return v;
@@ -66,11 +66,8 @@ __device__ __forceinline__ int group_sum_i(int v) {
 template <int G>
 __device__ __forceinline__ int groups_max(int v) {
   int m = __builtin_amdgcn_readfirstlane(v);
-  if (G <= 32) m = max(m, __builtin_amdgcn_readlane(v, 32));
-  if (G <= 16) {
-    m = max(m, __builtin_amdgcn_readlane(v, 16));
-    m = max(m, __builtin_amdgcn_readlane(v, 48));
-  }
+#pragma unroll
+  for (int g = G; g < 64; g += G) m = max(m, __builtin_amdgcn_readlane(v, g));
   return m;
 }
 

@@ -107,7 +107,7 @@ hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const
 // state in rec[(evaluations) % 2].
 hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0, NodeRec* rec1,
                              float* theta, int* nbroken, const VerletArgs& va, int nsteps,
-                             double t0, int initial_eval, bool damage, int* grid_out,
+                             double t0, int initial_eval, bool damage, int lanes, int* grid_out,
                              hipStream_t s);
 
 // Launchers (pdg_bonds.hip). `lanes` = lanes per node (16, 32 or 64).

@@ -94,7 +94,6 @@ __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lco
     lj[it] = it * G + l < deg ? v : 0;
   }
 }
-
 // LDS staging arrays of one workgroup (shared by both passes).
 struct TileLds {
   double sx[3][HMAX];
@@ -489,18 +488,24 @@ void p2_infl(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* 
 
 }  // namespace
 
+namespace {
+template <int G>
+void* persistent_fn(int influence, bool damage) {
+  if (influence == PDG_INFLUENCE_ONE)
+    return damage ? (void*)k_persistent<G, true, 0> : (void*)k_persistent<G, false, 0>;
+  if (influence == PDG_INFLUENCE_PARABOLIC_DECAY)
+    return damage ? (void*)k_persistent<G, true, 1> : (void*)k_persistent<G, false, 1>;
+  return damage ? (void*)k_persistent<G, true, 2> : (void*)k_persistent<G, false, 2>;
+}
+}  // namespace
+
 hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0, NodeRec* rec1,
                              float* theta, int* nbroken, const VerletArgs& va, int nsteps,
-                             double t0, int initial_eval, bool damage, int* grid_out,
+                             double t0, int initial_eval, bool damage, int lanes, int* grid_out,
                              hipStream_t s) {
   if (t.ntiles <= 0) return hipErrorInvalidValue;
-  void* fn;
-  if (a.influence == PDG_INFLUENCE_ONE)
-    fn = damage ? (void*)k_persistent<16, true, 0> : (void*)k_persistent<16, false, 0>;
-  else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY)
-    fn = damage ? (void*)k_persistent<16, true, 1> : (void*)k_persistent<16, false, 1>;
-  else
-    fn = damage ? (void*)k_persistent<16, true, 2> : (void*)k_persistent<16, false, 2>;
+  if (lanes != 8 && lanes != 16) return hipErrorInvalidValue;
+  void* fn = lanes == 8 ? persistent_fn<8>(a.influence, damage) : persistent_fn<16>(a.influence, damage);
   int dev = 0, ncu = 0, per_cu = 0, coop = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
@@ -523,7 +528,8 @@ hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec
                               int* nbroken, bool damage, bool count_near, int lanes, bool pipe,
                               hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
-  if (lanes == 16) p1_infl<16>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  if (lanes == 8) p1_infl<8>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  else if (lanes == 16) p1_infl<16>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else if (lanes == 32) p1_infl<32>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else p1_infl<64>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   return hipGetLastError();
@@ -533,7 +539,8 @@ hipError_t launch_pass2_tiled_force(const BondArgs& a, const TileArgs& t, const 
                                     double* f, int lanes, bool pipe, hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
   VerletArgs va{};
-  if (lanes == 16) p2_infl<16, false>(a, t, rec, nullptr, f, va, pipe, s);
+  if (lanes == 8) p2_infl<8, false>(a, t, rec, nullptr, f, va, pipe, s);
+  else if (lanes == 16) p2_infl<16, false>(a, t, rec, nullptr, f, va, pipe, s);
   else if (lanes == 32) p2_infl<32, false>(a, t, rec, nullptr, f, va, pipe, s);
   else p2_infl<64, false>(a, t, rec, nullptr, f, va, pipe, s);
   return hipGetLastError();
@@ -543,7 +550,8 @@ hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const
                                      NodeRec* rec_next, const VerletArgs& va, int lanes, bool pipe,
                                      hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
-  if (lanes == 16) p2_infl<16, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
+  if (lanes == 8) p2_infl<8, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
+  else if (lanes == 16) p2_infl<16, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   else if (lanes == 32) p2_infl<32, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   else p2_infl<64, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   return hipGetLastError();
