@@ -130,7 +130,7 @@ int Ctx::build_tiles() {
     own_off.push_back((int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin()));
     for (int k = n0; k < i; k++)
       for (int b = row[k]; b < row[k + 1]; b++)
-        lcol[b] = (unsigned short)(std::lower_bound(H.begin(), H.end(), col[b]) - H.begin());
+        lcol[b] = (unsigned short)(8 * (std::lower_bound(H.begin(), H.end(), col[b]) - H.begin()));
     halo.insert(halo.end(), H.begin(), H.end());
     halo_off.push_back((int)halo.size());
     tile_start.push_back(i);

@@ -77,9 +77,11 @@ struct VerletArgs {
 
 // Morton tiles of the LDS-staged kernels: a tile is a contiguous range of
 // internal nodes [tile_start[t], tile_start[t+1]); its halo (sorted internal
-// ids, own nodes included) is staged in LDS; lcol[b] is the position of bond
-// b's neighbour inside its tile's halo. Tiling needs every node degree <= 128.
+// ids, own nodes included) is staged in LDS; lcol[b] is 8 times the position
+// of bond b's neighbour inside its tile's halo (its byte offset in the 8-byte
+// LDS arrays). Tiling needs every node degree <= 128.
 constexpr int kTileHaloMax = 800;
+static_assert(8 * kTileHaloMax <= 65536, "halo byte offsets must fit 16 bits");
 constexpr int kTileNodesMax = 128;
 constexpr int kMaskPad = 2;    // >= mask words per node in the tiled kernels (degree <= 128)
 constexpr int kLcolPad = 128;  // >= the tiled kernels' maximum degree

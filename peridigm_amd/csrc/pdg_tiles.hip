@@ -98,9 +98,9 @@ __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lco
 struct TileLds {
   double sx[3][HMAX];
   double su[3][HMAX];
-  float svol[HMAX];
-  float salpha[HMAX];
-  float sp[HMAX];
+  float2 sva[HMAX];  // (volume, alpha)
+  float2 spx[HMAX];  // (p, unused): 8-byte slots, so one byte offset j * 8
+                     // addresses every array
   float sf[3 * kTileNodesMax];  // pass 2: the tile's forces, for the Verlet epilogue
 };
 
@@ -112,15 +112,40 @@ struct Nb {
   float vol, alpha, p;
 };
 
+// jb is the neighbour's byte offset in the 8-byte-slot arrays (local index
+// times 8, as stored in the tile index array).
+template <typename T>
+__device__ __forceinline__ T lds_at(const void* arr, int jb) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(arr) + jb);
+}
+
 template <bool FORCE>
-__device__ __forceinline__ Nb<FORCE> nb_load(const TileLds& L, int j) {
+__device__ __forceinline__ Nb<FORCE> nb_load(const TileLds& L, int jb) {
   Nb<FORCE> n;
-  n.x0 = L.sx[0][j]; n.x1 = L.sx[1][j]; n.x2 = L.sx[2][j];
-  n.u0 = L.su[0][j]; n.u1 = L.su[1][j]; n.u2 = L.su[2][j];
-  n.vol = L.svol[j];
-  if (FORCE) { n.alpha = L.salpha[j]; n.p = L.sp[j]; }
-  else { n.alpha = 0.0f; n.p = 0.0f; }
+  n.x0 = lds_at<double>(L.sx[0], jb); n.x1 = lds_at<double>(L.sx[1], jb);
+  n.x2 = lds_at<double>(L.sx[2], jb);
+  n.u0 = lds_at<double>(L.su[0], jb); n.u1 = lds_at<double>(L.su[1], jb);
+  n.u2 = lds_at<double>(L.su[2], jb);
+  if (FORCE) {
+    const float2 va = lds_at<float2>(L.sva, jb);
+    n.vol = va.x; n.alpha = va.y;
+    n.p = lds_at<float>(L.spx, jb);
+  } else {
+    n.vol = lds_at<float>(L.sva, jb);
+    n.alpha = 0.0f; n.p = 0.0f;
+  }
   return n;
+}
+
+// Break bit of bond k = it * G + l of a node (it a compile-time constant):
+// for G <= 32 the bit lies in one 32-bit half of one mask word.
+template <int G>
+__device__ __forceinline__ bool mask_bit(const unsigned long long (&m)[NWORDS], int it, int l) {
+  const int base = it * G;
+  if (G == 64) return ((m[base >> 6] >> l) & 1ull) != 0ull;
+  const unsigned long long w = m[base >> 6];
+  const unsigned half = (base & 32) ? (unsigned)(w >> 32) : (unsigned)w;
+  return ((half >> ((base & 31) + l)) & 1u) != 0u;
 }
 
 // Pass 1 (damage + dilatation) of one tile; every thread of the workgroup
@@ -131,14 +156,13 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
                                            int* __restrict__ nbroken, int tile, TileLds& L) {
   double (&sx)[3][HMAX] = L.sx;
   double (&su)[3][HMAX] = L.su;
-  float (&svol)[HMAX] = L.svol;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
   for (int h = threadIdx.x; h < hn; h += WG) {
     const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
     sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
     su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
-    svol[h] = r.vol;
+    L.sva[h].x = r.vol;
   }
   __syncthreads();
   const int own = t.own_off[tile];
@@ -187,7 +211,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
           const Nb<false>& cur = nb[it & 1];
           const int k = it * G + l;
           const bool act = k < deg;
-          bool brk = act && (((mo[(it * G) >> 6] >> (k & 63)) & 1ull) != 0ull);
+          bool brk = act && mask_bit<G>(mo, it, l);
           const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
                                                cur.x2, cur.u0, cur.u1, cur.u2);
           if (DAMAGE) {
@@ -217,9 +241,9 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       if (it < nmax) {
         const int k = it * G + l;
         const bool act = k < deg;
-        bool brk = act && (((mo[(it * G) >> 6] >> (k & 63)) & 1ull) != 0ull);
+        bool brk = act && mask_bit<G>(mo, it, l);
         if (act) {
-          const int j = lj[it];
+          const int j = lj[it] >> 3;
           const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, sx[0][j], sx[1][j],
                                                sx[2][j], su[0][j], su[1][j], su[2][j]);
           if (DAMAGE) {
@@ -229,7 +253,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
             brk = brk || (b.ext > lim);
           }
           const float w = omega_t<INFL>(b.zeta, c.hhat);
-          if (!brk) acc = fmaf(w * b.zeta * b.ext, svol[j], acc);
+          if (!brk) acc = fmaf(w * b.zeta * b.ext, L.sva[j].x, acc);
         }
         if (DAMAGE) {
           const unsigned long long bal = __ballot(brk);
@@ -281,18 +305,14 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
                                            int tile, TileLds& L) {
   double (&sx)[3][HMAX] = L.sx;
   double (&su)[3][HMAX] = L.su;
-  float (&svol)[HMAX] = L.svol;
-  float (&salpha)[HMAX] = L.salpha;
-  float (&sp)[HMAX] = L.sp;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
   for (int h = threadIdx.x; h < hn; h += WG) {
     const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
     sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
     su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
-    svol[h] = r.vol;
-    salpha[h] = r.alpha;
-    sp[h] = r.p;
+    L.sva[h] = make_float2(r.vol, r.alpha);
+    L.spx[h].x = r.p;
   }
   __syncthreads();
   const int own = t.own_off[tile];
@@ -312,8 +332,8 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
       li = own + (node - n0);
       xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
       ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
-      alpha_i = salpha[li];
-      p_i = sp[li];
+      alpha_i = L.sva[li].y;
+      p_i = L.spx[li].x;
       hhat = a.nc[node].hhat;
     }
     unsigned long long mo[NWORDS];
@@ -332,7 +352,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
             nb[(it + 1) & 1] = nb_load<true>(L, lj[(it + 1) % (MAXDEG / G)]);
           const Nb<true>& cur = nb[it & 1];
           const int k = it * G + l;
-          const bool ok = k < deg && ((mo[(it * G) >> 6] >> (k & 63)) & 1ull) == 0ull;
+          const bool ok = k < deg && !mask_bit<G>(mo, it, l);
           const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
                                              cur.x2, cur.u0, cur.u1, cur.u2);
           const float w = omega_t<INFL>(b.zeta, hhat);
@@ -349,13 +369,13 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     for (int it = 0; it < MAXDEG / G; ++it) {
       const int k = it * G + l;
       if (k < deg) {
-        const bool brk = ((mo[(it * G) >> 6] >> (k & 63)) & 1ull) != 0ull;
-        const int j = lj[it];
+        const bool brk = mask_bit<G>(mo, it, l);
+        const int j = lj[it] >> 3;
         const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, sx[0][j], sx[1][j],
                                            sx[2][j], su[0][j], su[1][j], su[2][j]);
         const float w = omega_t<INFL>(b.zeta, hhat);
-        float tt = fmaf(b.zeta, p_i + sp[j], b.ext * (alpha_i + salpha[j]));
-        tt = brk ? 0.0f : w * tt * svol[j] * b.ry;
+        float tt = fmaf(b.zeta, p_i + L.spx[j].x, b.ext * (alpha_i + L.sva[j].y));
+        tt = brk ? 0.0f : w * tt * L.sva[j].x * b.ry;
         f0 = fmaf(tt, b.y0, f0);
         f1 = fmaf(tt, b.y1, f1);
         f2 = fmaf(tt, b.y2, f2);
