@@ -39,6 +39,9 @@ struct QsState {
   // solver vectors (internal order, 3n)
   double *xs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *b = nullptr,
          *part = nullptr, *scal = nullptr;
+  // flexible CG history: directions and their images (FCG_SLOTS each, 3n)
+  double* ph[4] = {};
+  double* qh[4] = {};
   int nblk = 0;
   double* hscal = nullptr;  // pinned host copy of the solver scalars
   int fallbacks = 0;  // learned-preconditioner solves handed over to block-Jacobi CG
@@ -50,6 +53,8 @@ namespace {
 using namespace dev;
 
 constexpr int QG = 16;  // lanes per node
+constexpr int kScal = 32;   // solver scalar slots on the device
+constexpr int kFcgSlots = 4;  // flexible CG keeps kFcgSlots - 1 previous directions
 constexpr int RB = 256; // reduction block
 
 __device__ __forceinline__ double infl64(int kind, double zeta, double horizon) {
@@ -298,6 +303,8 @@ __global__ __launch_bounds__(RB) void k_block_apply(int n, const double* dinv, c
 // no host round trip: RZ = r.z, PQ = p.Kp, RR = r.r, RZN = new r.z, ZQ = z.Kp,
 // BETA, BRK = 1 after a breakdown (p.Kp <= 0).
 enum { SC_RZ = 0, SC_PQ = 1, SC_RR = 2, SC_RZN = 3, SC_ZQ = 4, SC_BETA = 5, SC_BRK = 6 };
+// flexible CG: PR = p.r, then per history slot k: PQH + k = p_k.Kp_k, ZQH + k = z.Kp_k
+enum { SC_PR = 7, SC_PQH = 8, SC_ZQH = 12 };
 
 // x += al p, r -= al q, partial sums of r.r; al = RZ / PQ (0 after a breakdown)
 __global__ __launch_bounds__(RB) void k_cg_step(int n, const double* sc, const double* p,
@@ -324,6 +331,46 @@ __global__ void k_cg_beta(double* sc, int flexible) {
   if (!(pq > 0.0)) sc[SC_BRK] = 1.0;
   sc[SC_BETA] = flexible ? -sc[SC_ZQ] / pq : sc[SC_RZN] / sc[SC_RZ];
   sc[SC_RZ] = sc[SC_RZN];
+}
+
+// Flexible CG step on history slot `slot`: al = PR / PQH[slot] (0 after a
+// breakdown), x += al p, r -= al q, partial sums of r.r.
+__global__ __launch_bounds__(RB) void k_fcg_step(int n, double* sc, int slot, const double* p,
+                                                 const double* q, double* x, double* r,
+                                                 double* part) {
+  __shared__ double sh[RB / 64];
+  const double pq = sc[SC_PQH + slot];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !(pq > 0.0)) sc[SC_BRK] = 1.0;
+  const double al = pq > 0.0 ? sc[SC_PR] / pq : 0.0;
+  double s = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// New flexible-CG direction: p_new = z - sum_k (ZQH[k] / PQH[k]) p_k over the
+// nh history slots listed in `slots` (Notay's truncated FCG).
+struct FcgHist {
+  const double* p[3];
+  int slot[3];
+  int nh;
+};
+__global__ __launch_bounds__(RB) void k_fcg_dir(int n, const double* sc, const double* z,
+                                                FcgHist h, double* pnew) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  double v = z[i];
+  for (int k = 0; k < h.nh; k++) {
+    const double pq = sc[SC_PQH + h.slot[k]];
+    const double c = pq > 0.0 ? sc[SC_ZQH + h.slot[k]] / pq : 0.0;
+    v -= c * h.p[k][i];
+  }
+  pnew[i] = v;
 }
 
 // p = z + beta p
@@ -377,8 +424,12 @@ static int ensure(Ctx* c) {
   for (double** v : vecs1) if (!rc) rc = A((void**)v, 8 * (size_t)N);
   if (!rc) rc = A((void**)&S->dinv, 72 * (size_t)N);
   if (!rc) rc = A((void**)&S->part, 8 * (size_t)S->nblk * 2);
-  if (!rc) rc = A((void**)&S->scal, 64);
-  if (!rc) rc = c->hip(hipHostMalloc((void**)&S->hscal, 64, hipHostMallocDefault), "pinned alloc");
+  if (!rc) rc = A((void**)&S->scal, 8 * kScal);
+  if (!rc) rc = c->hip(hipHostMalloc((void**)&S->hscal, 8 * kScal, hipHostMallocDefault), "pinned alloc");
+  for (int k = 0; k < 4 && !rc; k++) {
+    rc = A((void**)&S->ph[k], 24 * (size_t)N);
+    if (!rc) rc = A((void**)&S->qh[k], 24 * (size_t)N);
+  }
   if (!rc) rc = A((void**)&S->fixed, 3 * (size_t)N);
   if (rc) return rc;
   // constants in internal order
@@ -481,6 +532,10 @@ void Ctx::free_extra() {
     double* ptrs[] = {S->x, S->u, S->w, S->vol, S->m, S->alpha, S->beta, S->theta, S->dtheta, S->g,
                       S->out, S->dinv, S->xs, S->r, S->z, S->p, S->q, S->b, S->part, S->scal};
     for (double* p : ptrs) if (p) (void)hipFree(p);
+    for (int k = 0; k < 4; k++) {
+      if (S->ph[k]) (void)hipFree(S->ph[k]);
+      if (S->qh[k]) (void)hipFree(S->qh[k]);
+    }
     if (S->fixed) (void)hipFree(S->fixed);
     if (S->hscal) (void)hipHostFree(S->hscal);
     delete S;
@@ -489,10 +544,14 @@ void Ctx::free_extra() {
   free_gnn();
 }
 
-// Preconditioned CG (precond 1: block Jacobi) or flexible CG with the learned
-// preconditioner (precond 2) on K x = b, device vectors in internal order.
-// x holds the initial guess. Returns iterations in *iters, final relative
-// residual in *rel.
+// K x = b on device vectors in internal order; x holds the initial guess.
+// Returns iterations in *iters, the final relative residual in *rel.
+// Preconditioned CG (precond 0: none, 1: block Jacobi) or, for the learned
+// preconditioner (precond 2), Notay's flexible CG truncated to the last
+// kFcgSlots - 1 directions: the new direction is z minus its K-projections on
+// them, alpha = p.r / p.Kp. Scalars stay on the device; the host reads the
+// residual norm at checks only: every iteration for the first 8, then every 4
+// (a solve may run up to 3 iterations past the tolerance).
 int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int maxit, int* iters,
              double* rel) {
   QsState* S = c->qs;
@@ -501,6 +560,7 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   const unsigned vb = (unsigned)((n3 + RB - 1) / RB);
   const unsigned nb = (unsigned)((c->N + RB - 1) / RB);
   const dim3 red(S->nblk), blk(RB);
+  const bool flexible = precond == 2;
   int rc = 0;
   double bb = 0.0;
   if ((rc = qs::dot(c, b, b, &bb))) return rc;
@@ -525,18 +585,19 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     return c->hip(hipMemcpyAsync(z, r, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
   };
   // r = b - K x; z = M r; p = z
+  int cur = 0;  // flexible CG: history slot of the current direction
+  double* p = flexible ? S->ph[0] : S->p;
+  double* q = flexible ? S->qh[0] : S->q;
   if ((rc = qs::tangent(c, x, S->q, false))) return rc;
   hipLaunchKernelGGL(k_sub, dim3(vb), blk, 0, s, n3, b, S->q, S->r);
-  rc = c->hip(hipMemsetAsync(S->scal, 0, 64, s), "zero scalars");
+  rc = c->hip(hipMemsetAsync(S->scal, 0, 8 * kScal, s), "zero scalars");
   if (!rc) rc = precond_apply(S->r, S->z);
-  if (!rc) rc = c->hip(hipMemcpyAsync(S->p, S->z, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
+  if (!rc) rc = c->hip(hipMemcpyAsync(p, S->z, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
   if (rc) return rc;
   ddot(S->r, S->z, SC_RZ);
   ddot(S->r, S->r, SC_RR);
-  // host reads RR and BRK at checks only: every iteration for the first 8,
-  // then every 4 (a solve may run up to 3 iterations past the tolerance)
   auto check = [&](double* relres, bool* brk) -> int {
-    int r2 = c->hip(hipMemcpyAsync(S->hscal, S->scal, 64, hipMemcpyDeviceToHost, s), "d2h");
+    int r2 = c->hip(hipMemcpyAsync(S->hscal, S->scal, 8 * kScal, hipMemcpyDeviceToHost, s), "d2h");
     if (!r2) r2 = c->hip(hipStreamSynchronize(s), "sync");
     *relres = std::sqrt(S->hscal[SC_RR]) / bnorm;
     *brk = S->hscal[SC_BRK] != 0.0;
@@ -545,31 +606,52 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   double relres = 0.0;
   bool brk = false;
   if ((rc = check(&relres, &brk))) return rc;
-  int it = 0;
+  int it = 0, nhist = 0;
   // stagnation watch for the learned preconditioner: every 48 iterations the
   // residual must have halved, else flexible CG hands over to block-Jacobi CG
   double watch = relres;
   bool stalled = false;
   while (relres > tol && it < maxit) {
-    if ((rc = qs::tangent(c, S->p, S->q, true))) return rc;
-    ddot(S->p, S->q, SC_PQ);
-    hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->scal, S->p, S->q, x, S->r, S->part);
+    if ((rc = qs::tangent(c, p, q, true))) return rc;
+    if (flexible) {
+      ddot(p, q, SC_PQH + cur);
+      ddot(p, S->r, SC_PR);
+      hipLaunchKernelGGL(k_fcg_step, red, blk, 0, s, n3, S->scal, cur, p, q, x, S->r, S->part);
+    } else {
+      ddot(p, q, SC_PQ);
+      hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->scal, p, q, x, S->r, S->part);
+    }
     hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, S->part, 1, S->scal + SC_RR);
     it++;
     if (it <= 8 || it % 4 == 0 || it == maxit) {
       if ((rc = check(&relres, &brk))) return rc;
       if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
       if (relres <= tol) break;
-      if (precond == 2 && it % 48 == 0) {
+      if (flexible && it % 48 == 0) {
         if (!(relres < 0.5 * watch)) { stalled = true; break; }
         watch = relres;
       }
     }
     if ((rc = precond_apply(S->r, S->z))) return rc;
-    ddot(S->r, S->z, SC_RZN);
-    if (precond == 2) ddot(S->z, S->q, SC_ZQ);
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, S->scal, precond == 2 ? 1 : 0);
-    hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, S->p);
+    if (flexible) {
+      nhist = std::min(nhist + 1, kFcgSlots - 1);
+      FcgHist h{};
+      h.nh = nhist;
+      for (int k = 0; k < nhist; k++) {
+        const int slot = (cur - k + kFcgSlots) % kFcgSlots;
+        h.p[k] = S->ph[slot];
+        h.slot[k] = slot;
+        ddot(S->z, S->qh[slot], SC_ZQH + slot);
+      }
+      cur = (cur + 1) % kFcgSlots;
+      p = S->ph[cur];
+      q = S->qh[cur];
+      hipLaunchKernelGGL(k_fcg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, h, p);
+    } else {
+      ddot(S->r, S->z, SC_RZN);
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, S->scal, 0);
+      hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, p);
+    }
   }
   if (!stalled && relres > tol) {
     if ((rc = check(&relres, &brk))) return rc;
