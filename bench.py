@@ -14,7 +14,8 @@ dominant kernel (the force+Verlet pass) from HIP events recorded on the
 stream it runs on; `cpu_baseline` times the FP64 CPU oracle (a scalar
 restatement of the reference loops, one core) on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plate|bar|cylinder]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plate|bar|cylinder|batched]
+                    [--lanes 8|16|32|64] [--persistent] [--qs]
 """
 from __future__ import annotations
 
@@ -152,6 +153,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qs", action="store_true",
                     help="also measure the quasi-static solve (block-Jacobi CG vs learned FCG)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="time the steps as one persistent cooperative launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3x) to calibrate HBM counters")
     args = ap.parse_args()
@@ -196,7 +199,11 @@ def main():
     # timed region: exactly K steps, device-resident state
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ctx.verlet_run(args.steps, dt, t0=t_start, initial_eval=False)
+    grid = None
+    if args.persistent:
+        grid = ctx.verlet_run_persistent(args.steps, dt, t0=t_start, initial_eval=False)
+    else:
+        ctx.verlet_run(args.steps, dt, t0=t_start, initial_eval=False)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     t_start += args.steps * dt
@@ -240,6 +247,8 @@ def main():
                            "notch U[10h,60h], one node/bond space",
             }[args.workload],
             "nodes": N, "bonds": B, "lanes_per_node": args.lanes,
+            "launch": ("persistent cooperative, %d workgroups" % grid) if args.persistent
+                      else "two launches per step",
             "kernels": ("untiled" if (args.variant == 1 or ntiles == 0) else
                         "lds-tiled" + (", plain loops" if args.variant == 2 else ", pipelined loops")),
             "tiles": ntiles, "halo_per_node": halo_total / N,
