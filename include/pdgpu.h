@@ -176,6 +176,10 @@ int pdg_qs_solve(pdg_ctx* ctx, const double* u, const double* rhs, double* x, in
 int pdg_qs_newton(pdg_ctx* ctx, double* u, double tol, int maxNewton, int precond,
                   double linTol, int maxLin, int* newtonIters, int* linIters,
                   double* residualNorm);
+/* 1: linear solves replay each CG iteration's two fixed launch sequences as
+ * HIP graphs, captured once per context; 0 (default): every kernel is
+ * launched directly. Same arithmetic, bitwise. */
+int pdg_qs_set_graphs(pdg_ctx* ctx, int enable);
 /* Number of learned-preconditioner solves that stalled (residual not halved
  * in 50 iterations) and were finished by block-Jacobi CG, the fallback. */
 int pdg_qs_fallback_count(pdg_ctx* ctx, int* count);

@@ -264,6 +264,9 @@ int pdg_set_topology(pdg_ctx* h, int numOwned, const int* ownedIDs, const int* n
   if (!c) return PDG_E_ARG;
   if (numOwned < 0 || (numOwned > 0 && (!nlist || listSize < numOwned)))
     return c->fail(PDG_E_ARG, "invalid neighbourhood list");
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->free_extra();  // quasi-static and network state are sized by the old topology
   c->have_topo = false;
   c->have_consts = false;
   c->N = numOwned;
@@ -309,6 +312,9 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if (!c->have_topo) return c->fail(PDG_E_ARG, "pdg_set_topology must be called first");
   if (!x || !vol || !(horizon > 0.0) || infl < 0 || infl > 2)
     return c->fail(PDG_E_ARG, "invalid constants");
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->free_extra();  // quasi-static and network state hold the old constants
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(PDG_E_HIP, "hipSetDevice");
   const int N = c->N;
   c->K = K; c->mu = mu; c->horizon = horizon; c->infl = infl; c->sc = sc;

@@ -81,6 +81,7 @@ struct Ctx {
 
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
+  unsigned gnn_epoch = 0;  // bumped whenever the network's buffers are (re)allocated
 
   void free_gnn();
   double* qs_dinv();

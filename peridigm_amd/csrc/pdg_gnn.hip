@@ -475,6 +475,7 @@ static int ensure(Ctx* c, unsigned seed) {
   if (c->gnn && c->gnn->n == c->N) return PDG_OK;
   c->free_gnn();
   GnnState* G = new GnnState();
+  c->gnn_epoch++;
   c->gnn = G;
   const int N = c->N;
   G->n = N;

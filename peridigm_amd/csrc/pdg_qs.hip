@@ -44,6 +44,14 @@ struct QsState {
   double* qh[4] = {};
   int nblk = 0;
   double* hscal = nullptr;  // pinned host copy of the solver scalars
+  bool graphs = false;      // replay the CG iteration sequences as HIP graphs
+  // captured iteration sequences, per preconditioner, sequence (A, B) and
+  // flexible-CG history slot; valid for the x and b vectors and the network
+  // buffers they were captured with
+  hipGraphExec_t gexec[3][2][4] = {};
+  const double* gx = nullptr;
+  const double* gb = nullptr;
+  unsigned gepoch = 0;
   int fallbacks = 0;  // learned-preconditioner solves handed over to block-Jacobi CG
   std::vector<signed char> hfixed;
 };
@@ -119,10 +127,11 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
       const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
       const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
+      const double iy = 1.0 / yl;
       const double e = yl - zeta;
       const double c = infl64(a.influence, zeta, a.horizon) * zeta * a.vol[j];
       th += c * e;
-      const double n0 = Y0 / yl, n1 = Y1 / yl, n2 = Y2 / yl;
+      const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
       if (DUAL) {
         const double* wi = a.w + 3 * node;
         const double de = n0 * (a.w[3 * j] - wi[0]) + n1 * (a.w[3 * j + 1] - wi[1]) +
@@ -166,10 +175,11 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
       const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
       const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
+      const double iy = 1.0 / yl;
       const double e = yl - zeta;
       const double om = infl64(a.influence, zeta, a.horizon);
       const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
-      const double n0 = Y0 / yl, n1 = Y1 / yl, n2 = Y2 / yl;
+      const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
       const double vj = a.vol[j];
       if (MODE == 0) {
         f0 += T * n0 * vj; f1 += T * n1 * vj; f2 += T * n2 * vj;
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
         const double d0 = a.w[3 * j] - wi[0], d1 = a.w[3 * j + 1] - wi[1], d2 = a.w[3 * j + 2] - wi[2];
         const double de = n0 * d0 + n1 * d1 + n2 * d2;
         const double dT = om * ((dthi * bi + a.dtheta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * de);
-        const double s = T / yl;
+        const double s = T * iy;
         f0 += (dT * n0 + s * (d0 - n0 * de)) * vj;
         f1 += (dT * n1 + s * (d1 - n1 * de)) * vj;
         f2 += (dT * n2 + s * (d2 - n2 * de)) * vj;
@@ -400,7 +410,7 @@ namespace qs {
 
 static QsArgs args(Ctx* c) {
   QsState* S = c->qs;
-  QsArgs a;
+  QsArgs a{};
   a.n = c->N; a.row = c->d_row; a.col = c->d_col; a.mw = c->d_mw; a.mask = c->d_mask;
   a.x = S->x; a.u = S->u; a.w = S->w; a.vol = S->vol; a.m = S->m; a.alpha = S->alpha;
   a.beta = S->beta; a.theta = S->theta; a.dtheta = S->dtheta; a.g = S->g; a.fixed = S->fixed;
@@ -509,6 +519,20 @@ static int dot(Ctx* c, const double* x, const double* y, double* out) {
   return rc;
 }
 
+// Destroys the cached iteration graphs (after the stream has drained).
+static void drop_graphs(Ctx* c) {
+  QsState* S = c->qs;
+  if (!S) return;
+  bool any = false;
+  for (auto& a : S->gexec) for (auto& b : a) for (hipGraphExec_t g : b) any = any || g;
+  if (!any) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& a : S->gexec)
+    for (auto& b : a)
+      for (hipGraphExec_t& g : b)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
 }  // namespace qs
 
 int qs_tangent_apply(Ctx* c, const double* w, double* out) { return qs::tangent(c, w, out, true); }
@@ -536,6 +560,7 @@ void Ctx::free_extra() {
       if (S->ph[k]) (void)hipFree(S->ph[k]);
       if (S->qh[k]) (void)hipFree(S->qh[k]);
     }
+    qs::drop_graphs(this);
     if (S->fixed) (void)hipFree(S->fixed);
     if (S->hscal) (void)hipHostFree(S->hscal);
     delete S;
@@ -587,7 +612,6 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   // r = b - K x; z = M r; p = z
   int cur = 0;  // flexible CG: history slot of the current direction
   double* p = flexible ? S->ph[0] : S->p;
-  double* q = flexible ? S->qh[0] : S->q;
   if ((rc = qs::tangent(c, x, S->q, false))) return rc;
   hipLaunchKernelGGL(k_sub, dim3(vb), blk, 0, s, n3, b, S->q, S->r);
   rc = c->hip(hipMemsetAsync(S->scal, 0, 8 * kScal, s), "zero scalars");
@@ -611,48 +635,105 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   // residual must have halved, else flexible CG hands over to block-Jacobi CG
   double watch = relres;
   bool stalled = false;
-  while (relres > tol && it < maxit) {
-    if ((rc = qs::tangent(c, p, q, true))) return rc;
+  // The iteration is two fixed launch sequences: A (K p, p.Kp, step, r.r)
+  // and B (preconditioner, dots, new direction), split by the host check.
+  // For flexible CG both depend on the history slot `cur`; once the history
+  // is full (nhist = kFcgSlots - 1) they are the same every time that slot
+  // comes round. With graphs on, each (sequence, slot) is captured once per
+  // solve and replayed.
+  auto seq_a = [&](int cs) -> int {
+    double* pp = flexible ? S->ph[cs] : S->p;
+    double* qq = flexible ? S->qh[cs] : S->q;
+    int r2 = qs::tangent(c, pp, qq, true);
+    if (r2) return r2;
     if (flexible) {
-      ddot(p, q, SC_PQH + cur);
-      ddot(p, S->r, SC_PR);
-      hipLaunchKernelGGL(k_fcg_step, red, blk, 0, s, n3, S->scal, cur, p, q, x, S->r, S->part);
+      ddot(pp, qq, SC_PQH + cs);
+      ddot(pp, S->r, SC_PR);
+      hipLaunchKernelGGL(k_fcg_step, red, blk, 0, s, n3, S->scal, cs, pp, qq, x, S->r, S->part);
     } else {
-      ddot(p, q, SC_PQ);
-      hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->scal, p, q, x, S->r, S->part);
+      ddot(pp, qq, SC_PQ);
+      hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->scal, pp, qq, x, S->r, S->part);
     }
     hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, S->part, 1, S->scal + SC_RR);
+    return c->hip(hipGetLastError(), "cg step");
+  };
+  auto seq_b = [&](int cs, int nh) -> int {
+    int r2 = precond_apply(S->r, S->z);
+    if (r2) return r2;
+    if (flexible) {
+      FcgHist h{};
+      h.nh = nh;
+      for (int k = 0; k < nh; k++) {
+        const int slot = (cs - k + kFcgSlots) % kFcgSlots;
+        h.p[k] = S->ph[slot];
+        h.slot[k] = slot;
+        ddot(S->z, S->qh[slot], SC_ZQH + slot);
+      }
+      hipLaunchKernelGGL(k_fcg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, h,
+                         S->ph[(cs + 1) % kFcgSlots]);
+    } else {
+      ddot(S->r, S->z, SC_RZN);
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, S->scal, 0);
+      hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, S->p);
+    }
+    return c->hip(hipGetLastError(), "cg direction");
+  };
+  const bool graphs = S->graphs;
+  // graphs captured for other vectors or older network buffers are stale
+  if (graphs && (S->gx != x || S->gb != b || (flexible && S->gepoch != c->gnn_epoch))) {
+    qs::drop_graphs(c);
+    S->gx = x; S->gb = b; S->gepoch = c->gnn_epoch;
+  }
+  hipGraphExec_t* ga = S->gexec[precond][0];
+  hipGraphExec_t* gb = S->gexec[precond][1];
+  auto release = [&]() {};
+  auto capture = [&](hipGraphExec_t* out, auto&& body) -> int {
+    hipGraph_t g = nullptr;
+    int r2 = c->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (r2) return r2;
+    const int rb = body();
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rb) { if (g) (void)hipGraphDestroy(g); return rb; }
+    r2 = c->hip(e, "end capture");
+    if (!r2) r2 = c->hip(hipGraphInstantiate(out, g, nullptr, nullptr, 0), "instantiate");
+    if (g) (void)hipGraphDestroy(g);
+    return r2;
+  };
+  // runs sequence A or B for slot cs: replayed from a graph when the history
+  // is full and graphs are on, otherwise launched directly
+  auto run_a = [&](int cs, bool full) -> int {
+    if (!graphs || !full) return seq_a(cs);
+    if (!ga[cs]) { int r2 = capture(&ga[cs], [&]() { return seq_a(cs); }); if (r2) return r2; }
+    return c->hip(hipGraphLaunch(ga[cs], s), "graph launch");
+  };
+  auto run_b = [&](int cs, int nh, bool full) -> int {
+    if (!graphs || !full) return seq_b(cs, nh);
+    if (!gb[cs]) { int r2 = capture(&gb[cs], [&]() { return seq_b(cs, nh); }); if (r2) return r2; }
+    return c->hip(hipGraphLaunch(gb[cs], s), "graph launch");
+  };
+  const int hist_full = flexible ? kFcgSlots - 1 : 0;
+  while (relres > tol && it < maxit) {
+    if ((rc = run_a(cur, nhist >= hist_full))) { release(); return rc; }
     it++;
     if (it <= 8 || it % 4 == 0 || it == maxit) {
-      if ((rc = check(&relres, &brk))) return rc;
-      if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
+      if ((rc = check(&relres, &brk))) { release(); return rc; }
+      if (brk) { release(); return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0"); }
       if (relres <= tol) break;
       if (flexible && it % 48 == 0) {
         if (!(relres < 0.5 * watch)) { stalled = true; break; }
         watch = relres;
       }
     }
-    if ((rc = precond_apply(S->r, S->z))) return rc;
     if (flexible) {
-      nhist = std::min(nhist + 1, kFcgSlots - 1);
-      FcgHist h{};
-      h.nh = nhist;
-      for (int k = 0; k < nhist; k++) {
-        const int slot = (cur - k + kFcgSlots) % kFcgSlots;
-        h.p[k] = S->ph[slot];
-        h.slot[k] = slot;
-        ddot(S->z, S->qh[slot], SC_ZQH + slot);
-      }
+      const int nh = std::min(nhist + 1, kFcgSlots - 1);
+      if ((rc = run_b(cur, nh, nh >= hist_full && nhist >= hist_full))) { release(); return rc; }
+      nhist = nh;
       cur = (cur + 1) % kFcgSlots;
-      p = S->ph[cur];
-      q = S->qh[cur];
-      hipLaunchKernelGGL(k_fcg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, h, p);
     } else {
-      ddot(S->r, S->z, SC_RZN);
-      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, S->scal, 0);
-      hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, p);
+      if ((rc = run_b(0, 0, true))) { release(); return rc; }
     }
   }
+  release();
   if (!stalled && relres > tol) {
     if ((rc = check(&relres, &brk))) return rc;
     if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
@@ -709,6 +790,14 @@ int pdg_qs_residual(pdg_ctx* h, const double* u, double* residual, double* norm)
   if (residual) std::copy(r.begin(), r.end(), residual);
   if (norm) *norm = std::sqrt(n2) + 20.0 * ninf;
   return PDG_OK;
+}
+
+int pdg_qs_set_graphs(pdg_ctx* h, int enable) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts) return PDG_E_ARG;
+  int rc = qs::ensure(c);
+  if (!rc) c->qs->graphs = enable != 0;
+  return rc;
 }
 
 int pdg_qs_fallback_count(pdg_ctx* h, int* count) {

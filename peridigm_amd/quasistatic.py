@@ -60,6 +60,12 @@ def newton(ctx: Context, u, tol=1e-6, max_newton=20, precond="block_jacobi", lin
     return u, dict(newton=ni.value, linear=li.value, residual=rn.value)
 
 
+def set_graphs(ctx: Context, enable=True):
+    """Replay the CG iteration sequences as HIP graphs, or launch every
+    kernel directly (the default)."""
+    ctx.check(ctx.lib.pdg_qs_set_graphs(ctx.h, int(bool(enable))))
+
+
 def fallback_count(ctx: Context):
     """Learned-preconditioner solves finished by block-Jacobi CG so far."""
     n = C.c_int()

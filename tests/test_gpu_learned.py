@@ -70,6 +70,12 @@ def test_training_and_flexible_cg():
         xl, itl, rell = Q.solve(ctx, u0, r, precond="learned", rel_tol=1e-8)
         print(f"iterations: block-Jacobi CG {itj}, learned FCG {itl}")
         assert rell <= 1e-8
+        # the flexible-CG iteration replayed from HIP graphs is bitwise the
+        # direct-launch one
+        Q.set_graphs(ctx, False)
+        xl0, itl0, rel0 = Q.solve(ctx, u0, r, precond="learned", rel_tol=1e-8)
+        Q.set_graphs(ctx, True)
+        assert itl0 == itl and rel0 == rell and np.array_equal(xl0, xl)
         m = ctx.initialize()
     fixed = Q.end_dofs(p["left"], p["right"])
     _, jw = O.force_directional(x, u0, xl, m, vol, np.zeros(O.num_bonds(nl, N)), nl, p["K"], p["mu"], p["horizon"])

@@ -111,6 +111,49 @@ def test_compute_force_matches_oracle(mu, lanes, variant):
     assert _rel(f, fo) <= FORCE_TOL
 
 
+@pytest.mark.parametrize("lanes", [8, 16])
+def test_isolated_nodes_and_degree_over_128(lanes):
+    """Edge cases of the neighbour structure: (a) isolated nodes (empty
+    neighbourhoods, zero weighted volume) appended to a lattice bar run the
+    tiled kernels with theta = 0 and f = 0 there; (b) a dense random cloud
+    with degrees above 128 disables tiling and runs the untiled kernels
+    (8 lanes per node maps to 16 there). Both against the oracle."""
+    rng = np.random.default_rng(11)
+    x, vol, nl, hz = _bar(nx=20, ny=6, nz=5)
+    iso = np.array([[1.0, 1.0, 1.0], [2.0, -1.0, 0.5], [-3.0, 0.0, 0.0]])
+    x2 = np.vstack([x, iso]); vol2 = np.concatenate([vol, np.full(3, vol[0])])
+    nl2 = np.concatenate([nl, np.zeros(3, np.int32)])
+    y2 = x2 + 1e-7 * rng.standard_normal(x2.shape) + 2e-4 * x2
+    bd = np.zeros(O.num_bonds(nl2, len(vol2)))
+    with _ctx(nl2, x2, vol2, 130e9, 60e9, hz, lanes=lanes, sc=5e-4) as ctx:
+        assert ctx.tiling()[0] > 0
+        m = ctx.initialize()
+        th, f = ctx.compute_force(y2, bd)
+        bd1, dmg, _ = ctx.compute_damage(y2, bd)
+    mo = O.weighted_volume(x2, vol2, nl2, hz)
+    tho, fo = O.compute_force(x2, y2, mo, vol2, bd, nl2, 130e9, 60e9, hz)
+    bdo, dmgo, _ = O.compute_damage(x2, y2, bd, nl2, 5e-4)
+    assert np.array_equal(m, mo) and np.all(m[-3:] == 0.0)
+    assert _rel(th, tho) <= FORCE_TOL and _rel(f, fo) <= FORCE_TOL
+    assert np.all(th[-3:] == 0.0) and np.all(f[-3:] == 0.0)
+    assert np.array_equal(bd1, bdo) and np.array_equal(dmg, dmgo)
+
+    n = 1500
+    xc = rng.uniform(0.0, 1.0, size=(n, 3))
+    R = 0.3
+    nlc = O.radius_search(xc, R)
+    deg = O.list_to_csr(nlc, n)[0]
+    assert np.diff(deg).max() > 128
+    volc = np.full(n, 1.0 / n)
+    yc = xc + 1e-4 * xc + 1e-6 * rng.standard_normal(xc.shape)
+    with _ctx(nlc, xc, volc, 130e9, 50e9, R, lanes=lanes) as ctx:
+        assert ctx.tiling()[0] == 0
+        m = ctx.initialize()
+        th, f = ctx.compute_force(yc)
+    tho, fo = O.compute_force(xc, yc, m, volc, np.zeros(O.num_bonds(nlc, n)), nlc, 130e9, 50e9, R)
+    assert _rel(th, tho) <= FORCE_TOL and _rel(f, fo) <= FORCE_TOL
+
+
 def test_compute_force_with_broken_bonds_and_influence():
     x, vol, nl, hz = _bar(16, 8, 6)
     rng = np.random.default_rng(4)
