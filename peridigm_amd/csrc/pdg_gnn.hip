@@ -117,26 +117,49 @@ __device__ void build_inputs(int n, const int* __restrict__ row, const int* __re
     const int node = node0 + r;
     X[r * ldx + c] = node < n ? hin[(size_t)node * D + c] : (unsigned short)0;
   }
-  // mean over neighbours: 4 threads per node, D/4 features each
+  // mean over neighbours: 4 threads per node, D/4 features each, read as one
+  // 8- or 16-byte vector per neighbour; four neighbours are loaded per round
+  // (independent partial sums, combined in a fixed order) so their loads
+  // overlap
   constexpr int CH = D / 4;
+  constexpr int NW = CH / 2;  // 32-bit words (two bf16 each) per chunk
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
   const int node = node0 + r;
-  float acc[CH];
+  float acc[4][CH];
 #pragma unroll
-  for (int c = 0; c < CH; c++) acc[c] = 0.0f;
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int c = 0; c < CH; c++) acc[u][c] = 0.0f;
   int deg = 0;
+  auto add_row = [&](float (&a)[CH], int j) {
+    const unsigned* w = reinterpret_cast<const unsigned*>(hin + (size_t)j * D + q * CH);
+    unsigned v[NW];
+#pragma unroll
+    for (int k = 0; k < NW; k++) v[k] = w[k];
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+      a[2 * k] += __uint_as_float(v[k] << 16);
+      a[2 * k + 1] += __uint_as_float(v[k] & 0xffff0000u);
+    }
+  };
   if (node < n) {
     const int rs = row[node], re = row[node + 1];
     deg = re - rs;
-    for (int b = rs; b < re; b++) {
-      const unsigned short* hj = hin + (size_t)col[b] * D + q * CH;
-#pragma unroll
-      for (int c = 0; c < CH; c++) acc[c] += bf2f(hj[c]);
+    int b = rs;
+    for (; b + 4 <= re; b += 4) {
+      const int j0 = col[b], j1 = col[b + 1], j2 = col[b + 2], j3 = col[b + 3];
+      add_row(acc[0], j0);
+      add_row(acc[1], j1);
+      add_row(acc[2], j2);
+      add_row(acc[3], j3);
     }
+    for (; b < re; b++) add_row(acc[0], col[b]);
   }
+#pragma unroll
+  for (int c = 0; c < CH; c++) acc[0][c] = (acc[0][c] + acc[1][c]) + (acc[2][c] + acc[3][c]);
   const float inv = deg > 0 ? 1.0f / (float)deg : 0.0f;
 #pragma unroll
-  for (int c = 0; c < CH; c++) X[r * ldx + D + q * CH + c] = f2bf(acc[c] * inv);
+  for (int c = 0; c < CH; c++) X[r * ldx + D + q * CH + c] = f2bf(acc[0][c] * inv);
 }
 
 // Forward of one layer over 64-node tiles.
@@ -287,9 +310,17 @@ __global__ __launch_bounds__(256) void k_bwd_gather(int n, const int* __restrict
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int node = t / DI, c = t - (t / DI) * DI;
   if (node >= n) return;
-  float s = gself[(size_t)node * DI + c];
-  for (int b = row[node]; b < row[node + 1]; b++) s += gagg[(size_t)col[b] * DI + c];
-  gin[(size_t)node * DI + c] = s;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+  const int re = row[node + 1];
+  int b = row[node];
+  for (; b + 4 <= re; b += 4) {  // four independent loads in flight
+    s0 += gagg[(size_t)col[b] * DI + c];
+    s1 += gagg[(size_t)col[b + 1] * DI + c];
+    s2 += gagg[(size_t)col[b + 2] * DI + c];
+    s3 += gagg[(size_t)col[b + 3] * DI + c];
+  }
+  for (; b < re; b++) s0 += gagg[(size_t)col[b] * DI + c];
+  gin[(size_t)node * DI + c] = gself[(size_t)node * DI + c] + ((s0 + s1) + (s2 + s3));
 }
 
 // Deterministic sum of per-tile partials into the master gradient layout.
