@@ -639,8 +639,9 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   // and B (preconditioner, dots, new direction), split by the host check.
   // For flexible CG both depend on the history slot `cur`; once the history
   // is full (nhist = kFcgSlots - 1) they are the same every time that slot
-  // comes round. With graphs on, each (sequence, slot) is captured once per
-  // solve and replayed.
+  // comes round. With graphs on, each (preconditioner, sequence, slot) is
+  // captured once (kept in the solver state until its vectors or the
+  // network's buffers change) and replayed.
   auto seq_a = [&](int cs) -> int {
     double* pp = flexible ? S->ph[cs] : S->p;
     double* qq = flexible ? S->qh[cs] : S->q;
@@ -686,7 +687,6 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   }
   hipGraphExec_t* ga = S->gexec[precond][0];
   hipGraphExec_t* gb = S->gexec[precond][1];
-  auto release = [&]() {};
   auto capture = [&](hipGraphExec_t* out, auto&& body) -> int {
     hipGraph_t g = nullptr;
     int r2 = c->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -713,11 +713,11 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   };
   const int hist_full = flexible ? kFcgSlots - 1 : 0;
   while (relres > tol && it < maxit) {
-    if ((rc = run_a(cur, nhist >= hist_full))) { release(); return rc; }
+    if ((rc = run_a(cur, nhist >= hist_full))) return rc;
     it++;
     if (it <= 8 || it % 4 == 0 || it == maxit) {
-      if ((rc = check(&relres, &brk))) { release(); return rc; }
-      if (brk) { release(); return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0"); }
+      if ((rc = check(&relres, &brk))) return rc;
+      if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
       if (relres <= tol) break;
       if (flexible && it % 48 == 0) {
         if (!(relres < 0.5 * watch)) { stalled = true; break; }
@@ -726,14 +726,13 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     }
     if (flexible) {
       const int nh = std::min(nhist + 1, kFcgSlots - 1);
-      if ((rc = run_b(cur, nh, nh >= hist_full && nhist >= hist_full))) { release(); return rc; }
+      if ((rc = run_b(cur, nh, nh >= hist_full && nhist >= hist_full))) return rc;
       nhist = nh;
       cur = (cur + 1) % kFcgSlots;
     } else {
-      if ((rc = run_b(0, 0, true))) { release(); return rc; }
+      if ((rc = run_b(0, 0, true))) return rc;
     }
   }
-  release();
   if (!stalled && relres > tol) {
     if ((rc = check(&relres, &brk))) return rc;
     if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
