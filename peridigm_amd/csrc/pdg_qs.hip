@@ -60,7 +60,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int QG = 16;  // lanes per node
+constexpr int QG = 8;   // lanes per node
 constexpr int kScal = 32;   // solver scalar slots on the device
 constexpr int kFcgSlots = 4;  // flexible CG keeps kFcgSlots - 1 previous directions
 constexpr int RB = 256; // reduction block
