@@ -100,6 +100,27 @@ def cpu_baseline(steps=20):
                        f"(damage, dilatation, scatter force), {per_step*1e3:.1f} ms/step")
 
 
+def copy_bandwidth(mib=1024, reps=10):
+    """Measured device copy bandwidth (GB/s, bytes read + written) of a plain
+    device-to-device copy of `mib` MiB, timed with events on the current
+    stream: the practical ceiling beside the 8 TB/s spec."""
+    import torch
+    src = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    return 2.0 * mib * 1024 * 1024 / sec / 1e9
+
+
 def qs_figure(nsamples=100, epochs=20):
     """Quasi-static tensile bar 60x20x20 (24k nodes), 10 load steps to 1e-3
     strain, every linear solve to 1e-8 relative residual: wall time with
@@ -218,6 +239,7 @@ def main():
     st = ctx.download_state()
     ctx.close()
 
+    copy_gbs = copy_bandwidth()
     profiled = args.workload == "plate" and args.lanes == 8 and args.variant == 0
     traffic2 = MEASURED_TRAFFIC["pass2"] if profiled else None
     ms_per_step = elapsed / args.steps * 1e3
@@ -263,6 +285,7 @@ def main():
             "pass1_traffic": MEASURED_TRAFFIC["pass1"] if profiled else None,
             "algorithmic_bytes": bytes_pass2(B, N),
             "step_frac": (bytes_pass1(B, N) + bytes_pass2(B, N)) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "copy_gbs": copy_gbs, "frac_of_copy": a2 / copy_gbs,
         },
     }
     if args.qs:
