@@ -34,6 +34,33 @@ __global__ void k_set_u_disp(int n, const double* __restrict__ u, const int* __r
   rec[k].u[c] = u[3 * perm[k] + c] * (1.0 / L);
 }
 
+// Hilbert index of a 21-bit cell triple (Skilling's transpose algorithm:
+// undo excess work, Gray-encode, then interleave with axis 0 most significant).
+uint64_t spread3(uint64_t v);
+uint64_t hilbert3(uint64_t a, uint64_t b, uint64_t c) {
+  uint32_t X[3] = {(uint32_t)a, (uint32_t)b, (uint32_t)c};
+  const uint32_t M = 1u << 20;
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+    for (int i = 0; i < 3; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; i++) X[i] ^= t;
+  return spread3(X[2]) | (spread3(X[1]) << 1) | (spread3(X[0]) << 2);
+}
+
 uint64_t spread3(uint64_t v) {
   v &= 0x1fffff;
   v = (v | v << 32) & 0x1f00000000ffffull;
@@ -333,7 +360,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
       double t = std::floor((x[3 * (size_t)i + d] - lo[d]) / cell);
       q[d] = (uint64_t)std::min(std::max(t, 0.0), 2097151.0);
     }
-    key[i] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    key[i] = hilbert3(q[0], q[1], q[2]);
   }
   c->perm.resize(N);
   std::iota(c->perm.begin(), c->perm.end(), 0);
