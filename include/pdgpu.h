@@ -57,12 +57,12 @@ const char* pdg_last_error(const pdg_ctx* ctx);
  * untiled kernels use 16 when 8 is set). */
 int pdg_set_lanes_per_node(pdg_ctx* ctx, int lanes);
 
-/* Bond-kernel variant: 0 (default) LDS-staged Morton tiles when every tile
+/* Bond-kernel variant: 0 (default) LDS-staged Hilbert tiles when every tile
  * halo fits in LDS, else the untiled kernels; 1 forces the untiled kernels
  * (one 64-byte neighbour-record gather per bond); 2 is variant 0 with plain
  * (branching, unpipelined) bond loops, same arithmetic. */
 int pdg_set_kernel_variant(pdg_ctx* ctx, int variant);
-/* Number of Morton tiles (0: tiling unavailable) and total halo entries. */
+/* Number of Hilbert tiles (0: tiling unavailable) and total halo entries. */
 int pdg_get_tiling(const pdg_ctx* ctx, int* numTiles, long long* haloTotal);
 
 /* ---- topology and constants -------------------------------------------
@@ -71,7 +71,7 @@ int pdg_get_tiling(const pdg_ctx* ctx, int* numTiles, long long* haloTotal);
  * (single device, no ghosts) and the list must be symmetric (j in N(i) iff
  * i in N(j)): forces are gathered from both bond ends without atomics.
  * pdg_set_constants: Model_Coordinates x (3N), Volume (N), material and
- * damage constants (s_crit <= 0: no damage model). Builds the Morton node
+ * damage constants (s_crit <= 0: no damage model). Builds the Hilbert node
  * order, the CSR bond arrays, the bond-damage bit masks and the device state.
  */
 int pdg_set_topology(pdg_ctx* ctx, int numOwnedPoints, const int* ownedIDs,

@@ -160,7 +160,7 @@ class Context:
         return g.value
 
     def set_kernel_variant(self, variant):
-        """0: LDS-staged Morton tiles (default), 1: untiled gathers."""
+        """0: LDS-staged Hilbert tiles (default), 1: untiled gathers."""
         self.check(self.lib.pdg_set_kernel_variant(self.h, int(variant)))
 
     def tiling(self):

@@ -122,7 +122,7 @@ TileArgs Ctx::tile_args() const {
   return t;
 }
 
-// Greedy Morton tiling: extend a tile node by node while its halo (own nodes
+// Greedy tiling along the Hilbert order: extend a tile node by node while its halo (own nodes
 // plus all their neighbours) stays within kTileHaloMax and it holds at most
 // kTileNodesMax nodes.
 int Ctx::build_tiles() {
@@ -348,7 +348,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   c->x.assign(x, x + 3 * (size_t)N);
   c->vol.assign(vol, vol + N);
 
-  // Morton order of the nodes (cells of horizon/2), ties by original id.
+  // Hilbert-curve order of the nodes (cells of horizon/2), ties by original id.
   double lo[3] = {1e300, 1e300, 1e300};
   for (int i = 0; i < N; i++)
     for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], x[3 * (size_t)i + d]);

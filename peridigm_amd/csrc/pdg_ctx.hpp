@@ -29,7 +29,7 @@ struct Ctx {
   std::vector<int> nlist;
   std::vector<long long> head;
 
-  // Morton order and internal CSR
+  // Hilbert order and internal CSR
   std::vector<int> perm, iperm;  // perm[k]: original id of internal node k
   std::vector<int> row, col;     // CSR over internal ids, rows sorted
   std::vector<int> src;          // internal bond -> list-order bond index
@@ -63,7 +63,7 @@ struct Ctx {
   unsigned long long* d_near = nullptr;
   int* d_perm = nullptr;
 
-  // LDS-staged Morton tiles (empty if a node's neighbourhood exceeds the
+  // LDS-staged Hilbert tiles (empty if a node's neighbourhood exceeds the
   // halo capacity; the untiled kernels are used then)
   int variant = 0;  // 0: tiled (pipelined loops), 1: untiled, 2: tiled (plain loops)
   int ntiles = 0;

@@ -75,7 +75,7 @@ struct VerletArgs {
   int* err;           // set to 1 on a non-finite force
 };
 
-// Morton tiles of the LDS-staged kernels: a tile is a contiguous range of
+// Hilbert tiles of the LDS-staged kernels: a tile is a contiguous range of
 // internal nodes [tile_start[t], tile_start[t+1]); its halo (sorted internal
 // ids, own nodes included) is staged in LDS; lcol[b] is 8 times the position
 // of bond b's neighbour inside its tile's halo (its byte offset in the 8-byte

@@ -1,7 +1,7 @@
-// LDS-staged bond passes over Morton tiles.
+// LDS-staged bond passes over Hilbert tiles.
 //
 // One workgroup (8 waves) owns a tile: a contiguous range of at most 128
-// Morton-ordered nodes. It first stages the FP64 reference positions, FP64
+// Hilbert-ordered nodes. It first stages the FP64 reference positions, FP64
 // displacements and per-node scalars of the tile's halo (the union of the
 // tile's nodes and all their neighbours, at most 800 nodes) in LDS with one
 // 64-byte record load per halo node, then runs group-of-G-lanes bond loops
