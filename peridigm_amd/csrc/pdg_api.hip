@@ -122,10 +122,59 @@ TileArgs Ctx::tile_args() const {
   return t;
 }
 
+// Internal CSR from perm: rows sorted by internal neighbour id, src = list-order
+// bond index, rev = reverse bond, mw = first mask word of each row.
+int Ctx::build_csr() {
+  iperm.resize(N);
+  for (int k = 0; k < N; k++) iperm[perm[k]] = k;
+
+  // Internal CSR, rows sorted by internal neighbour id; src = list-order bond index.
+  row.assign(N + 1, 0);
+  for (int k = 0; k < N; k++) row[k + 1] = row[k] + nlist[head[perm[k]]];
+  col.resize(B);
+  src.resize(B);
+  std::vector<long long> bond0(N);  // first list-order bond index of each original node
+  {
+    long long b = 0;
+    for (int p = 0; p < N; p++) { bond0[p] = b; b += nlist[head[p]]; }
+  }
+  std::vector<std::pair<int, int>> tmp;
+  for (int k = 0; k < N; k++) {
+    const int p = perm[k];
+    const long long h0 = head[p];
+    const int nn = nlist[h0];
+    tmp.resize(nn);
+    for (int q = 0; q < nn; q++) tmp[q] = {iperm[nlist[h0 + 1 + q]], (int)(bond0[p] + q)};
+    std::sort(tmp.begin(), tmp.end());
+    for (int q = 0; q < nn; q++) {
+      if (q > 0 && tmp[q].first == tmp[q - 1].first)
+        return fail(PDG_E_UNSUPPORTED, "duplicate neighbour in neighbourhood list");
+      col[row[k] + q] = tmp[q].first;
+      src[row[k] + q] = tmp[q].second;
+    }
+  }
+  // Reverse bonds; the gather-only force needs a symmetric list.
+  rev.resize(B);
+  for (int k = 0; k < N; k++)
+    for (int b = row[k]; b < row[k + 1]; b++) {
+      const int j = col[b];
+      const int* s = col.data() + row[j];
+      const int* e = col.data() + row[j + 1];
+      const int* it = std::lower_bound(s, e, k);
+      if (it == e || *it != k)
+        return fail(PDG_E_UNSUPPORTED, "neighbourhood list is not symmetric");
+      rev[b] = (int)(it - col.data());
+    }
+  mw.assign(N + 1, 0);
+  for (int k = 0; k < N; k++) mw[k + 1] = mw[k] + (row[k + 1] - row[k] + 63) / 64;
+  return 0;
+}
+
 // Greedy tiling along the Hilbert order: extend a tile node by node while its halo (own nodes
 // plus all their neighbours) stays within kTileHaloMax and it holds at most
-// kTileNodesMax nodes.
-int Ctx::build_tiles() {
+// kTileNodesMax nodes. With `bounds`, the tiles are those node ranges instead
+// (each must still fit).
+int Ctx::build_tiles(const std::vector<int>* bounds) {
   ntiles = 0;
   tile_start.assign(1, 0);
   halo_off.assign(1, 0);
@@ -141,7 +190,8 @@ int Ctx::build_tiles() {
     const int tid = ntiles;
     H.clear();
     int i = n0;
-    while (i < N && i - n0 < kTileNodesMax) {
+    const int iend = bounds ? (*bounds)[tid + 1] : N;
+    while (i < iend && i - n0 < kTileNodesMax) {
       int add = mark[i] != tid ? 1 : 0;
       for (int b = row[i]; b < row[i + 1]; b++) add += mark[col[b]] != tid ? 1 : 0;
       if ((int)H.size() + add > kTileHaloMax) {
@@ -153,6 +203,7 @@ int Ctx::build_tiles() {
         if (mark[col[b]] != tid) { mark[col[b]] = tid; H.push_back(col[b]); }
       i++;
     }
+    if (bounds && i != iend) { ntiles = 0; tile_start.assign(1, 0); return 1; }
     std::sort(H.begin(), H.end());
     own_off.push_back((int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin()));
     for (int k = n0; k < i; k++)
@@ -366,50 +417,28 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   std::iota(c->perm.begin(), c->perm.end(), 0);
   std::sort(c->perm.begin(), c->perm.end(),
             [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
-  c->iperm.resize(N);
-  for (int k = 0; k < N; k++) c->iperm[c->perm[k]] = k;
-
-  // Internal CSR, rows sorted by internal neighbour id; src = list-order bond index.
-  c->row.assign(N + 1, 0);
-  for (int k = 0; k < N; k++) c->row[k + 1] = c->row[k] + c->nlist[c->head[c->perm[k]]];
-  c->col.resize(c->B);
-  c->src.resize(c->B);
-  std::vector<long long> bond0(N);  // first list-order bond index of each original node
-  {
-    long long b = 0;
-    for (int p = 0; p < N; p++) { bond0[p] = b; b += c->nlist[c->head[p]]; }
-  }
-  std::vector<std::pair<int, int>> tmp;
-  for (int k = 0; k < N; k++) {
-    const int p = c->perm[k];
-    const long long h0 = c->head[p];
-    const int nn = c->nlist[h0];
-    tmp.resize(nn);
-    for (int q = 0; q < nn; q++) tmp[q] = {c->iperm[c->nlist[h0 + 1 + q]], (int)(bond0[p] + q)};
-    std::sort(tmp.begin(), tmp.end());
-    for (int q = 0; q < nn; q++) {
-      if (q > 0 && tmp[q].first == tmp[q - 1].first)
-        return c->fail(PDG_E_UNSUPPORTED, "duplicate neighbour in neighbourhood list");
-      c->col[c->row[k] + q] = tmp[q].first;
-      c->src[c->row[k] + q] = tmp[q].second;
+  int rc0 = c->build_csr();
+  if (rc0) return rc0;
+  // Inside each tile, nodes by descending degree (ties in curve order): a
+  // wave's node groups then share one trip count, which cuts the lanes idle
+  // in the bond loops (plate: 84% -> 94% busy). The tiles keep their node
+  // sets, so their halos are unchanged.
+  if (c->build_tiles() == 0 && c->ntiles > 0) {
+    const std::vector<int> bounds = c->tile_start;
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    for (int t = 0; t < c->ntiles; t++) {
+      auto deg = [&](int k) { return c->row[k + 1] - c->row[k]; };
+      std::stable_sort(order.begin() + bounds[t], order.begin() + bounds[t + 1],
+                       [&](int a, int b) { return deg(a) > deg(b); });
     }
+    std::vector<int> p2(N);
+    for (int k = 0; k < N; k++) p2[k] = c->perm[order[k]];
+    c->perm.swap(p2);
+    if ((rc0 = c->build_csr())) return rc0;
+    if (c->build_tiles(&bounds) != 0 || c->ntiles != (int)bounds.size() - 1)
+      return c->fail(PDG_E_UNSUPPORTED, "tiles changed under the in-tile node order");
   }
-  // Reverse bonds; the gather-only force needs a symmetric list.
-  c->rev.resize(c->B);
-  for (int k = 0; k < N; k++)
-    for (int b = c->row[k]; b < c->row[k + 1]; b++) {
-      const int j = c->col[b];
-      const int* s = c->col.data() + c->row[j];
-      const int* e = c->col.data() + c->row[j + 1];
-      const int* it = std::lower_bound(s, e, k);
-      if (it == e || *it != k)
-        return c->fail(PDG_E_UNSUPPORTED, "neighbourhood list is not symmetric");
-      c->rev[b] = (int)(it - c->col.data());
-    }
-  c->mw.assign(N + 1, 0);
-  for (int k = 0; k < N; k++) c->mw[k + 1] = c->mw[k] + (c->row[k + 1] - c->row[k] + 63) / 64;
-
-  c->build_tiles();
 
   // Scales: L and S powers of two near the horizon and the mean volume.
   double vsum = 0.0;

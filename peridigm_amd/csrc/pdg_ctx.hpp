@@ -77,7 +77,8 @@ struct Ctx {
   bool tiled() const { return variant != 1 && ntiles > 0; }
   bool pipe() const { return variant != 2; }
   TileArgs tile_args() const;
-  int build_tiles();
+  int build_csr();
+  int build_tiles(const std::vector<int>* bounds = nullptr);
 
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
