@@ -15,7 +15,7 @@ stream it runs on; `cpu_baseline` times the FP64 CPU oracle (a scalar
 restatement of the reference loops, one core) on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plate|bar|cylinder|batched]
-                    [--lanes 8|16|32|64] [--persistent] [--qs]
+                    [--lanes 4|8|16|32|64] [--persistent] [--qs]
 """
 from __future__ import annotations
 
@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 # Measured HBM bytes per launch of the default configuration's kernels (plate
-# 400x400x4, 8 lanes per node, LDS-tiled pipelined loops): rocprofv3 --pmc
+# 400x400x4, 4 lanes per node, LDS-tiled pipelined loops): rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean of 73 dispatches), each
 # scaled by what the counter reads for a plain elementwise kernel that reads
 # and writes 256 MiB (FETCH_SIZE reads half: x2.0; WRITE_SIZE exact: x1.0).
@@ -169,7 +169,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--workload", default="plate")
-    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled, 2: tiles with plain loops")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qs", action="store_true",
@@ -240,7 +240,7 @@ def main():
     ctx.close()
 
     copy_gbs = copy_bandwidth()
-    profiled = args.workload == "plate" and args.lanes == 8 and args.variant == 0
+    profiled = args.workload == "plate" and args.lanes == 4 and args.variant == 0
     traffic2 = MEASURED_TRAFFIC["pass2"] if profiled else None
     ms_per_step = elapsed / args.steps * 1e3
     value = B * args.steps / elapsed

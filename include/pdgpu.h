@@ -53,8 +53,8 @@ int pdg_ctx_create(int device, pdg_ctx** out);
 int pdg_ctx_destroy(pdg_ctx* ctx);
 const char* pdg_last_error(const pdg_ctx* ctx);
 
-/* Lanes per node used by the bond kernels (8, 16, 32 or 64; default 8; the
- * untiled kernels use 16 when 8 is set). */
+/* Lanes per node used by the bond kernels (4, 8, 16, 32 or 64; default 4;
+ * the untiled kernels use 16 when 4 or 8 is set). */
 int pdg_set_lanes_per_node(pdg_ctx* ctx, int lanes);
 
 /* Bond-kernel variant: 0 (default) LDS-staged Hilbert tiles when every tile
@@ -132,7 +132,7 @@ int pdg_verlet_run(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEv
 /* Same run as pdg_verlet_run in ONE persistent cooperative launch (grid-wide
  * barriers between the two passes of every step), for batched multi-specimen
  * sweeps: the specimens are concatenated into one node/bond space. Needs the
- * tiled kernels and 8 or 16 lanes per node. *gridSize receives the workgroups used. */
+ * tiled kernels and 4, 8 or 16 lanes per node. *gridSize receives the workgroups used. */
 int pdg_verlet_run_persistent(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEval,
                               int* gridSize);
 

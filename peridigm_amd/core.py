@@ -32,7 +32,7 @@ def _p(a):
 class Context:
     """pdg_ctx on HIP device `device`."""
 
-    def __init__(self, device: int = 0, lanes: int = 8, variant: int = 0):
+    def __init__(self, device: int = 0, lanes: int = 4, variant: int = 0):
         self.lib = _lib.load()
         h = C.c_void_p()
         rc = self.lib.pdg_ctx_create(device, C.byref(h))

@@ -313,8 +313,8 @@ const char* pdg_last_error(const pdg_ctx* h) {
 int pdg_set_lanes_per_node(pdg_ctx* h, int lanes) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
-  if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
-    return c->fail(PDG_E_ARG, "lanes per node must be 8, 16, 32 or 64");
+  if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+    return c->fail(PDG_E_ARG, "lanes per node must be 4, 8, 16, 32 or 64");
   c->lanes = lanes;
   return PDG_OK;
 }
@@ -805,8 +805,8 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
   if (!c->tiled()) return c->fail(PDG_E_UNSUPPORTED, "persistent run needs the tiled kernels");
-  if (c->lanes != 8 && c->lanes != 16)
-    return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 8 or 16 lanes per node");
+  if (c->lanes != 4 && c->lanes != 8 && c->lanes != 16)
+    return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 4, 8 or 16 lanes per node");
   if (!initialEval && nsteps == 0) return PDG_OK;
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;

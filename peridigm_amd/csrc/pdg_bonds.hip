@@ -236,7 +236,7 @@ inline unsigned blocks_for(int n, int lanes) {
 
 hipError_t launch_pass1(const BondArgs& a, NodeRec* rec, float* theta, int* nbroken,
                         bool damage, bool count_near, int lanes, hipStream_t s) {
-  if (lanes == 8) lanes = 16;  // the untiled kernels use 16, 32 or 64 lanes
+  if (lanes < 16) lanes = 16;  // the untiled kernels use 16, 32 or 64 lanes
   if (a.n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(a.n, lanes)), block(256);
 #define PDG_P1(G)                                                                        \
@@ -255,7 +255,7 @@ hipError_t launch_pass1(const BondArgs& a, NodeRec* rec, float* theta, int* nbro
 
 hipError_t launch_pass2_force(const BondArgs& a, const NodeRec* rec, double* f, int lanes,
                               hipStream_t s) {
-  if (lanes == 8) lanes = 16;  // the untiled kernels use 16, 32 or 64 lanes
+  if (lanes < 16) lanes = 16;  // the untiled kernels use 16, 32 or 64 lanes
   if (a.n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(a.n, lanes)), block(256);
   VerletArgs va{};
@@ -270,7 +270,7 @@ hipError_t launch_pass2_force(const BondArgs& a, const NodeRec* rec, double* f, 
 
 hipError_t launch_pass2_verlet(const BondArgs& a, const NodeRec* rec, NodeRec* rec_next,
                                const VerletArgs& va, int lanes, hipStream_t s) {
-  if (lanes == 8) lanes = 16;  // the untiled kernels use 16, 32 or 64 lanes
+  if (lanes < 16) lanes = 16;  // the untiled kernels use 16, 32 or 64 lanes
   if (a.n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(a.n, lanes)), block(256);
   if (lanes == 16)

@@ -16,7 +16,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
-  int lanes = 8;
+  int lanes = 4;
   bool per_pass_timing = false;
   double t_total = 0.0, t_pass1 = 0.0, t_pass2 = 0.0;
   int n_pass = 0;                 // force evaluations timed in the last run

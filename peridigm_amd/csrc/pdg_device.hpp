@@ -35,15 +35,15 @@ __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
 }
 
-// Sum over aligned groups of G lanes (8, 16, 32 or 64), result in every lane
-// of the group. Within a 16-lane row it takes up to four DPP adds (quad
+// Sum over aligned groups of G lanes (4, 8, 16, 32 or 64), result in every
+// lane of the group. Within a 16-lane row it takes up to four DPP adds (quad
 // pairs, quads, half rows, rows); wider groups add shuffles across rows. Call
 // with all lanes active.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
   v += dpp_f<DPP_QXOR1>(v);
   v += dpp_f<DPP_QXOR2>(v);
-  v += dpp_f<DPP_HMIRROR>(v);
+  if (G >= 8) v += dpp_f<DPP_HMIRROR>(v);
   if (G >= 16) v += dpp_f<DPP_MIRROR>(v);
   if (G >= 32) v += __shfl_xor(v, 16);
   if (G >= 64) v += __shfl_xor(v, 32);
@@ -54,7 +54,7 @@ template <int G>
 __device__ __forceinline__ int group_sum_i(int v) {
   v += dpp_i<DPP_QXOR1>(v);
   v += dpp_i<DPP_QXOR2>(v);
-  v += dpp_i<DPP_HMIRROR>(v);
+  if (G >= 8) v += dpp_i<DPP_HMIRROR>(v);
   if (G >= 16) v += dpp_i<DPP_MIRROR>(v);
   if (G >= 32) v += __shfl_xor(v, 16);
   if (G >= 64) v += __shfl_xor(v, 32);

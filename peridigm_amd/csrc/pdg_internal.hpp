@@ -112,7 +112,8 @@ hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0
                              double t0, int initial_eval, bool damage, int lanes, int* grid_out,
                              hipStream_t s);
 
-// Launchers (pdg_bonds.hip). `lanes` = lanes per node (16, 32 or 64).
+// Launchers (pdg_bonds.hip). `lanes` = lanes per node (16, 32 or 64; smaller
+// values run as 16).
 hipError_t launch_pass1(const BondArgs& a, NodeRec* rec, float* theta, int* nbroken,
                         bool damage, bool count_near, int lanes, hipStream_t s);
 hipError_t launch_pass2_force(const BondArgs& a, const NodeRec* rec, double* f, int lanes,

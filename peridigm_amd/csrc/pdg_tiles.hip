@@ -39,10 +39,11 @@ using namespace dev;
 
 constexpr int HMAX = kTileHaloMax;
 constexpr int WG = 512;
-// Workgroups per CU the register allocation is tuned for: 49.5 KB of LDS per
-// workgroup (halo capacity 800) leaves room for three; measured faster than
-// the two that a 1024-node halo allows.
-constexpr int WG_PER_CU = 3;
+// Minimum waves per SIMD the register allocation must allow (the second
+// __launch_bounds__ argument): 4, i.e. two 512-thread workgroups per CU, so at
+// most 128 VGPRs. The 52.7 KB of LDS per workgroup (halo capacity 800) would
+// admit three, but that needs <= 80 VGPRs, which spills and measured slower.
+constexpr int WAVES_PER_SIMD = 4;
 constexpr int MAXDEG = 128;
 constexpr int NWORDS = MAXDEG / 64;
 
@@ -286,7 +287,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
 }
 
 template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
-__global__ __launch_bounds__(WG, WG_PER_CU) void k_pass1_tiled(BondArgs a, TileArgs t,
+__global__ __launch_bounds__(WG, WAVES_PER_SIMD) void k_pass1_tiled(BondArgs a, TileArgs t,
                                                     NodeRec* __restrict__ rec,
                                                     float* __restrict__ theta,
                                                     int* __restrict__ nbroken) {
@@ -404,7 +405,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
 }
 
 template <int G, bool VERLET, int INFL, bool PIPE>
-__global__ __launch_bounds__(WG, WG_PER_CU) void k_pass2_tiled(BondArgs a, TileArgs t,
+__global__ __launch_bounds__(WG, WAVES_PER_SIMD) void k_pass2_tiled(BondArgs a, TileArgs t,
                                                     const NodeRec* __restrict__ rec,
                                                     NodeRec* __restrict__ rec_next,
                                                     double* __restrict__ fout, VerletArgs va) {
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(WG, WG_PER_CU) void k_pass2_tiled(BondArgs a, TileA
 // order. Co-residency of the whole grid is guaranteed by the cooperative
 // launch, so the barriers always complete.
 template <int G, bool DAMAGE, int INFL>
-__global__ __launch_bounds__(WG, WG_PER_CU) void k_persistent(BondArgs a, TileArgs t, NodeRec* rec0,
+__global__ __launch_bounds__(WG, WAVES_PER_SIMD) void k_persistent(BondArgs a, TileArgs t, NodeRec* rec0,
                                                    NodeRec* rec1, float* theta, int* nbroken,
                                                    VerletArgs va, int nsteps, double t0,
                                                    int initial_eval) {
@@ -524,8 +525,10 @@ hipError_t launch_persistent(const BondArgs& a, const TileArgs& t, NodeRec* rec0
                              double t0, int initial_eval, bool damage, int lanes, int* grid_out,
                              hipStream_t s) {
   if (t.ntiles <= 0) return hipErrorInvalidValue;
-  if (lanes != 8 && lanes != 16) return hipErrorInvalidValue;
-  void* fn = lanes == 8 ? persistent_fn<8>(a.influence, damage) : persistent_fn<16>(a.influence, damage);
+  if (lanes != 4 && lanes != 8 && lanes != 16) return hipErrorInvalidValue;
+  void* fn = lanes == 4   ? persistent_fn<4>(a.influence, damage)
+             : lanes == 8 ? persistent_fn<8>(a.influence, damage)
+                          : persistent_fn<16>(a.influence, damage);
   int dev = 0, ncu = 0, per_cu = 0, coop = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
@@ -548,7 +551,8 @@ hipError_t launch_pass1_tiled(const BondArgs& a, const TileArgs& t, NodeRec* rec
                               int* nbroken, bool damage, bool count_near, int lanes, bool pipe,
                               hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
-  if (lanes == 8) p1_infl<8>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  if (lanes == 4) p1_infl<4>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  else if (lanes == 8) p1_infl<8>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else if (lanes == 16) p1_infl<16>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else if (lanes == 32) p1_infl<32>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else p1_infl<64>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
@@ -559,7 +563,8 @@ hipError_t launch_pass2_tiled_force(const BondArgs& a, const TileArgs& t, const 
                                     double* f, int lanes, bool pipe, hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
   VerletArgs va{};
-  if (lanes == 8) p2_infl<8, false>(a, t, rec, nullptr, f, va, pipe, s);
+  if (lanes == 4) p2_infl<4, false>(a, t, rec, nullptr, f, va, pipe, s);
+  else if (lanes == 8) p2_infl<8, false>(a, t, rec, nullptr, f, va, pipe, s);
   else if (lanes == 16) p2_infl<16, false>(a, t, rec, nullptr, f, va, pipe, s);
   else if (lanes == 32) p2_infl<32, false>(a, t, rec, nullptr, f, va, pipe, s);
   else p2_infl<64, false>(a, t, rec, nullptr, f, va, pipe, s);
@@ -570,7 +575,8 @@ hipError_t launch_pass2_tiled_verlet(const BondArgs& a, const TileArgs& t, const
                                      NodeRec* rec_next, const VerletArgs& va, int lanes, bool pipe,
                                      hipStream_t s) {
   if (t.ntiles <= 0) return hipSuccess;
-  if (lanes == 8) p2_infl<8, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
+  if (lanes == 4) p2_infl<4, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
+  else if (lanes == 8) p2_infl<8, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   else if (lanes == 16) p2_infl<16, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   else if (lanes == 32) p2_infl<32, true>(a, t, rec, rec_next, nullptr, va, pipe, s);
   else p2_infl<64, true>(a, t, rec, rec_next, nullptr, va, pipe, s);

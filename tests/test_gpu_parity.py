@@ -95,7 +95,7 @@ def _bar(nx=30, ny=8, nz=6, h=1e-3):
 
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("mu", [78e9, 45e9])
-@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
 def test_compute_force_matches_oracle(mu, lanes, variant):
     x, vol, nl, hz = _bar()
     rng = np.random.default_rng(1)
@@ -111,13 +111,13 @@ def test_compute_force_matches_oracle(mu, lanes, variant):
     assert _rel(f, fo) <= FORCE_TOL
 
 
-@pytest.mark.parametrize("lanes", [8, 16])
+@pytest.mark.parametrize("lanes", [4, 8, 16])
 def test_isolated_nodes_and_degree_over_128(lanes):
     """Edge cases of the neighbour structure: (a) isolated nodes (empty
     neighbourhoods, zero weighted volume) appended to a lattice bar run the
     tiled kernels with theta = 0 and f = 0 there; (b) a dense random cloud
     with degrees above 128 disables tiling and runs the untiled kernels
-    (8 lanes per node maps to 16 there). Both against the oracle."""
+    (4 and 8 lanes per node map to 16 there). Both against the oracle."""
     rng = np.random.default_rng(11)
     x, vol, nl, hz = _bar(nx=20, ny=6, nz=5)
     iso = np.array([[1.0, 1.0, 1.0], [2.0, -1.0, 0.5], [-3.0, 0.0, 0.0]])
@@ -293,7 +293,7 @@ def test_deterministic_and_lane_independent():
     p = plate(nx=40, ny=40, nz=4, notch_cells=10, pull=17.3)
     nl = O.filter_planes(p["x"], O.radius_search(p["x"], p["horizon"]), p["planes"])
     outs = []
-    for lanes, variant in ((16, 0), (16, 0), (16, 2), (8, 0), (32, 0), (64, 0), (16, 1), (64, 1)):
+    for lanes, variant in ((16, 0), (16, 0), (16, 2), (4, 0), (8, 0), (32, 0), (64, 0), (16, 1), (64, 1)):
         with Context(0, lanes=lanes, variant=variant) as ctx:
             ctx.set_topology(nl, len(p["volume"]))
             ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
