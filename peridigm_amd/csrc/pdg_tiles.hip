@@ -72,28 +72,31 @@ __device__ __forceinline__ float omega_t(float zeta, float hhat) {
 
 // Loads a node's mask words into registers. The mask array is padded by
 // NWORDS words and the index array by MAXDEG entries, so the loads past a
-// row's end stay in bounds and need no branch; what they read is replaced by
-// selects (words by 0; indices by 0, a staged slot, so the masked-out bonds
-// of inactive lanes still read initialised LDS and stay finite).
+// row's end stay in bounds and need no branch. Words past the row are
+// replaced by 0.
 __device__ __forceinline__ void load_words(const unsigned long long* __restrict__ mask, int w0,
                                            int deg, unsigned long long (&m)[NWORDS]) {
   const int nw = (deg + 63) >> 6;
+  const unsigned long long* __restrict__ p = mask + w0;
 #pragma unroll
   for (int w = 0; w < NWORDS; w++) {
-    const unsigned long long v = mask[(unsigned)(w0 + w)];
+    const unsigned long long v = p[w];
     m[w] = w < nw ? v : 0ull;
   }
 }
 
-// Prefetches a lane's local neighbour indices for the whole row.
+// Prefetches a lane's local neighbour indices for the whole row: one base
+// address, constant offsets. Entries past the row's end belong to later rows
+// (or the zero padding), so they are byte offsets below 8 * HMAX; the bonds
+// they stand for are masked out, and the LDS slots they address always hold
+// finite values (staged or zero-filled), so those bonds' arithmetic stays
+// finite.
 template <int G>
-__device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lcol, int rs, int deg,
-                                          int l, int (&lj)[MAXDEG / G]) {
+__device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lcol, int rs, int l,
+                                          int (&lj)[MAXDEG / G]) {
+  const unsigned short* __restrict__ p = lcol + rs + l;
 #pragma unroll
-  for (int it = 0; it < MAXDEG / G; it++) {
-    const int v = (int)lcol[(unsigned)(rs + it * G + l)];
-    lj[it] = it * G + l < deg ? v : 0;
-  }
+  for (int it = 0; it < MAXDEG / G; it++) lj[it] = (int)p[it * G];
 }
 // LDS staging arrays of one workgroup (shared by both passes).
 struct TileLds {
@@ -159,11 +162,18 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   double (&su)[3][HMAX] = L.su;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
-  for (int h = threadIdx.x; h < hn; h += WG) {
-    const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
-    sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
-    su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
-    L.sva[h].x = r.vol;
+  // slots past the halo are zero-filled: masked-out bonds may address them
+  for (int h = threadIdx.x; h < HMAX; h += WG) {
+    if (h < hn) {
+      const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
+      sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
+      su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
+      L.sva[h].x = r.vol;
+    } else {
+      sx[0][h] = 0.0; sx[1][h] = 0.0; sx[2][h] = 0.0;
+      su[0][h] = 0.0; su[1][h] = 0.0; su[2][h] = 0.0;
+      L.sva[h].x = 0.0f;
+    }
   }
   __syncthreads();
   const int own = t.own_off[tile];
@@ -191,7 +201,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     for (int w = 0; w < NWORDS; w++) mn[w] = 0ull;
     load_words(a.mask, w0, deg, mo);
     int lj[MAXDEG / G];
-    load_lcol<G>(t.lcol, rs, deg, l, lj);
+    load_lcol<G>(t.lcol, rs, l, lj);
     const float scz = c.sc;
     const int nmax = groups_max<G>((deg + G - 1) / G);
     float acc = 0.0f;
@@ -308,12 +318,20 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   double (&su)[3][HMAX] = L.su;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
-  for (int h = threadIdx.x; h < hn; h += WG) {
-    const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
-    sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
-    su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
-    L.sva[h] = make_float2(r.vol, r.alpha);
-    L.spx[h].x = r.p;
+  // slots past the halo are zero-filled: masked-out bonds may address them
+  for (int h = threadIdx.x; h < HMAX; h += WG) {
+    if (h < hn) {
+      const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
+      sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
+      su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
+      L.sva[h] = make_float2(r.vol, r.alpha);
+      L.spx[h].x = r.p;
+    } else {
+      sx[0][h] = 0.0; sx[1][h] = 0.0; sx[2][h] = 0.0;
+      su[0][h] = 0.0; su[1][h] = 0.0; su[2][h] = 0.0;
+      L.sva[h] = make_float2(0.0f, 0.0f);
+      L.spx[h].x = 0.0f;
+    }
   }
   __syncthreads();
   const int own = t.own_off[tile];
@@ -340,7 +358,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     unsigned long long mo[NWORDS];
     load_words(a.mask, w0, deg, mo);
     int lj[MAXDEG / G];
-    load_lcol<G>(t.lcol, rs, deg, l, lj);
+    load_lcol<G>(t.lcol, rs, l, lj);
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
       const int nmax = groups_max<G>((deg + G - 1) / G);
