@@ -100,21 +100,19 @@ __device__ __forceinline__ BondGeom bond_geom(double xi0, double xi1, double xi2
   const float e2 = (float)(uj2 - ui2);
   const float z2 = fmaf(b.x2, b.x2, fmaf(b.x1, b.x1, b.x0 * b.x0));
   b.zeta = fsqrt(z2);
+  // |Y|^2 = |xi|^2 + 2 xi.eta + eta.eta and eta.(xi + Y) = 2 xi.eta + eta.eta
+  // share the two small dot products
+  const float xe = fmaf(e2, b.x2, fmaf(e1, b.x1, e0 * b.x0));
+  const float ee = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+  const float dot = fmaf(2.0f, xe, ee);
+  const float yy = z2 + dot;
   if (NEED_RY) {
     b.y0 = b.x0 + e0;
     b.y1 = b.x1 + e1;
     b.y2 = b.x2 + e2;
-    const float yy = fmaf(b.y2, b.y2, fmaf(b.y1, b.y1, b.y0 * b.y0));
-    const float dot = fmaf(e2, b.x2 + b.y2, fmaf(e1, b.x1 + b.y1, e0 * (b.x0 + b.y0)));
     b.ry = frsq(yy);
     b.ext = dot * frcp(yy * b.ry + b.zeta);
   } else {
-    // without the direction: |Y|^2 = |xi|^2 + 2 xi.eta + eta.eta and
-    // eta.(xi + Y) = 2 xi.eta + eta.eta share the two small dot products
-    const float xe = fmaf(e2, b.x2, fmaf(e1, b.x1, e0 * b.x0));
-    const float ee = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
-    const float dot = fmaf(2.0f, xe, ee);
-    const float yy = z2 + dot;
     b.y0 = b.y1 = b.y2 = 0.0f;
     b.ry = 0.0f;
     b.ext = dot * frcp(fsqrt(yy) + b.zeta);

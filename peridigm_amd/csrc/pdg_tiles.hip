@@ -152,6 +152,26 @@ __device__ __forceinline__ bool mask_bit(const unsigned long long (&m)[NWORDS], 
   return ((half >> ((base & 31) + l)) & 1u) != 0u;
 }
 
+// Lane masks of the wave (compare results kept in scalar registers):
+// predicate codes of the compare builtins.
+enum { CMP_FOGT = 2, CMP_IEQ = 32, CMP_ISLT = 40 };
+
+// Lanes whose bond k = it * G + l is not marked broken in m.
+template <int G>
+__device__ __forceinline__ unsigned long long unbroken_lanes(const unsigned long long (&m)[NWORDS],
+                                                             int it, int l) {
+  const int base = it * G;
+  unsigned v;
+  if (G == 64) {
+    v = (unsigned)((m[base >> 6] >> l) & 1ull);
+  } else {
+    const unsigned long long w = m[base >> 6];
+    const unsigned half = (base & 32) ? (unsigned)(w >> 32) : (unsigned)w;
+    v = half & (1u << ((base & 31) + l));
+  }
+  return __builtin_amdgcn_uicmp(v, 0u, CMP_IEQ);
+}
+
 // Pass 1 (damage + dilatation) of one tile; every thread of the workgroup
 // calls it with the same tile.
 template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
@@ -204,13 +224,14 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     load_lcol<G>(t.lcol, rs, l, lj);
     const float scz = c.sc;
     const int nmax = groups_max<G>((deg + G - 1) / G);
+    const int nact = deg - l;  // bond it * G + l exists iff it * G < nact
     float acc = 0.0f;
     int nnear = 0;
     if (PIPE) {
       // branch-free bond loop: every lane computes, inactive and broken
       // bonds are selected away; the next bond's LDS data is in flight. The
-      // per-iteration break ballots stay wave-uniform (scalar registers) and
-      // are turned into the group's mask words after the loop.
+      // per-iteration ballots of new breaks stay wave-uniform (scalar
+      // registers) and are merged into the group's mask words after the loop.
       unsigned long long balv[MAXDEG / G];
       Nb<false> nb[2];
       nb[0] = nb_load<false>(L, lj[0]);
@@ -220,20 +241,24 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
           if (it + 1 < MAXDEG / G && it + 1 < nmax)
             nb[(it + 1) & 1] = nb_load<false>(L, lj[(it + 1) % (MAXDEG / G)]);
           const Nb<false>& cur = nb[it & 1];
-          const int k = it * G + l;
-          const bool act = k < deg;
-          bool brk = act && mask_bit<G>(mo, it, l);
+          // lanes whose bond exists and is unbroken; the masks stay in scalar
+          // registers, so combining them costs no vector instructions
+          const unsigned long long okm =
+              __builtin_amdgcn_sicmp(it * G, nact, CMP_ISLT) & unbroken_lanes<G>(mo, it, l);
           const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
                                                cur.x2, cur.u0, cur.u1, cur.u2);
+          unsigned long long brkm = 0ull;  // lanes whose bond breaks now
           if (DAMAGE) {
             const float lim = scz * b.zeta;
-            if (NEAR && act && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
-            brk = brk || (act && b.ext > lim);
+            if (NEAR && __builtin_amdgcn_inverse_ballot_w64(okm) &&
+                fabsf(b.ext - lim) <= a.near_rel * lim)
+              nnear++;
+            brkm = okm & __builtin_amdgcn_fcmpf(b.ext, lim, CMP_FOGT);
           }
           const float w = omega_t<INFL>(b.zeta, c.hhat);
           const float nacc = fmaf(w * b.zeta * b.ext, cur.vol, acc);
-          acc = (act && !brk) ? nacc : acc;
-          if (DAMAGE) balv[it] = __ballot(brk);
+          acc = __builtin_amdgcn_inverse_ballot_w64(okm & ~brkm) ? nacc : acc;
+          if (DAMAGE) balv[it] = brkm;
         } else if (DAMAGE) {
           balv[it] = 0ull;
         }
@@ -245,6 +270,8 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
               G == 64 ? balv[it] : ((balv[it] >> (g * G)) & ((1ull << (G & 63)) - 1ull));
           mn[(it * G) >> 6] |= gb << ((it * G) & 63);
         }
+#pragma unroll
+        for (int w = 0; w < NWORDS; w++) mn[w] |= mo[w];
       }
     } else {
 #pragma unroll
@@ -362,6 +389,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
       const int nmax = groups_max<G>((deg + G - 1) / G);
+      const int nact = deg - l;  // bond it * G + l exists iff it * G < nact
       Nb<true> nb[2];
       nb[0] = nb_load<true>(L, lj[0]);
 #pragma unroll
@@ -370,8 +398,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
           if (it + 1 < MAXDEG / G && it + 1 < nmax)
             nb[(it + 1) & 1] = nb_load<true>(L, lj[(it + 1) % (MAXDEG / G)]);
           const Nb<true>& cur = nb[it & 1];
-          const int k = it * G + l;
-          const bool ok = k < deg && !mask_bit<G>(mo, it, l);
+          const bool ok = it * G < nact && !mask_bit<G>(mo, it, l);
           const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
                                              cur.x2, cur.u0, cur.u1, cur.u2);
           const float w = omega_t<INFL>(b.zeta, hhat);
