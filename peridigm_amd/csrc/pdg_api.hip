@@ -170,50 +170,77 @@ int Ctx::build_csr() {
   return 0;
 }
 
-// Greedy tiling along the Hilbert order: extend a tile node by node while its halo (own nodes
-// plus all their neighbours) stays within kTileHaloMax and it holds at most
-// kTileNodesMax nodes. With `bounds`, the tiles are those node ranges instead
-// (each must still fit).
-int Ctx::build_tiles(const std::vector<int>* bounds) {
+// Greedy tile boundaries along the Hilbert order (positions in perm,
+// neighbours by reference id): extend a tile node by node while its halo (own
+// nodes plus all their neighbours) stays within kTileHaloMax and it holds at
+// most kTileNodesMax nodes. Returns 1 (no tiles) when a node has more than 128
+// neighbours or one node's halo alone does not fit.
+int Ctx::tile_bounds(std::vector<int>& bounds) const {
+  bounds.assign(1, 0);
+  for (int p = 0; p < N; p++)
+    if (nlist[head[p]] > 128) { bounds.assign(1, 0); return 1; }
+  std::vector<int> mark(N, -1);
+  int n0 = 0, tid = 0;
+  while (n0 < N) {
+    int i = n0, hs = 0;
+    while (i < N && i - n0 < kTileNodesMax) {
+      const int p = perm[i];
+      const long long h0 = head[p];
+      const int nn = nlist[h0];
+      int add = mark[p] != tid ? 1 : 0;
+      for (int q = 0; q < nn; q++) add += mark[nlist[h0 + 1 + q]] != tid ? 1 : 0;
+      if (hs + add > kTileHaloMax) {
+        if (i == n0) { bounds.assign(1, 0); return 1; }
+        break;
+      }
+      mark[p] = tid;
+      for (int q = 0; q < nn; q++) mark[nlist[h0 + 1 + q]] = tid;
+      hs += add;
+      i++;
+    }
+    bounds.push_back(i);
+    n0 = i;
+    tid++;
+  }
+  return 0;
+}
+
+// Tile arrays over the internal CSR for the node ranges in `bounds`: each
+// tile's halo (own nodes plus all their neighbours, sorted), the offset of its
+// own nodes in it, and every bond's byte offset into the LDS arrays. An empty
+// `bounds` (one entry) leaves the context untiled. Returns 1 if a halo does
+// not fit.
+int Ctx::build_tiles(const std::vector<int>& bounds) {
   ntiles = 0;
   tile_start.assign(1, 0);
   halo_off.assign(1, 0);
   halo.clear();
   own_off.clear();
   lcol.assign(B, 0);
-  for (int i = 0; i < N; i++)
-    if (row[i + 1] - row[i] > 128) { ntiles = 0; tile_start.assign(1, 0); return 1; }
+  if (bounds.size() < 2) return 1;
   std::vector<int> mark(N, -1);
   std::vector<int> H;
-  int n0 = 0;
-  while (n0 < N) {
-    const int tid = ntiles;
+  for (size_t t = 0; t + 1 < bounds.size(); t++) {
+    const int n0 = bounds[t], n1 = bounds[t + 1];
     H.clear();
-    int i = n0;
-    const int iend = bounds ? (*bounds)[tid + 1] : N;
-    while (i < iend && i - n0 < kTileNodesMax) {
-      int add = mark[i] != tid ? 1 : 0;
-      for (int b = row[i]; b < row[i + 1]; b++) add += mark[col[b]] != tid ? 1 : 0;
-      if ((int)H.size() + add > kTileHaloMax) {
-        if (i == n0) { ntiles = 0; tile_start.assign(1, 0); return 1; }
-        break;
-      }
-      if (mark[i] != tid) { mark[i] = tid; H.push_back(i); }
+    for (int i = n0; i < n1; i++) {
+      if (mark[i] != (int)t) { mark[i] = (int)t; H.push_back(i); }
       for (int b = row[i]; b < row[i + 1]; b++)
-        if (mark[col[b]] != tid) { mark[col[b]] = tid; H.push_back(col[b]); }
-      i++;
+        if (mark[col[b]] != (int)t) { mark[col[b]] = (int)t; H.push_back(col[b]); }
     }
-    if (bounds && i != iend) { ntiles = 0; tile_start.assign(1, 0); return 1; }
+    if ((int)H.size() > kTileHaloMax) {
+      ntiles = 0; tile_start.assign(1, 0); halo_off.assign(1, 0); halo.clear(); own_off.clear();
+      return 1;
+    }
     std::sort(H.begin(), H.end());
     own_off.push_back((int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin()));
-    for (int k = n0; k < i; k++)
+    for (int k = n0; k < n1; k++)
       for (int b = row[k]; b < row[k + 1]; b++)
         lcol[b] = (unsigned short)(8 * (std::lower_bound(H.begin(), H.end(), col[b]) - H.begin()));
     halo.insert(halo.end(), H.begin(), H.end());
     halo_off.push_back((int)halo.size());
-    tile_start.push_back(i);
+    tile_start.push_back(n1);
     ntiles++;
-    n0 = i;
   }
   return 0;
 }
@@ -417,28 +444,21 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   std::iota(c->perm.begin(), c->perm.end(), 0);
   std::sort(c->perm.begin(), c->perm.end(),
             [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+  // Tiles along the curve; inside each tile, nodes by descending degree
+  // (ties in curve order): a wave's node groups then share one trip count,
+  // which cuts the lanes idle in the bond loops (plate: 84% -> 94% busy). The
+  // tiles keep their node sets, so their halos are unchanged.
+  std::vector<int> bounds;
+  if (c->tile_bounds(bounds) == 0) {
+    for (size_t t = 0; t + 1 < bounds.size(); t++)
+      std::stable_sort(c->perm.begin() + bounds[t], c->perm.begin() + bounds[t + 1],
+                       [&](int a, int b) { return c->nlist[c->head[a]] > c->nlist[c->head[b]]; });
+  }
   int rc0 = c->build_csr();
   if (rc0) return rc0;
-  // Inside each tile, nodes by descending degree (ties in curve order): a
-  // wave's node groups then share one trip count, which cuts the lanes idle
-  // in the bond loops (plate: 84% -> 94% busy). The tiles keep their node
-  // sets, so their halos are unchanged.
-  if (c->build_tiles() == 0 && c->ntiles > 0) {
-    const std::vector<int> bounds = c->tile_start;
-    std::vector<int> order(N);
-    std::iota(order.begin(), order.end(), 0);
-    for (int t = 0; t < c->ntiles; t++) {
-      auto deg = [&](int k) { return c->row[k + 1] - c->row[k]; };
-      std::stable_sort(order.begin() + bounds[t], order.begin() + bounds[t + 1],
-                       [&](int a, int b) { return deg(a) > deg(b); });
-    }
-    std::vector<int> p2(N);
-    for (int k = 0; k < N; k++) p2[k] = c->perm[order[k]];
-    c->perm.swap(p2);
-    if ((rc0 = c->build_csr())) return rc0;
-    if (c->build_tiles(&bounds) != 0 || c->ntiles != (int)bounds.size() - 1)
-      return c->fail(PDG_E_UNSUPPORTED, "tiles changed under the in-tile node order");
-  }
+  // an untileable mesh (bounds = {0}) leaves the context untiled
+  if (c->build_tiles(bounds) != 0 && bounds.size() > 1)
+    return c->fail(PDG_E_UNSUPPORTED, "a tile halo outgrew its capacity under the in-tile node order");
 
   // Scales: L and S powers of two near the horizon and the mean volume.
   double vsum = 0.0;

@@ -78,7 +78,8 @@ struct Ctx {
   bool pipe() const { return variant != 2; }
   TileArgs tile_args() const;
   int build_csr();
-  int build_tiles(const std::vector<int>* bounds = nullptr);
+  int tile_bounds(std::vector<int>& bounds) const;
+  int build_tiles(const std::vector<int>& bounds);
 
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
