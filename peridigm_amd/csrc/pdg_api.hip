@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pdg_internal.hpp"
@@ -59,6 +61,23 @@ uint64_t hilbert3(uint64_t a, uint64_t b, uint64_t c) {
     if (X[2] & Q) t ^= Q - 1;
   for (int i = 0; i < 3; i++) X[i] ^= t;
   return spread3(X[2]) | (spread3(X[1]) << 1) | (spread3(X[0]) << 2);
+}
+
+// Runs f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads
+// (one chunk in the calling thread when n < min_items).
+template <class F>
+void parallel_for(long long n, F&& f, long long min_items = 65536) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nt = std::min(16u, std::max(1u, hw));
+  if (n < min_items || nt == 1) { f(0LL, n); return; }
+  const long long chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++) {
+    const long long lo = (long long)t * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([&f, lo, hi] { f(lo, hi); });
+  }
+  for (auto& x : th) x.join();
 }
 
 uint64_t spread3(uint64_t v) {
@@ -138,33 +157,38 @@ int Ctx::build_csr() {
     long long b = 0;
     for (int p = 0; p < N; p++) { bond0[p] = b; b += nlist[head[p]]; }
   }
-  std::vector<std::pair<int, int>> tmp;
-  for (int k = 0; k < N; k++) {
-    const int p = perm[k];
-    const long long h0 = head[p];
-    const int nn = nlist[h0];
-    tmp.resize(nn);
-    for (int q = 0; q < nn; q++) tmp[q] = {iperm[nlist[h0 + 1 + q]], (int)(bond0[p] + q)};
-    std::sort(tmp.begin(), tmp.end());
-    for (int q = 0; q < nn; q++) {
-      if (q > 0 && tmp[q].first == tmp[q - 1].first)
-        return fail(PDG_E_UNSUPPORTED, "duplicate neighbour in neighbourhood list");
-      col[row[k] + q] = tmp[q].first;
-      src[row[k] + q] = tmp[q].second;
+  std::atomic<int> dup{0}, asym{0};
+  parallel_for(N, [&](long long k0, long long k1) {
+    std::vector<std::pair<int, int>> tmp;
+    for (int k = (int)k0; k < (int)k1; k++) {
+      const int p = perm[k];
+      const long long h0 = head[p];
+      const int nn = nlist[h0];
+      tmp.resize(nn);
+      for (int q = 0; q < nn; q++) tmp[q] = {iperm[nlist[h0 + 1 + q]], (int)(bond0[p] + q)};
+      std::sort(tmp.begin(), tmp.end());
+      for (int q = 0; q < nn; q++) {
+        if (q > 0 && tmp[q].first == tmp[q - 1].first) dup = 1;
+        col[row[k] + q] = tmp[q].first;
+        src[row[k] + q] = tmp[q].second;
+      }
     }
-  }
+  });
+  if (dup) return fail(PDG_E_UNSUPPORTED, "duplicate neighbour in neighbourhood list");
   // Reverse bonds; the gather-only force needs a symmetric list.
   rev.resize(B);
-  for (int k = 0; k < N; k++)
-    for (int b = row[k]; b < row[k + 1]; b++) {
-      const int j = col[b];
-      const int* s = col.data() + row[j];
-      const int* e = col.data() + row[j + 1];
-      const int* it = std::lower_bound(s, e, k);
-      if (it == e || *it != k)
-        return fail(PDG_E_UNSUPPORTED, "neighbourhood list is not symmetric");
-      rev[b] = (int)(it - col.data());
-    }
+  parallel_for(N, [&](long long k0, long long k1) {
+    for (int k = (int)k0; k < (int)k1; k++)
+      for (int b = row[k]; b < row[k + 1]; b++) {
+        const int j = col[b];
+        const int* s = col.data() + row[j];
+        const int* e = col.data() + row[j + 1];
+        const int* it = std::lower_bound(s, e, k);
+        if (it == e || *it != k) { asym = 1; rev[b] = b; continue; }
+        rev[b] = (int)(it - col.data());
+      }
+  });
+  if (asym) return fail(PDG_E_UNSUPPORTED, "neighbourhood list is not symmetric");
   mw.assign(N + 1, 0);
   for (int k = 0; k < N; k++) mw[k + 1] = mw[k] + (row[k + 1] - row[k] + 63) / 64;
   return 0;
@@ -218,30 +242,36 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
   own_off.clear();
   lcol.assign(B, 0);
   if (bounds.size() < 2) return 1;
-  std::vector<int> mark(N, -1);
-  std::vector<int> H;
-  for (size_t t = 0; t + 1 < bounds.size(); t++) {
-    const int n0 = bounds[t], n1 = bounds[t + 1];
-    H.clear();
-    for (int i = n0; i < n1; i++) {
-      if (mark[i] != (int)t) { mark[i] = (int)t; H.push_back(i); }
-      for (int b = row[i]; b < row[i + 1]; b++)
-        if (mark[col[b]] != (int)t) { mark[col[b]] = (int)t; H.push_back(col[b]); }
+  const int nt = (int)bounds.size() - 1;
+  std::vector<std::vector<int>> Hs(nt);
+  std::vector<int> own(nt);
+  std::atomic<int> over{0};
+  parallel_for(nt, [&](long long t0, long long t1) {
+    for (int t = (int)t0; t < (int)t1; t++) {
+      const int n0 = bounds[t], n1 = bounds[t + 1];
+      std::vector<int>& H = Hs[t];
+      H.clear();
+      for (int i = n0; i < n1; i++) {
+        H.push_back(i);
+        H.insert(H.end(), col.begin() + row[i], col.begin() + row[i + 1]);
+      }
+      std::sort(H.begin(), H.end());
+      H.erase(std::unique(H.begin(), H.end()), H.end());
+      if ((int)H.size() > kTileHaloMax) { over = 1; continue; }
+      own[t] = (int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin());
+      for (int k = n0; k < n1; k++)
+        for (int b = row[k]; b < row[k + 1]; b++)
+          lcol[b] = (unsigned short)(8 * (std::lower_bound(H.begin(), H.end(), col[b]) - H.begin()));
     }
-    if ((int)H.size() > kTileHaloMax) {
-      ntiles = 0; tile_start.assign(1, 0); halo_off.assign(1, 0); halo.clear(); own_off.clear();
-      return 1;
-    }
-    std::sort(H.begin(), H.end());
-    own_off.push_back((int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin()));
-    for (int k = n0; k < n1; k++)
-      for (int b = row[k]; b < row[k + 1]; b++)
-        lcol[b] = (unsigned short)(8 * (std::lower_bound(H.begin(), H.end(), col[b]) - H.begin()));
-    halo.insert(halo.end(), H.begin(), H.end());
+  }, 32);
+  if (over) return 1;
+  for (int t = 0; t < nt; t++) {
+    own_off.push_back(own[t]);
+    halo.insert(halo.end(), Hs[t].begin(), Hs[t].end());
     halo_off.push_back((int)halo.size());
-    tile_start.push_back(n1);
-    ntiles++;
+    tile_start.push_back(bounds[t + 1]);
   }
+  ntiles = nt;
   return 0;
 }
 
