@@ -211,6 +211,7 @@ def main():
     ctx.set_bcs(p["bcs"])
     ctx.upload_state(v=p["v0"])
     ntiles, halo_total = ctx.tiling()
+    tile_nodes, lanes_run = ctx.tile_config()
     dt = p["dt"]
 
     # warmup: initial force evaluation + W steps
@@ -240,7 +241,7 @@ def main():
     ctx.close()
 
     copy_gbs = copy_bandwidth()
-    profiled = args.workload == "plate" and args.lanes == 4 and args.variant == 0
+    profiled = args.workload == "plate" and lanes_run == 4 and tile_nodes == 128 and args.variant == 0
     traffic2 = MEASURED_TRAFFIC["pass2"] if profiled else None
     ms_per_step = elapsed / args.steps * 1e3
     value = B * args.steps / elapsed
@@ -268,12 +269,12 @@ def main():
                 "batched": "64 notched plates 5k-60k nodes (seed 2024), sc U[3e-4,1e-3], "
                            "notch U[10h,60h], one node/bond space",
             }[args.workload],
-            "nodes": N, "bonds": B, "lanes_per_node": args.lanes,
+            "nodes": N, "bonds": B, "lanes_per_node": lanes_run,
             "launch": ("persistent cooperative, %d workgroups" % grid) if args.persistent
                       else "two launches per step",
             "kernels": ("untiled" if (args.variant == 1 or ntiles == 0) else
                         "lds-tiled" + (", plain loops" if args.variant == 2 else ", pipelined loops")),
-            "tiles": ntiles, "halo_per_node": halo_total / N,
+            "tiles": ntiles, "tile_nodes": tile_nodes, "halo_per_node": halo_total / N,
             "bond_arithmetic": "fp32 compensated", "integration": "fp64",
             "broken_bonds_end": int(st["bond_damage"].sum()),
         },

@@ -54,8 +54,18 @@ int pdg_ctx_destroy(pdg_ctx* ctx);
 const char* pdg_last_error(const pdg_ctx* ctx);
 
 /* Lanes per node used by the bond kernels (4, 8, 16, 32 or 64; default 4;
- * the untiled kernels use 16 when 4 or 8 is set). */
+ * the untiled kernels use 16 when 4 or 8 is set; the tiled kernels use at
+ * least 512 / tile size, so one workgroup round covers a tile). */
 int pdg_set_lanes_per_node(pdg_ctx* ctx, int lanes);
+
+/* Nodes per LDS tile: 32, 64 or 128, or 0 (default) to choose from the mesh
+ * at pdg_set_constants: 128, halved down to 32 while the mesh gives fewer
+ * than two workgroups per CU. Takes effect at the next pdg_set_constants.
+ * No reference counterpart (a tuning control of this implementation). */
+int pdg_set_tile_nodes(pdg_ctx* ctx, int nodes);
+/* Tile size and lanes per node the bond kernels run with (tile size 0 when
+ * the untiled kernels run). No reference counterpart. */
+int pdg_tile_config(const pdg_ctx* ctx, int* tileNodes, int* lanes);
 
 /* Bond-kernel variant: 0 (default) LDS-staged Hilbert tiles when every tile
  * halo fits in LDS, else the untiled kernels; 1 forces the untiled kernels

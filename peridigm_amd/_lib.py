@@ -45,6 +45,8 @@ SIGNATURES = {
     "pdg_last_error": (C.c_char_p, [_vp]),
     "pdg_set_lanes_per_node": (_i, [_vp, _i]),
     "pdg_set_kernel_variant": (_i, [_vp, _i]),
+    "pdg_set_tile_nodes": (_i, [_vp, _i]),
+    "pdg_tile_config": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "pdg_get_tiling": (_i, [_vp, C.POINTER(_i), C.POINTER(_ll)]),
     "pdg_set_topology": (_i, [_vp, _i, _vp, _vp, _ll]),
     "pdg_set_constants": (_i, [_vp, _vp, _vp, _d, _d, _d, _i, _d]),

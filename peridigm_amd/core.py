@@ -32,7 +32,7 @@ def _p(a):
 class Context:
     """pdg_ctx on HIP device `device`."""
 
-    def __init__(self, device: int = 0, lanes: int = 4, variant: int = 0):
+    def __init__(self, device: int = 0, lanes: int = 4, variant: int = 0, tile_nodes: int = 0):
         self.lib = _lib.load()
         h = C.c_void_p()
         rc = self.lib.pdg_ctx_create(device, C.byref(h))
@@ -44,6 +44,7 @@ class Context:
         self._keep = []
         self.check(self.lib.pdg_set_lanes_per_node(self.h, lanes))
         self.check(self.lib.pdg_set_kernel_variant(self.h, variant))
+        self.check(self.lib.pdg_set_tile_nodes(self.h, tile_nodes))
 
     # ------------------------------------------------------------ plumbing
     def check(self, rc):
@@ -167,6 +168,13 @@ class Context:
         n = C.c_int(); t = C.c_longlong()
         self.check(self.lib.pdg_get_tiling(self.h, C.byref(n), C.byref(t)))
         return n.value, t.value
+
+    def tile_config(self):
+        """(nodes per tile, lanes per node) the bond kernels run with; the
+        tile size is 0 when the untiled kernels run."""
+        t = C.c_int(); g = C.c_int()
+        self.check(self.lib.pdg_tile_config(self.h, C.byref(t), C.byref(g)))
+        return t.value, g.value
 
     def set_per_pass_timing(self, on=True):
         self.check(self.lib.pdg_set_per_pass_timing(self.h, int(bool(on))))

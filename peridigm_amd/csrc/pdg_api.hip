@@ -72,11 +72,19 @@ void parallel_for(long long n, F&& f, long long min_items = 65536) {
   if (n < min_items || nt == 1) { f(0LL, n); return; }
   const long long chunk = (n + nt - 1) / nt;
   std::vector<std::thread> th;
+  th.reserve(nt);
+  long long done = 0;  // chunks below `done` are handed to threads
   for (unsigned t = 0; t < nt; t++) {
     const long long lo = (long long)t * chunk, hi = std::min(n, lo + chunk);
     if (lo >= hi) break;
-    th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    try {
+      th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    } catch (...) {  // no thread available: the caller runs the rest
+      break;
+    }
+    done = hi;
   }
+  if (done < n) f(done, n);
   for (auto& x : th) x.join();
 }
 
@@ -123,7 +131,7 @@ void Ctx::free_device() {
 hipError_t Ctx::pass1(NodeRec* rec, int* nbroken, bool damage, bool count_near) {
   if (tiled())
     return launch_pass1_tiled(bond_args(), tile_args(), rec, d_theta, nbroken, damage, count_near,
-                              lanes, pipe(), stream);
+                              tile_lanes(), pipe(), stream);
   return launch_pass1(bond_args(), rec, d_theta, nbroken, damage, count_near, lanes, stream);
 }
 
@@ -197,7 +205,7 @@ int Ctx::build_csr() {
 // Greedy tile boundaries along the Hilbert order (positions in perm,
 // neighbours by reference id): extend a tile node by node while its halo (own
 // nodes plus all their neighbours) stays within kTileHaloMax and it holds at
-// most kTileNodesMax nodes. Returns 1 (no tiles) when a node has more than 128
+// most tile_nodes nodes. Returns 1 (no tiles) when a node has more than 128
 // neighbours or one node's halo alone does not fit.
 int Ctx::tile_bounds(std::vector<int>& bounds) const {
   bounds.assign(1, 0);
@@ -207,7 +215,7 @@ int Ctx::tile_bounds(std::vector<int>& bounds) const {
   int n0 = 0, tid = 0;
   while (n0 < N) {
     int i = n0, hs = 0;
-    while (i < N && i - n0 < kTileNodesMax) {
+    while (i < N && i - n0 < tile_nodes) {
       const int p = perm[i];
       const long long h0 = head[p];
       const int nn = nlist[h0];
@@ -376,6 +384,23 @@ int pdg_set_lanes_per_node(pdg_ctx* h, int lanes) {
   return PDG_OK;
 }
 
+int pdg_set_tile_nodes(pdg_ctx* h, int nodes) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (nodes != 0 && nodes != 32 && nodes != 64 && nodes != 128)
+    return c->fail(PDG_E_ARG, "tile size must be 0 (chosen from the mesh), 32, 64 or 128 nodes");
+  c->tile_nodes_req = nodes;
+  return PDG_OK;
+}
+
+int pdg_tile_config(const pdg_ctx* h, int* tileNodes, int* lanes) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (tileNodes) *tileNodes = c->tiled() ? c->tile_nodes : 0;
+  if (lanes) *lanes = c->tiled() ? c->tile_lanes() : std::max(c->lanes, 16);
+  return PDG_OK;
+}
+
 int pdg_set_kernel_variant(pdg_ctx* h, int variant) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
@@ -478,6 +503,19 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   // (ties in curve order): a wave's node groups then share one trip count,
   // which cuts the lanes idle in the bond loops (plate: 84% -> 94% busy). The
   // tiles keep their node sets, so their halos are unchanged.
+  // Tile size: 128 nodes, halved (down to 32) while the mesh gives fewer
+  // than two workgroups per CU, so small meshes still fill the GPU.
+  if (c->tile_nodes_req) {
+    c->tile_nodes = c->tile_nodes_req;
+  } else {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        ncu <= 0)
+      ncu = 256;
+    int t = kTileNodesMax;
+    while (t > 32 && (N + t - 1) / t < 2 * ncu) t /= 2;
+    c->tile_nodes = t;
+  }
   std::vector<int> bounds;
   if (c->tile_bounds(bounds) == 0) {
     for (size_t t = 0; t + 1 < bounds.size(); t++)
@@ -701,7 +739,7 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
                        c->d_rec[c->cur], c->L);
   if (!rc) rc = c->upload_mask(bdNP1, true);
   if (!rc) rc = c->hip(c->pass1(c->d_rec[c->cur], nullptr, false, false), "pass1");
-  if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->lanes, c->pipe(), s)
+  if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->tile_lanes(), c->pipe(), s)
                                   : launch_pass2_force(c->bond_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s),
                        "pass2");
   if (!rc) rc = c->download_vec3(c->d_fout, force);
@@ -814,7 +852,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
     va.t_cur = step_time(k + 1);
     va.t_prev = step_time(k);
     va.inv_dtbc = 1.0 / (va.t_cur - va.t_prev);
-    if (!r) r = c->hip(c->tiled() ? launch_pass2_tiled_verlet(ba, c->tile_args(), c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, c->pipe(), s)
+    if (!r) r = c->hip(c->tiled() ? launch_pass2_tiled_verlet(ba, c->tile_args(), c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->tile_lanes(), c->pipe(), s)
                                   : launch_pass2_verlet(ba, c->d_rec[c->cur], c->d_rec[c->cur ^ 1], va, c->lanes, s),
                        "pass2");
     if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 2], s), "event");
@@ -855,7 +893,8 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
   if (!c->tiled()) return c->fail(PDG_E_UNSUPPORTED, "persistent run needs the tiled kernels");
-  if (c->lanes != 4 && c->lanes != 8 && c->lanes != 16)
+  const int tl = c->tile_lanes();
+  if (tl != 4 && tl != 8 && tl != 16)
     return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 4, 8 or 16 lanes per node");
   if (!initialEval && nsteps == 0) return PDG_OK;
   (void)hipSetDevice(c->device);
@@ -880,7 +919,7 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   rc = c->hip(hipEventRecord(c->ev[0], s), "event");
   if (!rc) rc = c->hip(launch_persistent(c->bond_args(), c->tile_args(), r0, r1, c->d_theta,
                                          c->d_nbroken, va, nsteps, t0, initialEval, c->sc > 0.0,
-                                         c->lanes, gridSize, s), "persistent launch");
+                                         tl, gridSize, s), "persistent launch");
   if (!rc) rc = c->hip(hipEventRecord(c->ev[3], s), "event");
   if (!rc) rc = c->hip(hipEventSynchronize(c->ev[3]), "sync");
   if (rc) return rc;

@@ -1,6 +1,7 @@
 // The context behind the opaque pdg_ctx handle.
 #pragma once
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,8 @@ struct Ctx {
   // LDS-staged Hilbert tiles (empty if a node's neighbourhood exceeds the
   // halo capacity; the untiled kernels are used then)
   int variant = 0;  // 0: tiled (pipelined loops), 1: untiled, 2: tiled (plain loops)
+  int tile_nodes_req = 0;           // requested tile size (0: chosen from the mesh size)
+  int tile_nodes = kTileNodesMax;   // tile size of the current tiling
   int ntiles = 0;
   std::vector<int> tile_start, halo_off, halo, own_off;
   std::vector<unsigned short> lcol;
@@ -76,6 +79,9 @@ struct Ctx {
   unsigned short* d_lcol = nullptr;
   bool tiled() const { return variant != 1 && ntiles > 0; }
   bool pipe() const { return variant != 2; }
+  // lanes per node of the tiled passes: at least enough that one round of a
+  // workgroup covers a whole tile
+  int tile_lanes() const { return std::max(lanes, kTileThreads / tile_nodes); }
   TileArgs tile_args() const;
   int build_csr();
   int tile_bounds(std::vector<int>& bounds) const;

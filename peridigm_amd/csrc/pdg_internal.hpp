@@ -83,6 +83,7 @@ struct VerletArgs {
 constexpr int kTileHaloMax = 800;
 static_assert(8 * kTileHaloMax <= 65536, "halo byte offsets must fit 16 bits");
 constexpr int kTileNodesMax = 128;
+constexpr int kTileThreads = 512;  // threads per tiled-pass workgroup
 constexpr int kMaskPad = 2;    // >= mask words per node in the tiled kernels (degree <= 128)
 constexpr int kLcolPad = 128;  // >= the tiled kernels' maximum degree
 

@@ -38,7 +38,7 @@ namespace {
 using namespace dev;
 
 constexpr int HMAX = kTileHaloMax;
-constexpr int WG = 512;
+constexpr int WG = kTileThreads;
 // Minimum waves per SIMD the register allocation must allow (the second
 // __launch_bounds__ argument): 4, i.e. two 512-thread workgroups per CU, so at
 // most 128 VGPRs. The 52.7 KB of LDS per workgroup (halo capacity 800) would

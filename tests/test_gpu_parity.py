@@ -20,9 +20,12 @@ pytestmark = pytest.mark.gpu
 FORCE_TOL = 1e-5
 
 
-def _ctx(nl, x, vol, K=130e9, mu=78e9, horizon=1.0, sc=0.0, lanes=16, infl=0, variant=0):
+def _ctx(nl, x, vol, K=130e9, mu=78e9, horizon=1.0, sc=0.0, lanes=16, infl=0, variant=0,
+         tile_nodes=128):
+    """Context with the given kernel shape. The tile size defaults to 128 so
+    that the requested lanes per node run on these small meshes too."""
     from peridigm_amd import Context
-    c = Context(0, lanes=lanes, variant=variant)
+    c = Context(0, lanes=lanes, variant=variant, tile_nodes=tile_nodes)
     c.set_topology(nl, len(vol))
     c.set_constants(x, vol, K, mu, horizon, infl, sc)
     return c
@@ -107,6 +110,26 @@ def test_compute_force_matches_oracle(mu, lanes, variant):
         m = ctx.initialize()
         th, f = ctx.compute_force(y)
     tho, fo = O.compute_force(x, y, m, vol, np.zeros(O.num_bonds(nl, len(vol))), nl, 130e9, mu, hz)
+    assert _rel(th, tho) <= FORCE_TOL
+    assert _rel(f, fo) <= FORCE_TOL
+
+
+@pytest.mark.parametrize("tile_nodes, lanes, want", [(0, 4, (32, 16)), (32, 4, (32, 16)),
+                                                    (64, 4, (64, 8)), (128, 4, (128, 4)),
+                                                    (64, 16, (64, 16))])
+def test_tile_sizes_match_oracle(tile_nodes, lanes, want):
+    """Tiles of 32, 64 and 128 nodes (and the automatic choice, which picks 32
+    for a mesh this small) against the oracle; the tiled passes run at least
+    512 / tile size lanes per node."""
+    x, vol, nl, hz = _bar()
+    rng = np.random.default_rng(5)
+    u = 1e-6 * rng.standard_normal(x.shape) + 1e-4 * x
+    y = x + u
+    with _ctx(nl, x, vol, 130e9, 60e9, hz, lanes=lanes, tile_nodes=tile_nodes) as ctx:
+        assert ctx.tile_config() == want
+        m = ctx.initialize()
+        th, f = ctx.compute_force(y)
+    tho, fo = O.compute_force(x, y, m, vol, np.zeros(O.num_bonds(nl, len(vol))), nl, 130e9, 60e9, hz)
     assert _rel(th, tho) <= FORCE_TOL
     assert _rel(f, fo) <= FORCE_TOL
 
@@ -294,7 +317,7 @@ def test_deterministic_and_lane_independent():
     nl = O.filter_planes(p["x"], O.radius_search(p["x"], p["horizon"]), p["planes"])
     outs = []
     for lanes, variant in ((16, 0), (16, 0), (16, 2), (4, 0), (8, 0), (32, 0), (64, 0), (16, 1), (64, 1)):
-        with Context(0, lanes=lanes, variant=variant) as ctx:
+        with Context(0, lanes=lanes, variant=variant, tile_nodes=128) as ctx:
             ctx.set_topology(nl, len(p["volume"]))
             ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
             ctx.set_density(np.full(len(p["volume"]), p["density"]))
