@@ -18,8 +18,8 @@ namespace pdg {
 //   vol: V / S, alpha: 15 mu / m * L * S, p: theta * (3K/m - alpha/3) * L * S.
 // L and S are powers of two, so the scalings are exact.
 struct __align__(64) NodeRec {
+  double u[3];  // first: the per-step displacement write fills one 32-byte sector
   double x[3];
-  double u[3];
   float vol;
   float alpha;
   float p;
