@@ -120,27 +120,42 @@ __device__ __forceinline__ BondGeom bond_geom(double xi0, double xi1, double xi2
   return b;
 }
 
+// The global inputs of one Verlet component, loadable ahead of the force.
+struct VerletPre {
+  double rho, v;
+  int slot;  // boundary-condition slot (-1: none, or no new step starts)
+};
+
+__device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, int l) {
+  VerletPre p;
+  p.rho = va.rho[node];
+  p.v = va.v[3 * node + l];
+  p.slot = va.start_step ? va.bc_slot[node] : -1;
+  return p;
+}
+
 // Velocity-Verlet epilogue for one component l of node `node` (FP64,
 // src/core/Peridigm.cpp:1480-1501, :1569-1577): finishes the current step
 // (v += dt/2 a) and, unless this is the last force evaluation, starts the next
 // one (v += dt/2 a, prescribed-displacement velocity, u += dt v). u_scaled is
 // the node's displacement / L from the current record; the new one goes to
-// rec_next.
+// rec_next. `pre` holds the component's inputs from verlet_pre.
 __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, int l, float fc,
-                                                double u_scaled, NodeRec* __restrict__ rec_next) {
+                                                double u_scaled, NodeRec* __restrict__ rec_next,
+                                                const VerletPre& pre) {
   const int q = 3 * node + l;
   const double f = (double)fc;
   if (!isfinite(f)) *va.err = 1;
   double acc = f;
   acc += 0.0;  // external force (none)
-  acc /= va.rho[node];
-  double v = va.v[q];
+  acc /= pre.rho;
+  double v = pre.v;
   double u = u_scaled * va.L;
   const double dt2 = va.dt / 2.0;
   if (va.end_kick) v = v + dt2 * acc;
   if (va.start_step) {
     v = v + dt2 * acc;
-    const int slot = va.bc_slot[node];
+    const int slot = pre.slot;
     if (slot >= 0 && ((va.bcs[slot].mask >> l) & 1)) {
       const double c0 = va.bcs[slot].c0[l], c1 = va.bcs[slot].c1[l];
       const double cur = c1 * va.t_cur + c0;
@@ -155,6 +170,11 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
   }
   va.v[q] = v;
   rec_next[node].u[l] = u * va.inv_L;
+}
+
+__device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, int l, float fc,
+                                                double u_scaled, NodeRec* __restrict__ rec_next) {
+  verlet_epilogue(va, node, l, fc, u_scaled, rec_next, verlet_pre(va, node, l));
 }
 
 }  // namespace dev

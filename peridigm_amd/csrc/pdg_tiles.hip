@@ -45,6 +45,7 @@ constexpr int WG = kTileThreads;
 // admit three, but that needs <= 80 VGPRs, which spills and measured slower.
 constexpr int WAVES_PER_SIMD = 4;
 constexpr int MAXDEG = 128;
+static_assert(3 * kTileNodesMax <= WG, "one Verlet component per thread");
 constexpr int NWORDS = MAXDEG / 64;
 
 __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
@@ -152,6 +153,32 @@ __device__ __forceinline__ bool mask_bit(const unsigned long long (&m)[NWORDS], 
   return ((half >> ((base & 31) + l)) & 1u) != 0u;
 }
 
+// A node group's row data: CSR range, mask words, local indices, constants.
+template <int G>
+struct RowData {
+  int rs, deg, w0;
+  unsigned long long mo[NWORDS];
+  int lj[MAXDEG / G];
+  NodeConst c;
+};
+
+template <int G>
+__device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, int node,
+                                         bool valid, int l, RowData<G>& r) {
+  r.rs = 0;
+  r.deg = 0;
+  r.w0 = 0;
+  r.c = NodeConst{0.0f, 0.0f, 0.0f, 1.0f};
+  if (valid) {
+    r.rs = a.row[node];
+    r.deg = a.row[node + 1] - r.rs;
+    r.w0 = a.mw[node];
+    r.c = a.nc[node];
+  }
+  load_words(a.mask, r.w0, r.deg, r.mo);
+  load_lcol<G>(t.lcol, r.rs, l, r.lj);
+}
+
 // Lane masks of the wave (compare results kept in scalar registers):
 // predicate codes of the compare builtins.
 enum { CMP_FOGT = 2, CMP_IEQ = 32, CMP_ISLT = 40 };
@@ -182,6 +209,14 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   double (&su)[3][HMAX] = L.su;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
+  constexpr int NG = WG / G;
+  const int grp = threadIdx.x / G;
+  const int g = (threadIdx.x & 63) / G;
+  const int l = threadIdx.x & (G - 1);
+  // the first round's row data is loaded before the halo staging, so the two
+  // sets of global loads are in flight together
+  RowData<G> rd;
+  row_load<G>(a, t, n0 + grp, n0 + grp < n1, l, rd);
   // slots past the halo are zero-filled: masked-out bonds may address them
   for (int h = threadIdx.x; h < HMAX; h += WG) {
     if (h < hn) {
@@ -197,31 +232,23 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   }
   __syncthreads();
   const int own = t.own_off[tile];
-  constexpr int NG = WG / G;
-  const int grp = threadIdx.x / G;
-  const int g = (threadIdx.x & 63) / G;
-  const int l = threadIdx.x & (G - 1);
   for (int base = n0; base < n1; base += NG) {
     const int node = base + grp;
     const bool valid = node < n1;
-    int rs = 0, deg = 0, w0 = 0;
+    if (base != n0) row_load<G>(a, t, node, valid, l, rd);
+    const int deg = rd.deg, w0 = rd.w0;
+    const unsigned long long (&mo)[NWORDS] = rd.mo;
+    const int (&lj)[MAXDEG / G] = rd.lj;
+    const NodeConst c = rd.c;
     double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
-    NodeConst c{0.0f, 0.0f, 0.0f, 1.0f};
     if (valid) {
-      rs = a.row[node];
-      deg = a.row[node + 1] - rs;
-      w0 = a.mw[node];
       const int li = own + (node - n0);
       xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
       ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
-      c = a.nc[node];
     }
-    unsigned long long mo[NWORDS], mn[NWORDS];
+    unsigned long long mn[NWORDS];
 #pragma unroll
     for (int w = 0; w < NWORDS; w++) mn[w] = 0ull;
-    load_words(a.mask, w0, deg, mo);
-    int lj[MAXDEG / G];
-    load_lcol<G>(t.lcol, rs, l, lj);
     const float scz = c.sc;
     const int nmax = groups_max<G>((deg + G - 1) / G);
     const int nact = deg - l;  // bond it * G + l exists iff it * G < nact
@@ -345,6 +372,18 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   double (&su)[3][HMAX] = L.su;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
+  constexpr int NG = WG / G;
+  const int grp = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  // the first round's row data and this thread's Verlet inputs (component
+  // q = threadIdx.x; a tile has at most 3 * 128 <= WG components) are loaded
+  // before the halo staging, so all these global loads are in flight together
+  RowData<G> rd;
+  row_load<G>(a, t, n0 + grp, n0 + grp < n1, l, rd);
+  const int q = threadIdx.x;
+  const bool has_q = q < 3 * (n1 - n0);
+  VerletPre vp{0.0, 0.0, -1};
+  if (VERLET && has_q) vp = verlet_pre(va, n0 + q / 3, q - 3 * (q / 3));
   // slots past the halo are zero-filled: masked-out bonds may address them
   for (int h = threadIdx.x; h < HMAX; h += WG) {
     if (h < hn) {
@@ -362,30 +401,23 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   }
   __syncthreads();
   const int own = t.own_off[tile];
-  constexpr int NG = WG / G;
-  const int grp = threadIdx.x / G;
-  const int l = threadIdx.x & (G - 1);
   for (int base = n0; base < n1; base += NG) {
     const int node = base + grp;
     const bool valid = node < n1;
-    int rs = 0, deg = 0, w0 = 0, li = 0;
+    if (base != n0) row_load<G>(a, t, node, valid, l, rd);
+    const int deg = rd.deg;
+    const unsigned long long (&mo)[NWORDS] = rd.mo;
+    const int (&lj)[MAXDEG / G] = rd.lj;
+    const float hhat = rd.c.hhat;
     double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
-    float alpha_i = 0.0f, p_i = 0.0f, hhat = 1.0f;
+    float alpha_i = 0.0f, p_i = 0.0f;
     if (valid) {
-      rs = a.row[node];
-      deg = a.row[node + 1] - rs;
-      w0 = a.mw[node];
-      li = own + (node - n0);
+      const int li = own + (node - n0);
       xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
       ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
       alpha_i = L.sva[li].y;
       p_i = L.spx[li].x;
-      hhat = a.nc[node].hhat;
     }
-    unsigned long long mo[NWORDS];
-    load_words(a.mask, w0, deg, mo);
-    int lj[MAXDEG / G];
-    load_lcol<G>(t.lcol, rs, l, lj);
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
       const int nmax = groups_max<G>((deg + G - 1) / G);
@@ -441,9 +473,9 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     // Verlet epilogue of the whole tile at once: one component per thread
     // instead of 3 of every 16 lanes per node round
     __syncthreads();
-    for (int q = threadIdx.x; q < 3 * (n1 - n0); q += WG) {
+    if (has_q) {
       const int k = q / 3, l3 = q - 3 * k;
-      verlet_epilogue(va, n0 + k, l3, L.sf[q], su[l3][own + k], rec_next);
+      verlet_epilogue(va, n0 + k, l3, L.sf[q], su[l3][own + k], rec_next, vp);
     }
   }
   __syncthreads();  // LDS is reused by the next tile
