@@ -1,7 +1,8 @@
 // LDS-staged bond passes over Hilbert tiles.
 //
 // One workgroup (8 waves) owns a tile: a contiguous range of at most 128
-// Hilbert-ordered nodes. It first stages the FP64 reference positions, FP64
+// Hilbert-ordered nodes (32 or 64 on small meshes), ordered by degree inside
+// the tile. It first stages the FP64 reference positions, FP64
 // displacements and per-node scalars of the tile's halo (the union of the
 // tile's nodes and all their neighbours, at most 800 nodes) in LDS with one
 // 64-byte record load per halo node, then runs group-of-G-lanes bond loops
@@ -12,8 +13,8 @@
 // A node's damage-mask words (at most 2, degree <= 128) are held in registers
 // for the whole bond loop, the lane's 16-bit neighbour indices are prefetched
 // before it, and the loop is unrolled so every register index is static;
-// pass 1 rebuilds the words from per-iteration ballots and writes back only
-// the words that changed. The influence function is a template parameter.
+// pass 1 merges the per-iteration ballots of new breaks into the words and
+// writes back only the words that changed. The influence function is a template parameter.
 //
 // Workgroup -> tile mapping is XCD-aware: consecutive workgroup ids are
 // dispatched to the 8 XCDs in turn, so each XCD gets a contiguous run of
