@@ -470,7 +470,8 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
   if (!c->have_topo) return c->fail(PDG_E_ARG, "pdg_set_topology must be called first");
-  if (!x || !vol || !(horizon > 0.0) || infl < 0 || infl > 2)
+  // an empty block (no owned points) may pass null arrays
+  if ((c->N > 0 && (!x || !vol)) || !(horizon > 0.0) || infl < 0 || infl > 2)
     return c->fail(PDG_E_ARG, "invalid constants");
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
@@ -656,7 +657,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
 
 int pdg_set_critical_stretch(pdg_ctx* h, const double* sc) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !sc) return PDG_E_ARG;
+  if (!c || !c->have_consts || (c->N > 0 && !sc)) return PDG_E_ARG;
   if (!(c->sc > 0.0)) return c->fail(PDG_E_ARG, "damage model disabled (critical stretch <= 0 at set-up)");
   std::vector<NodeConst> nc(c->N);
   int rc = c->hip(hipMemcpyAsync(nc.data(), c->d_nc, c->N * sizeof(NodeConst), hipMemcpyDeviceToHost, c->stream), "d2h nc");
@@ -674,7 +675,7 @@ int pdg_set_critical_stretch(pdg_ctx* h, const double* sc) {
 
 int pdg_set_density(pdg_ctx* h, const double* rho) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !rho) return PDG_E_ARG;
+  if (!c || !c->have_consts || (c->N > 0 && !rho)) return PDG_E_ARG;
   std::vector<double> r(c->N);
   for (int k = 0; k < c->N; k++) {
     r[k] = rho[c->perm[k]];
@@ -687,7 +688,7 @@ int pdg_set_density(pdg_ctx* h, const double* rho) {
 
 int pdg_initialize(pdg_ctx* h, double* m) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !m) return PDG_E_ARG;
+  if (!c || !c->have_consts || (c->N > 0 && !m)) return PDG_E_ARG;
   std::copy(c->m.begin(), c->m.end(), m);
   return PDG_OK;
 }
@@ -695,7 +696,7 @@ int pdg_initialize(pdg_ctx* h, double* m) {
 int pdg_compute_damage(pdg_ctx* h, const double* y, const double* bdN, double* bdNP1,
                        double* damage, long long* nearCritical) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !y || !bdNP1 || !damage) return PDG_E_ARG;
+  if (!c || !c->have_consts || (c->N > 0 && (!y || !bdNP1 || !damage))) return PDG_E_ARG;
   if (!(c->sc > 0.0)) return c->fail(PDG_E_ARG, "no damage model (critical stretch <= 0)");
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
@@ -729,7 +730,7 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
                       double* dilatation, double* force) {
   (void)dt;
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !y || !force) return PDG_E_ARG;
+  if (!c || !c->have_consts || (c->N > 0 && (!y || !force))) return PDG_E_ARG;
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
   const int N = c->N;

@@ -114,6 +114,28 @@ def test_compute_force_matches_oracle(mu, lanes, variant):
     assert _rel(f, fo) <= FORCE_TOL
 
 
+@pytest.mark.parametrize("n", [0, 1, 2])
+def test_empty_and_tiny_blocks(n):
+    """Blocks with no owned points, one isolated point, or two points out of
+    each other's reach (an MPI rank can own such a block): every drop-in call
+    and a short Verlet run succeed and give zero dilatation and force."""
+    x = np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0]])[:n]
+    vol = np.full(n, 1e-9)
+    nl = np.zeros(n, np.int32)  # every neighbourhood empty
+    with _ctx(nl, x, vol, 130e9, 60e9, 3e-3, sc=5e-4, tile_nodes=0) as ctx:
+        m = ctx.initialize()
+        assert m.shape == (n,) and not np.any(m)
+        th, f = ctx.compute_force(x + 1e-6)
+        assert th.shape == (n,) and f.shape == (n, 3)
+        assert not np.any(th) and not np.any(f)
+        ctx.set_density(np.full(n, 7800.0))
+        ctx.upload_state(v=np.full((n, 3), 1.0))
+        ctx.verlet_run(3, 1e-8)
+        st = ctx.download_state()
+        assert np.allclose(st["u"], 3e-8, rtol=1e-12, atol=0.0)
+        assert not np.any(st["bond_damage"])
+
+
 @pytest.mark.parametrize("tile_nodes, lanes, want", [(0, 4, (32, 16)), (32, 4, (32, 16)),
                                                     (64, 4, (64, 8)), (128, 4, (128, 4)),
                                                     (64, 16, (64, 16))])
