@@ -286,19 +286,27 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
 // Host list (reference order) -> device mask words (internal order).
 int Ctx::upload_mask(const double* bd, bool check_symmetric) {
   std::vector<unsigned long long> words(mw.back(), 0ull);
-  for (int i = 0; i < N; i++) {
-    const int rs = row[i], deg = row[i + 1] - rs;
-    for (int k = 0; k < deg; k++) {
-      const double d = bd ? bd[src[rs + k]] : 0.0;
-      if (d != 0.0 && d != 1.0)
-        return fail(PDG_E_UNSUPPORTED, "bond damage values must be 0 or 1");
-      if (d == 1.0) words[mw[i] + (k >> 6)] |= 1ull << (k & 63);
-      if (check_symmetric && bd && bd[src[rev[rs + k]]] != d)
-        return fail(PDG_E_UNSUPPORTED, "bond damage must be symmetric (d_ij == d_ji)");
-    }
+  std::atomic<int> bad_value{0}, asym{0};
+  if (bd) {
+    // each node writes only its own words
+    parallel_for(N, [&](long long i0, long long i1) {
+      for (int i = (int)i0; i < (int)i1; i++) {
+        const int rs = row[i], deg = row[i + 1] - rs;
+        for (int k = 0; k < deg; k++) {
+          const double d = bd[src[rs + k]];
+          if (d != 0.0 && d != 1.0) bad_value = 1;
+          if (d == 1.0) words[mw[i] + (k >> 6)] |= 1ull << (k & 63);
+          if (check_symmetric && bd[src[rev[rs + k]]] != d) asym = 1;
+        }
+      }
+    });
   }
-  return hip(hipMemcpyAsync(d_mask, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream),
-             "upload mask");
+  if (bad_value) return fail(PDG_E_UNSUPPORTED, "bond damage values must be 0 or 1");
+  if (asym) return fail(PDG_E_UNSUPPORTED, "bond damage must be symmetric (d_ij == d_ji)");
+  int rc = hip(hipMemcpyAsync(d_mask, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream),
+               "upload mask");
+  if (!rc) rc = hip(hipStreamSynchronize(stream), "sync");
+  return rc;
 }
 
 int Ctx::download_mask(double* bd) {
@@ -308,11 +316,13 @@ int Ctx::download_mask(double* bd) {
   if (rc) return rc;
   rc = hip(hipStreamSynchronize(stream), "sync");
   if (rc) return rc;
-  for (int i = 0; i < N; i++) {
-    const int rs = row[i], deg = row[i + 1] - rs;
-    for (int k = 0; k < deg; k++)
-      bd[src[rs + k]] = ((words[mw[i] + (k >> 6)] >> (k & 63)) & 1ull) ? 1.0 : 0.0;
-  }
+  parallel_for(N, [&](long long i0, long long i1) {
+    for (int i = (int)i0; i < (int)i1; i++) {
+      const int rs = row[i], deg = row[i + 1] - rs;
+      for (int k = 0; k < deg; k++)
+        bd[src[rs + k]] = ((words[mw[i] + (k >> 6)] >> (k & 63)) & 1ull) ? 1.0 : 0.0;
+    }
+  });
   return PDG_OK;
 }
 
