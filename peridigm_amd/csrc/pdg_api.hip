@@ -285,6 +285,8 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
 
 // Host list (reference order) -> device mask words (internal order).
 int Ctx::upload_mask(const double* bd, bool check_symmetric) {
+  if (!bd)  // no field: every bond intact
+    return hip(hipMemsetAsync(d_mask, 0, (size_t)mw.back() * 8, stream), "zero mask");
   std::vector<unsigned long long> words(mw.back(), 0ull);
   std::atomic<int> bad_value{0}, asym{0};
   if (bd) {
@@ -333,15 +335,19 @@ int Ctx::download_vec3(const double* dsrc, double* out) {
   if (rc) return rc;
   rc = hip(hipStreamSynchronize(stream), "sync");
   if (rc) return rc;
-  for (int k = 0; k < N; k++)
-    for (int c = 0; c < 3; c++) out[3 * (size_t)perm[k] + c] = tmp[3 * (size_t)k + c];
+  parallel_for(N, [&](long long k0, long long k1) {
+    for (int k = (int)k0; k < (int)k1; k++)
+      for (int c = 0; c < 3; c++) out[3 * (size_t)perm[k] + c] = tmp[3 * (size_t)k + c];
+  });
   return PDG_OK;
 }
 
 int Ctx::upload_vec3(const double* in, double* ddst) {
   std::vector<double> tmp(3 * (size_t)N);
-  for (int k = 0; k < N; k++)
-    for (int c = 0; c < 3; c++) tmp[3 * (size_t)k + c] = in ? in[3 * (size_t)perm[k] + c] : 0.0;
+  parallel_for(N, [&](long long k0, long long k1) {
+    for (int k = (int)k0; k < (int)k1; k++)
+      for (int c = 0; c < 3; c++) tmp[3 * (size_t)k + c] = in ? in[3 * (size_t)perm[k] + c] : 0.0;
+  });
   int rc = hip(hipMemcpyAsync(ddst, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, stream), "h2d");
   if (rc) return rc;
   return hip(hipStreamSynchronize(stream), "sync");
