@@ -236,6 +236,26 @@ def test_asymmetric_damage_rejected():
             ctx.compute_force(x, bd)
 
 
+def test_damage_field_checks_on_a_large_block():
+    """The bond-damage field of a block above the host threading threshold
+    (72,000 points): a value other than 0/1 and an asymmetric pair are
+    rejected, and an all-intact field gives bitwise the force of no field."""
+    from peridigm_amd import PdgError
+    x, vol, nl, hz = _bar(60, 60, 20)
+    nb = O.num_bonds(nl, len(vol))
+    y = x * (1.0 + 1e-4)
+    with _ctx(nl, x, vol, horizon=hz, lanes=4) as ctx:
+        th0, f0 = ctx.compute_force(y)
+        th1, f1 = ctx.compute_force(y, np.zeros(nb))
+        assert np.array_equal(f0, f1) and np.array_equal(th0, th1)
+        bad = np.zeros(nb); bad[nb // 2] = 0.5
+        with pytest.raises(PdgError):
+            ctx.compute_force(y, bad)
+        asym = np.zeros(nb); asym[nb - 1] = 1.0
+        with pytest.raises(PdgError):
+            ctx.compute_force(y, asym)
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 def test_compute_damage_bit_exact_outside_band(variant):
     x, vol, nl, hz = _bar(24, 10, 6)
