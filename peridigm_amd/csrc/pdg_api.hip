@@ -283,6 +283,22 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
   return 0;
 }
 
+// Pinned host buffer of at least `bytes` (hipHostMalloc), kept for later
+// calls; nullptr if it cannot be allocated (callers then copy from the
+// caller's memory directly).
+char* Ctx::stage(size_t bytes) {
+  if (bytes <= h_stage_bytes) return h_stage;
+  if (h_stage) (void)hipHostFree(h_stage);
+  h_stage = nullptr;
+  h_stage_bytes = 0;
+  if (hipHostMalloc((void**)&h_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+    h_stage = nullptr;
+    return nullptr;
+  }
+  h_stage_bytes = bytes;
+  return h_stage;
+}
+
 // Host list (reference order) -> device mask words (internal order).
 int Ctx::upload_mask(const double* bd, bool check_symmetric) {
   if (!bd)  // no field: every bond intact
@@ -380,6 +396,7 @@ int pdg_ctx_destroy(pdg_ctx* h) {
   (void)hipStreamSynchronize(c->stream);
   c->free_device();
   c->free_extra();
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   for (int i = 0; i < 4; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -750,7 +767,19 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
   const int N = c->N;
-  int rc = c->hip(hipMemcpyAsync(c->d_y, y, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d y");
+  // y in and f, theta out go through one pinned buffer (PCIe at full rate);
+  // the stream orders the copies, and the call ends synchronised
+  const size_t vbytes = 24 * (size_t)N, tbytes = 4 * (size_t)N;
+  char* st = N > 0 ? c->stage(vbytes + tbytes) : nullptr;
+  int rc = 0;
+  if (st) {
+    parallel_for(3 * (long long)N, [&](long long q0, long long q1) {
+      std::memcpy(st + 8 * q0, y + q0, 8 * (size_t)(q1 - q0));
+    });
+    rc = c->hip(hipMemcpyAsync(c->d_y, st, vbytes, hipMemcpyHostToDevice, s), "h2d y");
+  } else {
+    rc = c->hip(hipMemcpyAsync(c->d_y, y, vbytes, hipMemcpyHostToDevice, s), "h2d y");
+  }
   if (!rc && N > 0)
     hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y, c->d_perm,
                        c->d_rec[c->cur], c->L);
@@ -759,15 +788,34 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
   if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->tile_lanes(), c->pipe(), s)
                                   : launch_pass2_force(c->bond_args(), c->d_rec[c->cur], c->d_fout, c->lanes, s),
                        "pass2");
-  if (!rc) rc = c->download_vec3(c->d_fout, force);
   if (rc) return rc;
-  if (dilatation) {
-    std::vector<float> th(N);
-    rc = c->hip(hipMemcpyAsync(th.data(), c->d_theta, 4 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h theta");
-    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (!st) {
+    rc = c->download_vec3(c->d_fout, force);
     if (rc) return rc;
-    for (int k = 0; k < N; k++) dilatation[c->perm[k]] = (double)th[k];
+    if (dilatation) {
+      std::vector<float> th(N);
+      rc = c->hip(hipMemcpyAsync(th.data(), c->d_theta, tbytes, hipMemcpyDeviceToHost, s), "d2h theta");
+      if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+      if (rc) return rc;
+      for (int k = 0; k < N; k++) dilatation[c->perm[k]] = (double)th[k];
+    }
+    return PDG_OK;
   }
+  rc = c->hip(hipMemcpyAsync(st, c->d_fout, vbytes, hipMemcpyDeviceToHost, s), "d2h force");
+  if (!rc && dilatation)
+    rc = c->hip(hipMemcpyAsync(st + vbytes, c->d_theta, tbytes, hipMemcpyDeviceToHost, s), "d2h theta");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (rc) return rc;
+  const double* fs = reinterpret_cast<const double*>(st);
+  const float* ts = reinterpret_cast<const float*>(st + vbytes);
+  const int* perm = c->perm.data();
+  parallel_for(N, [&](long long k0, long long k1) {
+    for (int k = (int)k0; k < (int)k1; k++) {
+      const size_t p = (size_t)perm[k];
+      for (int d = 0; d < 3; d++) force[3 * p + d] = fs[3 * (size_t)k + d];
+      if (dilatation) dilatation[p] = (double)ts[k];
+    }
+  });
   return PDG_OK;
 }
 

@@ -83,6 +83,10 @@ struct Ctx {
   // workgroup covers a whole tile
   int tile_lanes() const { return std::max(lanes, kTileThreads / tile_nodes); }
   TileArgs tile_args() const;
+  // pinned host staging for the drop-in calls' transfers (grown on demand)
+  char* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+  char* stage(size_t bytes);
   int build_csr();
   int tile_bounds(std::vector<int>& bounds) const;
   int build_tiles(const std::vector<int>& bounds);
