@@ -305,18 +305,30 @@ int Ctx::upload_mask(const double* bd, bool check_symmetric) {
     return hip(hipMemsetAsync(d_mask, 0, (size_t)mw.back() * 8, stream), "zero mask");
   std::vector<unsigned long long> words(mw.back(), 0ull);
   std::atomic<int> bad_value{0}, asym{0};
-  if (bd) {
-    // each node writes only its own words
-    parallel_for(N, [&](long long i0, long long i1) {
-      for (int i = (int)i0; i < (int)i1; i++) {
-        const int rs = row[i], deg = row[i + 1] - rs;
-        for (int k = 0; k < deg; k++) {
-          const double d = bd[src[rs + k]];
-          if (d != 0.0 && d != 1.0) bad_value = 1;
-          if (d == 1.0) words[mw[i] + (k >> 6)] |= 1ull << (k & 63);
-          if (check_symmetric && bd[src[rev[rs + k]]] != d) asym = 1;
-        }
+  // each node writes only its own words; a node's bonds are one contiguous
+  // run of the list-order field
+  parallel_for(N, [&](long long i0, long long i1) {
+    for (int i = (int)i0; i < (int)i1; i++) {
+      const int rs = row[i], deg = row[i + 1] - rs;
+      for (int k = 0; k < deg; k++) {
+        const double d = bd[src[rs + k]];
+        if (d != 0.0 && d != 1.0) bad_value = 1;
+        if (d == 1.0) words[mw[i] + (k >> 6)] |= 1ull << (k & 63);
       }
+    }
+  });
+  if (check_symmetric && !bad_value) {
+    // d_ij == d_ji, checked on the bit words (a few MB) rather than on the
+    // list-order field
+    auto bit = [&](int node, int k) {
+      return (words[mw[node] + (k >> 6)] >> (k & 63)) & 1ull;
+    };
+    parallel_for(N, [&](long long i0, long long i1) {
+      for (int i = (int)i0; i < (int)i1; i++)
+        for (int b = row[i]; b < row[i + 1]; b++) {
+          const int j = col[b];
+          if (bit(i, b - row[i]) != bit(j, rev[b] - row[j])) { asym = 1; return; }
+        }
     });
   }
   if (bad_value) return fail(PDG_E_UNSUPPORTED, "bond damage values must be 0 or 1");
