@@ -299,6 +299,27 @@ char* Ctx::stage(size_t bytes) {
   return h_stage;
 }
 
+// Current positions y (reference order, host) -> the records' u = y - x,
+// through the pinned staging buffer when it is available.
+int Ctx::upload_y(const double* y) {
+  if (N <= 0) return PDG_OK;
+  const size_t vbytes = 24 * (size_t)N;
+  char* st = stage(vbytes + 4 * (size_t)N);
+  int rc;
+  if (st) {
+    parallel_for(3 * (long long)N, [&](long long q0, long long q1) {
+      std::memcpy(st + 8 * q0, y + q0, 8 * (size_t)(q1 - q0));
+    });
+    rc = hip(hipMemcpyAsync(d_y, st, vbytes, hipMemcpyHostToDevice, stream), "h2d y");
+  } else {
+    rc = hip(hipMemcpyAsync(d_y, y, vbytes, hipMemcpyHostToDevice, stream), "h2d y");
+  }
+  if (!rc)
+    hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, stream, N, d_y, d_perm,
+                       d_rec[cur], L);
+  return rc;
+}
+
 // Host list (reference order) -> device mask words (internal order).
 int Ctx::upload_mask(const double* bd, bool check_symmetric) {
   if (!bd)  // no field: every bond intact
@@ -746,10 +767,7 @@ int pdg_compute_damage(pdg_ctx* h, const double* y, const double* bdN, double* b
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
   const int N = c->N;
-  int rc = c->hip(hipMemcpyAsync(c->d_y, y, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d y");
-  if (!rc && N > 0)
-    hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y, c->d_perm,
-                       c->d_rec[c->cur], c->L);
+  int rc = c->upload_y(y);
   if (!rc) rc = c->upload_mask(bdN, false);
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
   if (!rc) rc = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, true, nearCritical != nullptr), "pass1");
@@ -782,19 +800,8 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
   // y in and f, theta out go through one pinned buffer (PCIe at full rate);
   // the stream orders the copies, and the call ends synchronised
   const size_t vbytes = 24 * (size_t)N, tbytes = 4 * (size_t)N;
-  char* st = N > 0 ? c->stage(vbytes + tbytes) : nullptr;
-  int rc = 0;
-  if (st) {
-    parallel_for(3 * (long long)N, [&](long long q0, long long q1) {
-      std::memcpy(st + 8 * q0, y + q0, 8 * (size_t)(q1 - q0));
-    });
-    rc = c->hip(hipMemcpyAsync(c->d_y, st, vbytes, hipMemcpyHostToDevice, s), "h2d y");
-  } else {
-    rc = c->hip(hipMemcpyAsync(c->d_y, y, vbytes, hipMemcpyHostToDevice, s), "h2d y");
-  }
-  if (!rc && N > 0)
-    hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y, c->d_perm,
-                       c->d_rec[c->cur], c->L);
+  int rc = c->upload_y(y);
+  char* st = N > 0 ? c->stage(vbytes + tbytes) : nullptr;  // the buffer upload_y used
   if (!rc) rc = c->upload_mask(bdNP1, true);
   if (!rc) rc = c->hip(c->pass1(c->d_rec[c->cur], nullptr, false, false), "pass1");
   if (!rc) rc = c->hip(c->tiled() ? launch_pass2_tiled_force(c->bond_args(), c->tile_args(), c->d_rec[c->cur], c->d_fout, c->tile_lanes(), c->pipe(), s)

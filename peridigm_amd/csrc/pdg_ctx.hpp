@@ -87,6 +87,7 @@ struct Ctx {
   char* h_stage = nullptr;
   size_t h_stage_bytes = 0;
   char* stage(size_t bytes);
+  int upload_y(const double* y);
   int build_csr();
   int tile_bounds(std::vector<int>& bounds) const;
   int build_tiles(const std::vector<int>& bounds);
