@@ -308,6 +308,27 @@ __global__ __launch_bounds__(RB) void k_block_apply(int n, const double* dinv, c
   z[3 * i + 2] = D[6] * r0 + D[7] * r1 + D[8] * r2;
 }
 
+// z = Dinv r and partial sums of r.z in one pass (grid-stride over nodes, one
+// partial per block, like k_dot)
+__global__ __launch_bounds__(RB) void k_block_apply_dot(int n, const double* dinv, const double* r,
+                                                        double* z, double* part) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+    const double* D = dinv + 9 * i;
+    const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+    const double z0 = D[0] * r0 + D[1] * r1 + D[2] * r2;
+    const double z1 = D[3] * r0 + D[4] * r1 + D[5] * r2;
+    const double z2 = D[6] * r0 + D[7] * r1 + D[8] * r2;
+    z[3 * i] = z0;
+    z[3 * i + 1] = z1;
+    z[3 * i + 2] = z2;
+    s += r0 * z0 + r1 * z1 + r2 * z2;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // x += al p; r -= al q; partial sums of r.r
 // Solver scalars kept on the device (slots of `sc`), so a CG iteration needs
 // no host round trip: RZ = r.z, PQ = p.Kp, RR = r.r, RZN = new r.z, ZQ = z.Kp,
@@ -332,6 +353,21 @@ __global__ __launch_bounds__(RB) void k_cg_step(int n, const double* sc, const d
   }
   s = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// RZN = sum of the nb partials, then the CG beta step (as k_cg_beta with
+// flexible = 0) in the same launch
+__global__ __launch_bounds__(RB) void k_final_beta(int nb, const double* part, double* sc) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x != 0) return;
+  sc[SC_RZN] = s;
+  const double pq = sc[SC_PQ];
+  if (!(pq > 0.0)) sc[SC_BRK] = 1.0;
+  sc[SC_BETA] = s / sc[SC_RZ];
+  sc[SC_RZ] = s;
 }
 
 // beta = RZN / RZ (CG) or -ZQ / PQ (flexible CG, one-term truncation); RZ = RZN
@@ -659,7 +695,16 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     return c->hip(hipGetLastError(), "cg step");
   };
   auto seq_b = [&](int cs, int nh) -> int {
-    int r2 = precond_apply(S->r, S->z);
+    int r2 = 0;
+    if (flexible) {
+      r2 = precond_apply(S->r, S->z);
+    } else if (precond == 1) {
+      // z = Dinv r with the r.z partials in the same pass
+      hipLaunchKernelGGL(k_block_apply_dot, red, blk, 0, s, c->N, S->dinv, S->r, S->z, S->part);
+    } else {
+      r2 = precond_apply(S->r, S->z);
+      if (!r2) hipLaunchKernelGGL(k_dot, red, blk, 0, s, n3, S->r, S->z, S->part);
+    }
     if (r2) return r2;
     if (flexible) {
       FcgHist h{};
@@ -673,8 +718,7 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
       hipLaunchKernelGGL(k_fcg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, h,
                          S->ph[(cs + 1) % kFcgSlots]);
     } else {
-      ddot(S->r, S->z, SC_RZN);
-      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, S->scal, 0);
+      hipLaunchKernelGGL(k_final_beta, dim3(1), blk, 0, s, S->nblk, S->part, S->scal);
       hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, S->p);
     }
     return c->hip(hipGetLastError(), "cg direction");
