@@ -158,7 +158,13 @@ int pdg_last_timing(const pdg_ctx* ctx, double* totalMs, double* pass1Ms, double
  * synchronisation, only the total time is recorded. */
 int pdg_set_per_pass_timing(pdg_ctx* ctx, int on);
 
-/* Count of intact bonds flagged near-critical during the last run. */
+/* Near-critical count of the last pdg_verlet_run / pdg_verlet_run_persistent
+ * (or of the last pdg_compute_damage that asked for one): the number of
+ * (bond, damage evaluation) pairs in which an intact bond's FP32 stretch lay
+ * within 1e-6*s_crit of s_crit, summed over every evaluation of the run. Its
+ * break decision may differ from an FP64 evaluation's. No reference
+ * counterpart (src/damage/Peridigm_CriticalStretchDamageModel.cpp:147-161
+ * makes the FP64 decision these bonds approximate). */
 int pdg_near_critical_count(const pdg_ctx* ctx, long long* count);
 
 /* ---- quasi-static (src/core/Peridigm.cpp:2470-3043, :4176-4317) --------
@@ -181,11 +187,18 @@ int pdg_qs_block_diagonal(pdg_ctx* ctx, const double* u, double* blocksInv);
 int pdg_qs_solve(pdg_ctx* ctx, const double* u, const double* rhs, double* x, int precond,
                  double relTol, int maxIter, int* iters, double* relRes);
 /* Newton at fixed prescribed values (u holds them on kinematic DOFs): until
- * ||r||_2 + 20||r||_inf <= tol * max(1, ||reactions||_2) (:2716-2746), each
- * linear solve to linTol. */
+ * ||r||_2 + 20||r||_inf <= tol * ||reactions||_2, the reactions (f V on the
+ * kinematic DOFs) taken once at the entry state, as the reference takes them
+ * once per load step (:2700-2732); each linear solve to linTol. */
 int pdg_qs_newton(pdg_ctx* ctx, double* u, double tol, int maxNewton, int precond,
                   double linTol, int maxLin, int* newtonIters, int* linIters,
                   double* residualNorm);
+/* 1: the Newton tolerance is absolute (multiplier 1), the reference's
+ * "Absolute Tolerance" / useAbsoluteTolerance (src/core/Peridigm.cpp:2514-2517,
+ * :2716-2732); 0 (default): scaled by the reaction norm. */
+int pdg_qs_set_absolute_tolerance(pdg_ctx* ctx, int on);
+/* The tolerance multiplier (reaction norm, or 1) of the last pdg_qs_newton. */
+int pdg_qs_tolerance_multiplier(const pdg_ctx* ctx, double* multiplier);
 /* 1: linear solves replay each CG iteration's two fixed launch sequences as
  * HIP graphs, captured once per context; 0 (default): every kernel is
  * launched directly. Same arithmetic, bitwise. */

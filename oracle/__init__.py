@@ -142,16 +142,33 @@ def force_given_theta(x, y, m, vol, theta, bond_damage, nlist, K, mu, horizon, i
 
 
 def compute_damage(x, y, bd_n, nlist, sc, near_rel=1e-6):
-    """CriticalStretchDamageModel::computeDamage: returns (bd_np1, damage, n_near)."""
+    """CriticalStretchDamageModel::computeDamage: returns (bd_np1, damage, n_near).
+    sc: critical stretch, scalar or one value per node (bond i->j uses node i's)."""
     x = _c(x, np.float64); y = _c(y, np.float64); bd_n = _c(bd_n, np.float64)
     nlist = _c(nlist, np.int32)
     N = x.shape[0]
     out = np.zeros_like(bd_n)
     dmg = np.zeros(N)
     near = C.c_longlong(0)
-    lib().orc_compute_damage(C.c_int(N), _p(x), _p(y), _p(bd_n), _p(out), _p(dmg), _p(nlist),
-                             C.c_double(sc), C.byref(near), C.c_double(near_rel))
+    if np.ndim(sc) > 0:
+        scn = _c(np.broadcast_to(sc, (N,)), np.float64)
+        lib().orc_compute_damage_sc(C.c_int(N), _p(x), _p(y), _p(bd_n), _p(out), _p(dmg),
+                                    _p(nlist), _p(scn), C.byref(near), C.c_double(near_rel))
+    else:
+        lib().orc_compute_damage(C.c_int(N), _p(x), _p(y), _p(bd_n), _p(out), _p(dmg),
+                                 _p(nlist), C.c_double(sc), C.byref(near), C.c_double(near_rel))
     return out, dmg, near.value
+
+
+def degrees(nlist, N):
+    """Neighbour count of every point of a flattened [n, j...] list."""
+    nl = np.asarray(nlist)
+    deg = np.zeros(N, np.int64)
+    li = 0
+    for i in range(N):
+        deg[i] = nl[li]
+        li += int(nl[li]) + 1
+    return deg
 
 
 def bond_stretch(x, y, nlist):

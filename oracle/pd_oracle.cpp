@@ -487,6 +487,18 @@ void orc_compute_damage(int N, const double* x, const double* y, const double* b
   damage_impl(N, x, y, bdNP1, damage, nlist, sc, near_count, near_rel);
 }
 
+// Same call with one critical stretch per node (a bond i->j uses node i's
+// value): one damage-model block per specimen when several specimens share
+// one list.
+void orc_compute_damage_sc(int N, const double* x, const double* y, const double* bdN,
+                           double* bdNP1, double* damage, const int* nlist,
+                           const double* sc_node, long long* near_count, double near_rel) {
+  long long B = list_bonds(N, nlist);
+  if (bdNP1 != bdN) std::memcpy(bdNP1, bdN, sizeof(double) * B);
+  if (near_count) *near_count = 0;
+  damage_impl(N, x, y, bdNP1, damage, nlist, 0.0, near_count, near_rel, sc_node);
+}
+
 // FP64 bond stretch in list order, (|y_j-y_i| - |x_j-x_i|)/|x_j-x_i| exactly as
 // the damage model forms it (used by tests to classify near-critical bonds).
 void orc_bond_stretch(int N, const double* x, const double* y, const int* nlist, double* s) {

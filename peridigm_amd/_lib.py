@@ -73,6 +73,8 @@ SIGNATURES = {
     "pdg_qs_newton": (_i, [_vp, _vp, _d, _i, _i, _d, _i, C.POINTER(_i), C.POINTER(_i),
                            C.POINTER(_d)]),
     "pdg_qs_fallback_count": (_i, [_vp, C.POINTER(_i)]),
+    "pdg_qs_set_absolute_tolerance": (_i, [_vp, _i]),
+    "pdg_qs_tolerance_multiplier": (_i, [_vp, C.POINTER(_d)]),
     "pdg_qs_set_graphs": (_i, [_vp, _i]),
     "pdg_gnn_init": (_i, [_vp, C.c_uint]),
     "pdg_gnn_num_params": (_i, []),

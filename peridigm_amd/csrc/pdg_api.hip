@@ -288,6 +288,9 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
 // caller's memory directly).
 char* Ctx::stage(size_t bytes) {
   if (bytes <= h_stage_bytes) return h_stage;
+  // a copy queued by an earlier call (one that failed before its own
+  // synchronisation) may still read the old buffer
+  (void)hipStreamSynchronize(stream);
   if (h_stage) (void)hipHostFree(h_stage);
   h_stage = nullptr;
   h_stage_bytes = 0;
@@ -300,12 +303,16 @@ char* Ctx::stage(size_t bytes) {
 }
 
 // Current positions y (reference order, host) -> the records' u = y - x,
-// through the pinned staging buffer when it is available.
+// through the pinned staging buffer when it is available. The stream is
+// synchronised first: a call that failed after its upload (a rejected bond
+// field) returns without waiting, and its copy may still read the buffer.
+// Such a call leaves u set to its y.
 int Ctx::upload_y(const double* y) {
   if (N <= 0) return PDG_OK;
   const size_t vbytes = 24 * (size_t)N;
+  int rc = hip(hipStreamSynchronize(stream), "sync");
+  if (rc) return rc;
   char* st = stage(vbytes + 4 * (size_t)N);
-  int rc;
   if (st) {
     parallel_for(3 * (long long)N, [&](long long q0, long long q1) {
       std::memcpy(st + 8 * q0, y + q0, 8 * (size_t)(q1 - q0));
@@ -909,6 +916,8 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
   va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.inv_L = 1.0 / c->L; va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
+  // near-critical bonds are counted over every damage evaluation of the run
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
   if (rc) return rc;
   c->t_pass1 = c->t_pass2 = 0.0;
   rc = c->hip(hipEventRecord(c->ev[0], s), "event");
@@ -929,7 +938,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   auto eval = [&](int k, bool end_kick) -> int {
     int r = 0;
     if (nev) r = c->hip(hipEventRecord(c->tev[3 * nevals], s), "event");
-    if (!r) r = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, damage, false), "pass1");
+    if (!r) r = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, damage, damage), "pass1");
     if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 1], s), "event");
     va.end_kick = end_kick ? 1 : 0;
     va.start_step = k < nsteps ? 1 : 0;
@@ -987,6 +996,7 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
   va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.inv_L = 1.0 / c->L; va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
+  if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
   if (rc) return rc;
   // a continuing run starts step 1 from the stored acceleration
   if (!initialEval) {

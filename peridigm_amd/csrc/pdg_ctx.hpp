@@ -95,6 +95,8 @@ struct Ctx {
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
   unsigned gnn_epoch = 0;  // bumped whenever the network's buffers are (re)allocated
+  bool qs_abs_tol = false;         // Newton tolerance is absolute (no reaction scaling)
+  double qs_tol_multiplier = 1.0;  // reaction norm used by the last pdg_qs_newton
 
   void free_gnn();
   double* qs_dinv();

@@ -899,8 +899,11 @@ int pdg_qs_solve(pdg_ctx* h, const double* u, const double* rhs, double* x, int 
 
 // Newton iterations at fixed prescribed values: u (reference order) holds
 // the prescribed values on fixed DOFs and the initial guess elsewhere.
-// Converged when ||r||_2 + 20 ||r||_inf <= tol * max(1, ||reactions||_2)
-// (src/core/Peridigm.cpp:2716-2746). Each linear solve runs to linTol.
+// Converged when ||r||_2 + 20 ||r||_inf <= tol * ||reactions||_2, with the
+// reactions (f V on the kinematic DOFs) computed once, at the entry state,
+// as the reference does once per load step (src/core/Peridigm.cpp:2700-2732);
+// with the absolute-tolerance flag set the multiplier is 1. Each linear
+// solve runs to linTol.
 int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond, double linTol,
                   int maxLin, int* newtonIters, int* linIters, double* residualNorm) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
@@ -914,22 +917,24 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
   int nit = 0, lit = 0;
   bool converged = false;
   double rnorm = 0.0;
-  std::vector<double> hr(n3), hreac(n3);
-  for (;; nit++) {
-    // reactions (f V on kinematic DOFs) for the tolerance: residual with no
-    // rows zeroed; then the residual proper, which stays in S->out
+  std::vector<double> hr(n3);
+  double multiplier = 1.0;
+  if (!c->qs_abs_tol) {
+    // reactions: the residual with no rows zeroed, read on the kinematic DOFs
+    std::vector<double> hreac(n3);
+    std::vector<signed char> none(n3, 0);
+    rc = c->hip(hipMemcpyAsync(S->fixed, none.data(), n3, hipMemcpyHostToDevice, c->stream), "h2d");
+    if (!rc) rc = qs::residual(c);
+    if (!rc) rc = c->hip(hipMemcpyAsync(hreac.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
+    if (!rc) rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), n3, hipMemcpyHostToDevice, c->stream), "h2d");
+    if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
     double reac = 0.0;
-    {
-      std::vector<signed char> none(n3, 0);
-      rc = c->hip(hipMemcpyAsync(S->fixed, none.data(), n3, hipMemcpyHostToDevice, c->stream), "h2d");
-      if (!rc) rc = qs::residual(c);
-      if (!rc) rc = c->hip(hipMemcpyAsync(hreac.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
-      if (!rc) rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), n3, hipMemcpyHostToDevice, c->stream), "h2d");
-      if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
-      if (rc) return rc;
-      for (int q = 0; q < n3; q++) if (S->hfixed[q]) reac += hreac[q] * hreac[q];
-      reac = std::sqrt(reac);
-    }
+    for (int q = 0; q < n3; q++) if (S->hfixed[q]) reac += hreac[q] * hreac[q];
+    multiplier = std::sqrt(reac);
+  }
+  c->qs_tol_multiplier = multiplier;
+  for (;; nit++) {
     if ((rc = qs::residual(c))) return rc;
     rc = c->hip(hipMemcpyAsync(hr.data(), S->out, 8 * (size_t)n3, hipMemcpyDeviceToHost, c->stream), "d2h");
     if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
@@ -937,7 +942,7 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
     double n2 = 0.0, ninf = 0.0;
     for (double v : hr) { n2 += v * v; ninf = std::max(ninf, std::fabs(v)); }
     rnorm = std::sqrt(n2) + 20.0 * ninf;
-    if (rnorm <= tol * std::max(1.0, reac)) { converged = true; break; }
+    if (rnorm <= tol * multiplier) { converged = true; break; }
     if (nit == maxNewton) break;
     if (precond == 1 && (rc = qs::block_jacobi(c))) return rc;
     if (precond == 2 && (rc = qs::block_jacobi(c))) return rc;  // learned preconditioner uses D too
@@ -957,6 +962,20 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
   rc = qs::to_host(c, S->u, u);
   if (rc) return rc;
   if (!converged) return c->fail(PDG_E_NOCONV, "Newton did not converge");
+  return PDG_OK;
+}
+
+int pdg_qs_set_absolute_tolerance(pdg_ctx* h, int on) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  c->qs_abs_tol = on != 0;
+  return PDG_OK;
+}
+
+int pdg_qs_tolerance_multiplier(const pdg_ctx* h, double* multiplier) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c || !multiplier) return PDG_E_ARG;
+  *multiplier = c->qs_tol_multiplier;
   return PDG_OK;
 }
 

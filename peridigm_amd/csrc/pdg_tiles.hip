@@ -182,7 +182,7 @@ __device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, i
 
 // Lane masks of the wave (compare results kept in scalar registers):
 // predicate codes of the compare builtins.
-enum { CMP_FOGT = 2, CMP_IEQ = 32, CMP_ISLT = 40 };
+enum { CMP_FOGT = 2, CMP_FOLE = 5, CMP_IEQ = 32, CMP_ISLT = 40 };
 
 // Lanes whose bond k = it * G + l is not marked broken in m.
 template <int G>
@@ -254,7 +254,8 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     const int nmax = groups_max<G>((deg + G - 1) / G);
     const int nact = deg - l;  // bond it * G + l exists iff it * G < nact
     float acc = 0.0f;
-    int nnear = 0;
+    int nnear = 0;               // plain loops: per lane
+    unsigned nnear_w = 0;        // pipelined loops: per wave (scalar)
     if (PIPE) {
       // branch-free bond loop: every lane computes, inactive and broken
       // bonds are selected away; the next bond's LDS data is in flight. The
@@ -278,9 +279,13 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
           unsigned long long brkm = 0ull;  // lanes whose bond breaks now
           if (DAMAGE) {
             const float lim = scz * b.zeta;
-            if (NEAR && __builtin_amdgcn_inverse_ballot_w64(okm) &&
-                fabsf(b.ext - lim) <= a.near_rel * lim)
-              nnear++;
+            if (NEAR) {
+              // intact bonds with |s - s_c| <= near_rel * s_c, counted as a
+              // wave-wide lane mask (scalar population count)
+              const unsigned long long nm =
+                  okm & __builtin_amdgcn_fcmpf(fabsf(b.ext - lim), a.near_rel * lim, CMP_FOLE);
+              nnear_w += (unsigned)__builtin_popcountll(nm);
+            }
             brkm = okm & __builtin_amdgcn_fcmpf(b.ext, lim, CMP_FOGT);
           }
           const float w = omega_t<INFL>(b.zeta, c.hhat);
@@ -347,6 +352,8 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       }
       if (NEAR && nnear) atomicAdd(a.near_ctr, (unsigned long long)nnear);
     }
+    if (NEAR && PIPE && nnear_w && (threadIdx.x & 63) == 0)
+      atomicAdd(a.near_ctr, (unsigned long long)nnear_w);
   }
   __syncthreads();  // LDS is reused by the next tile
 }
@@ -512,7 +519,7 @@ __global__ __launch_bounds__(WG, WAVES_PER_SIMD) void k_persistent(BondArgs a, T
   for (int k = first; k <= nsteps; k++) {
     for (int q = blockIdx.x; q < t.ntiles; q += nb) {
       const int tile = xcd_tile_strided(q, nb, t.ntiles);
-      tile_pass1<G, DAMAGE, false, INFL, true>(a, t, rec[cur], theta, nbroken, tile, L);
+      tile_pass1<G, DAMAGE, DAMAGE, INFL, true>(a, t, rec[cur], theta, nbroken, tile, L);
     }
     grid.sync();
     VerletArgs v = va;

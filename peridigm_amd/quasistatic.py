@@ -60,6 +60,19 @@ def newton(ctx: Context, u, tol=1e-6, max_newton=20, precond="block_jacobi", lin
     return u, dict(newton=ni.value, linear=li.value, residual=rn.value)
 
 
+def set_absolute_tolerance(ctx: Context, on=True):
+    """Newton tolerance absolute (multiplier 1) instead of scaled by the
+    reaction norm (the reference's "Absolute Tolerance")."""
+    ctx.check(ctx.lib.pdg_qs_set_absolute_tolerance(ctx.h, int(bool(on))))
+
+
+def tolerance_multiplier(ctx: Context):
+    """Reaction norm (or 1) that scaled the last Newton call's tolerance."""
+    m = C.c_double()
+    ctx.check(ctx.lib.pdg_qs_tolerance_multiplier(ctx.h, C.byref(m)))
+    return m.value
+
+
 def set_graphs(ctx: Context, enable=True):
     """Replay the CG iteration sequences as HIP graphs, or launch every
     kernel directly (the default)."""

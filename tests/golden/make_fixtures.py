@@ -137,15 +137,103 @@ def bond_breaking(ref):
     return out
 
 
+def _exodus_frames(f, frames):
+    """Nodal and element variables of an Exodus (netCDF-3) output file at the
+    given frame indices, keyed by variable name."""
+    def names(k):
+        return [b"".join(r).strip(b"\x00").decode() for r in f.variables[k][:]]
+    out = {}
+    for i, nme in enumerate(names("name_nod_var")):
+        out[nme] = np.asarray(f.variables["vals_nod_var%d" % (i + 1)][:])[frames]
+    for i, nme in enumerate(names("name_elem_var")):
+        out[nme] = np.asarray(f.variables["vals_elem_var%deb1" % (i + 1)][:])[frames]
+    return out
+
+
+def compression_qs_explicit(ref):
+    """Compression_QS_Explicit_3x2x2: 20 quasi-static load steps (frames
+    0..20), then 500 Verlet steps of dt 1e-7 from t = 5e-5 with the same
+    prescribed displacements (frame 21 is the explicit solver's initial
+    output, frame 21+k the state after step k)."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/regression/Compression_QS_Explicit_3x2x2")
+    f = netcdf_file(os.path.join(d, "Compression_QS_Explicit_3x2x2_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    frames = list(range(0, 24)) + [26, 31, 71, 121, 221, 321, 421, 521]
+    vals = _exodus_frames(f, frames)
+    out = dict(
+        deck=dict(n=[3, 2, 2], origin=[-1.5, -1.0, -1.0], length=[3.0, 2.0, 2.0], horizon=1.75,
+                  spherical=True, density=7800.0, K=130.0e9, mu=78.0e9,
+                  qs=dict(t0=0.0, t1=5.0e-5, load_steps=20, absolute_tolerance=1.0e-2,
+                          max_iterations=10),
+                  explicit=dict(t0=5.0e-5, t1=1.0e-4, dt=1.0e-7),
+                  # "Prescribed Displacement" node sets (1-based ids in the deck:
+                  # "1 4 7 10", "3 6 9 12", "1 4", "1 7"), coordinate, value(t)
+                  bcs=[dict(nodes=[0, 3, 6, 9], coord=0, c0=0.0, c1=0.0),
+                       dict(nodes=[2, 5, 8, 11], coord=0, c0=0.0, c1=-0.1 / 0.00005),
+                       dict(nodes=[0, 3], coord=2, c0=0.0, c1=0.0),
+                       dict(nodes=[0, 6], coord=1, c0=0.0, c1=0.0)]),
+        tolerances=dict(displacement=1e-9, velocity=5e-8, force_density=1.0,
+                        weighted_volume=1e-12, dilatation=1e-12, coordinates=1e-12),
+        frames=frames, time=time[frames].tolist(),
+        coordx=np.asarray(f.variables["coordx"][:]).tolist(),
+        coordy=np.asarray(f.variables["coordy"][:]).tolist(),
+        coordz=np.asarray(f.variables["coordz"][:]).tolist())
+    for k, v in vals.items():
+        out[k] = v.tolist()
+    f.close()
+    return out
+
+
+def wave_in_bar(ref):
+    """WaveInBar: Exodus hex mesh (525 HEX8 elements) turned into one point
+    per element, explicit LPS with an initial velocity of -100 m/s on the
+    elements that touch node set 1; 12 output frames (every 20 steps)."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/regression/WaveInBar")
+    g = netcdf_file(os.path.join(d, "WaveInBar.g"), "r", mmap=False)
+    X = np.stack([np.asarray(g.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1)
+    conn = np.asarray(g.variables["connect1"][:]) - 1
+    ns = np.asarray(g.variables["node_ns1"][:]) - 1
+    hexes = X[conn]                                   # (525, 8, 3)
+    # the mesh is made of axis-aligned boxes: volume = product of the extents
+    lo, hi = hexes.min(1), hexes.max(1)
+    assert all(np.allclose(np.sort(np.unique(np.round(h[:, c], 12))), [lo[e, c], hi[e, c]])
+               for e, h in enumerate(hexes) for c in range(3))
+    vol = np.prod(hi - lo, axis=1)
+    in_set = np.isin(conn, ns).any(axis=1)
+    g.close()
+    f = netcdf_file(os.path.join(d, "WaveInBar_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    frames = list(range(len(time)))
+    vals = _exodus_frames(f, frames)
+    out = dict(
+        hex_nodes=hexes, volume=vol, initial_velocity_elements=np.nonzero(in_set)[0],
+        coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+        time=time, density=np.array(2200.0), K=np.array(14.90e9), mu=np.array(8.94e9),
+        horizon=np.array(0.00601), dt=np.array(9.0659e-08), output_every=np.array(20),
+        initial_velocity_x=np.array(-100.0),
+        tol_displacement=np.array(5e-14), tol_velocity=np.array(1e-8),
+        tol_force_density=np.array(5.0), tol_weighted_volume=np.array(1e-15),
+        tol_dilatation=np.array(5e-13), tol_coordinates=np.array(1e-12))
+    for k, v in vals.items():
+        out[k] = v
+    f.close()
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     here = os.path.dirname(os.path.abspath(__file__))
+    np.savez_compressed(os.path.join(here, "wave_in_bar.npz"), **wave_in_bar(ref))
+    print("wrote wave_in_bar.npz")
     fx = {
         "elastic_unit_tests.json": elastic_unit_tests(),
         "neighborhood_list_bar.json": neighborhood_list_bar(),
         "pdquickgrid_2x2x2.json": pdquickgrid_2x2x2(),
         "finite_plane.json": finite_plane(),
         "bond_breaking_initial_velocity.json": bond_breaking(ref),
+        "compression_qs_explicit_3x2x2.json": compression_qs_explicit(ref),
     }
     for name, obj in fx.items():
         with open(os.path.join(here, name), "w") as fh:

@@ -13,6 +13,7 @@ import os
 import numpy as np
 import pytest
 
+import _checks as C
 import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -322,6 +323,11 @@ def test_explicit_bar_matches_oracle():
 
 
 def test_explicit_notched_plate_damage_matches_oracle():
+    """Damaged notched plate, 150 steps. Every damage evaluation is checked at
+    the GPU's own state against the oracle (mismatches only inside the 1e-6
+    band); the whole trajectory is then compared with the oracle's own run,
+    where a bond breaking one step earlier or later moves its neighbourhood
+    (a bounded mismatch count is the second check)."""
     from peridigm_amd import Context
     from peridigm_amd.problems import plate
     p = plate(nx=48, ny=48, nz=4, notch_cells=16, pull=17.3, dt=2.5e-8)
@@ -335,8 +341,9 @@ def test_explicit_notched_plate_damage_matches_oracle():
         ctx.set_density(np.full(len(p["volume"]), p["density"]))
         ctx.set_bcs(p["bcs"])
         ctx.upload_state()
-        ctx.verlet_run(nsteps, p["dt"])
-        st = ctx.download_state()
+        mism_steps, checked, near_gpu, st = C.stepwise_run(ctx, p["x"], nl, p["critical_stretch"],
+                                                           nsteps, p["dt"])
+    assert checked == nsteps + 1
     nb = r["bond_damage"].sum()
     assert nb > 0, "the workload must break bonds"
     mism = np.count_nonzero(st["bond_damage"] != r["bond_damage"])
@@ -344,12 +351,48 @@ def test_explicit_notched_plate_damage_matches_oracle():
     umax = np.abs(r["u"]).max()
     err_max = np.abs(du).max() / umax
     err_rms = np.sqrt((du ** 2).mean()) / np.sqrt((r["u"] ** 2).mean())
-    print(f"broken={int(nb)} mask_mismatch={mism} u_err max={err_max:.2e} rms={err_rms:.2e}")
-    # a bond that breaks one step earlier or later in FP32 than in FP64 moves
-    # its neighbourhood: bounded mismatch count, small RMS, looser max
+    print(f"broken={int(nb)} same-state mismatches={mism_steps} near(gpu)={near_gpu} "
+          f"trajectory mask mismatch={mism} u_err max={err_max:.2e} rms={err_rms:.2e}")
     assert mism <= max(2, 1e-3 * nb)
     assert err_rms <= 1e-5
     assert err_max <= 1e-4
+
+
+def test_near_critical_count_in_runs():
+    """Explicit runs count intact bonds whose FP32 stretch lies in the 1e-6
+    band, over every damage evaluation of the run: the count restarts with
+    each run and equals the sum of the counts of the same steps run one at a
+    time. A critical stretch set on purpose at a bond's stretch gives a
+    nonzero count that the oracle confirms (its own FP64 near count)."""
+    from peridigm_amd import Context
+    x, vol, nl = O.quickgrid(12, 6, 6, [0, 0, 0], [12e-3, 6e-3, 6e-3], 3.015e-3)
+    N = len(vol)
+    rng = np.random.default_rng(3)
+    u = 1e-4 * x + 1e-8 * rng.standard_normal(x.shape)
+    s = O.bond_stretch(x, x + u, nl)
+    sc = float(np.median(s))  # many bonds sit at this stretch
+    with Context(0) as ctx:
+        ctx.set_topology(nl, N)
+        ctx.set_constants(x, vol, 130e9, 78e9, 3.015e-3, 0, sc)
+        ctx.set_density(np.full(N, 7800.0))
+        ctx.upload_state(u=u)
+        ctx.verlet_run(0, 1e-12)
+        n0 = ctx.near_critical_count()
+        _, _, near_o = O.compute_damage(x, x + u, np.zeros(ctx.B), nl, sc)
+        assert n0 > 0 and near_o > 0
+        assert abs(n0 - near_o) <= max(2, 0.05 * near_o)
+        ctx.upload_state(u=u)
+        ctx.verlet_run(3, 1e-12)
+        n3 = ctx.near_critical_count()
+        ctx.upload_state(u=u)
+        parts = []
+        ctx.verlet_run(0, 1e-12)
+        parts.append(ctx.near_critical_count())
+        for k in range(3):
+            ctx.verlet_run(1, 1e-12, t0=k * 1e-12, initial_eval=False)
+            parts.append(ctx.near_critical_count())
+        assert n3 == sum(parts)
+        print(f"near-critical: gpu {n0}, oracle {near_o}; 3-step run {n3} = {parts}")
 
 
 def test_deterministic_and_lane_independent():
@@ -429,6 +472,9 @@ def test_persistent_launch_bitwise_equals_per_step_launches():
 
 
 def test_batched_specimens_match_oracle():
+    """Five uneven plates with their own critical stretch in one persistent
+    launch: the last step's damage decisions at the same state, and the
+    trajectory against the oracle's run with per-node s_c."""
     from peridigm_amd import Context
     from peridigm_amd.problems import batched_plates
     p = batched_plates(nspec=5, seed=7, nodes=(1500, 5000), notch=(5, 15), pull=17.3)
@@ -445,13 +491,16 @@ def test_batched_specimens_match_oracle():
         ctx.set_density(np.full(N, p["density"]))
         ctx.set_bcs(p["bcs"])
         ctx.upload_state()
-        ctx.verlet_run_persistent(nsteps, p["dt"])
+        ctx.verlet_run_persistent(nsteps - 1, p["dt"])
+        bd_prev = ctx.download_state()["bond_damage"]
+        ctx.verlet_run_persistent(1, p["dt"], t0=(nsteps - 1) * p["dt"], initial_eval=False)
         st = ctx.download_state()
+    last, _ = C.same_state_masks(p["x"], st["u"], nl, bd_prev, st["bond_damage"], sc)
     nb = r["bond_damage"].sum()
     mism = np.count_nonzero(st["bond_damage"] != r["bond_damage"])
     du = st["u"] - r["u"]
-    print(f"specimens={len(p['specimens'])} N={N} broken={int(nb)} mismatch={mism} "
-          f"u_err max={np.abs(du).max() / np.abs(r['u']).max():.2e}")
+    print(f"specimens={len(p['specimens'])} N={N} broken={int(nb)} last-step mismatches={last} "
+          f"trajectory mismatch={mism} u_err max={np.abs(du).max() / np.abs(r['u']).max():.2e}")
     assert nb > 0
     assert mism <= max(2, 1e-3 * nb)
     assert np.abs(du).max() <= 1e-4 * np.abs(r["u"]).max()
@@ -459,7 +508,8 @@ def test_batched_specimens_match_oracle():
 
 def test_expanding_cylinder_matches_oracle():
     """Perturbed hollow cylinder (radius search, radial initial velocity,
-    critical stretch 1.5e-3) at test scale against the oracle."""
+    critical stretch 1.5e-3) at test scale against the oracle: every damage
+    evaluation at the same state, then the trajectory."""
     from peridigm_amd import Context
     from peridigm_amd.discretization import radius_search
     from peridigm_amd.problems import cylinder
@@ -475,13 +525,13 @@ def test_expanding_cylinder_matches_oracle():
         ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, p["critical_stretch"])
         ctx.set_density(np.full(N, p["density"]))
         ctx.upload_state(v=p["v0"])
-        ctx.verlet_run(nsteps, p["dt"])
-        st = ctx.download_state()
+        mism_steps, checked, near_gpu, st = C.stepwise_run(ctx, p["x"], nl, p["critical_stretch"],
+                                                           nsteps, p["dt"])
     nb = r["bond_damage"].sum()
     mism = np.count_nonzero(st["bond_damage"] != r["bond_damage"])
     du = st["u"] - r["u"]
-    print(f"cylinder N={N} broken={int(nb)} mismatch={mism} "
-          f"u_err max={np.abs(du).max() / np.abs(r['u']).max():.2e}")
+    print(f"cylinder N={N} broken={int(nb)} same-state mismatches={mism_steps} "
+          f"trajectory mismatch={mism} u_err max={np.abs(du).max() / np.abs(r['u']).max():.2e}")
     assert nb > 0
     assert mism <= max(2, 1e-3 * nb)
     assert np.abs(du).max() <= 1e-4 * np.abs(r["u"]).max()

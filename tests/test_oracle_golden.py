@@ -183,6 +183,110 @@ def test_bond_breaking_initial_velocity_gold(golden_dir):
         assert np.abs(r["damage"] - g["Damage"][k]).max() <= tol["damage"]
 
 
+def test_prescribed_displacement_closed_form():
+    """DirichletIncrementBC on the velocity (Peridigm_BoundaryCondition.cpp:
+    136-143, 202-241) inside the Verlet step (Peridigm.cpp:1480-1501): after
+    step n a prescribed DOF with value c1 t + c0 holds u = u_0 + c1 (t_n - t_0)
+    and, after the closing half kick, v = c1 + dt/2 a_n. A constant value
+    (c1 = 0) is reached in the first step through the zero-increment rule
+    (the increment is taken against the current displacement) and then held.
+    Applying the BC before the half kick, dividing by t_n - t_0, or dropping
+    the zero-increment rule each break one of these."""
+    x, vol, nl = O.quickgrid(6, 4, 4, [0, 0, 0], [6e-3, 4e-3, 4e-3], 3.015e-3)
+    N = len(vol)
+    left = np.nonzero(x[:, 0] < 1e-3)[0]
+    right = np.nonzero(x[:, 0] > 5e-3)[0]
+    c1, c0 = 2.0, 3.0e-7
+    bcs = [(right, 0, 0.0, c1), (left, 1, c0, 0.0)]
+    dt, t0 = 1.0e-8, 2.5e-7
+    rng = np.random.default_rng(4)
+    v0 = rng.uniform(-1, 1, (N, 3))  # overwritten on prescribed DOFs in step 1
+    for n in (1, 2, 7, 25):
+        r = O.explicit_run(x, vol, 7800.0, nl, 130e9, 78e9, 3.015e-3, 0.0, dt, n, v0=v0, bcs=bcs,
+                           t0=t0)
+        tn = t0 + n * dt
+        u_lin = c1 * (tn - t0)
+        assert np.abs(r["u"][right, 0] - u_lin).max() <= 1e-12 * abs(u_lin)
+        assert np.abs(r["v"][right, 0] - (c1 + dt / 2 * r["a"][right, 0])).max() <= 1e-12 * c1
+        assert np.abs(r["u"][left, 1] - c0).max() <= 1e-14 * c0
+        v_const = 0.0 if n > 1 else c0 / dt
+        assert np.abs(r["v"][left, 1] - (v_const + dt / 2 * r["a"][left, 1])).max() <= \
+            1e-9 * max(abs(v_const), 1.0)
+        # free DOFs moved (the BC rows are not simply frozen)
+        assert np.abs(r["u"][right, 1]).max() > 0.0
+
+
+def test_compression_qs_explicit_gold(golden_dir):
+    """Explicit phase of Compression_QS_Explicit_3x2x2 (12-point PdQuickGrid,
+    four prescribed-displacement sets, one of them ramped): from the explicit
+    solver's initial frame (t = 5e-5) the oracle's Verlet steps reproduce the
+    reference's displacement, velocity, force density, dilatation and
+    weighted volume at the .comp tolerances."""
+    g = load(golden_dir, "compression_qs_explicit_3x2x2.json")
+    dk = g["deck"]
+    tol = g["tolerances"]
+    x, vol, nl = O.quickgrid(*dk["n"], dk["origin"], dk["length"], dk["horizon"],
+                             spherical=dk["spherical"])
+    assert np.abs(x[:, 0] - g["coordx"]).max() <= tol["coordinates"]
+    assert np.abs(x[:, 1] - g["coordy"]).max() <= tol["coordinates"]
+    assert np.abs(x[:, 2] - g["coordz"]).max() <= tol["coordinates"]
+    bcs = [(b["nodes"], b["coord"], b["c0"], b["c1"]) for b in dk["bcs"]]
+    frames = g["frames"]
+    k0 = frames.index(21)
+    u0 = np.stack([g["Displacement" + c][k0] for c in "XYZ"], 1)
+    v0 = np.stack([g["Velocity" + c][k0] for c in "XYZ"], 1)
+    ex = dk["explicit"]
+    assert g["time"][k0] == ex["t0"]
+    checked = 0
+    for k, fr in enumerate(frames):
+        if fr <= 21:
+            continue
+        n = fr - 21
+        r = O.explicit_run(x, vol, dk["density"], nl, dk["K"], dk["mu"], dk["horizon"], 0.0,
+                           ex["dt"], n, u0=u0, v0=v0, bcs=bcs, t0=ex["t0"])
+        assert abs(g["time"][k] - (ex["t0"] + n * ex["dt"])) <= 1e-14
+        for c, nm in enumerate("XYZ"):
+            assert np.abs(r["u"][:, c] - g["Displacement" + nm][k]).max() <= tol["displacement"]
+            assert np.abs(r["v"][:, c] - g["Velocity" + nm][k]).max() <= tol["velocity"]
+            assert np.abs(r["force"][:, c] - g["Force_Density" + nm][k]).max() <= tol["force_density"]
+        assert np.abs(r["theta"] - g["Dilatation"][k]).max() <= tol["dilatation"]
+        assert np.abs(r["m"] - g["Weighted_Volume"][k]).max() <= tol["weighted_volume"]
+        checked += 1
+    assert checked >= 8
+
+
+def test_wave_in_bar_gold(golden_dir):
+    """WaveInBar (Exodus hex mesh, 525 points, radius search <= R^2, initial
+    velocity on the elements touching node set 1): the oracle's Verlet run
+    reproduces every output frame of the reference's displacement,
+    velocity, force density, dilatation and weighted volume at the .comp
+    tolerances."""
+    g = np.load(os.path.join(golden_dir, "wave_in_bar.npz"))
+    x = g["coords"]
+    vol = g["volume"]
+    N = len(vol)
+    nl = O.radius_search(x, float(g["horizon"]))
+    v = np.zeros((N, 3))
+    v[g["initial_velocity_elements"], 0] = float(g["initial_velocity_x"])
+    assert np.array_equal(v[:, 0], g["VelocityX"][0])
+    u = None
+    every = int(g["output_every"])
+    dt = float(g["dt"])
+    for k in range(1, len(g["time"])):
+        assert abs(g["time"][k] - k * every * dt) <= 1e-14
+        r = O.explicit_run(x, vol, float(g["density"]), nl, float(g["K"]), float(g["mu"]),
+                           float(g["horizon"]), 0.0, dt, every, u0=u, v0=v,
+                           t0=(k - 1) * every * dt)
+        u, v = r["u"], r["v"]
+        for c, nm in enumerate("XYZ"):
+            assert np.abs(u[:, c] - g["Displacement" + nm][k]).max() <= float(g["tol_displacement"])
+            assert np.abs(v[:, c] - g["Velocity" + nm][k]).max() <= float(g["tol_velocity"])
+            assert np.abs(r["force"][:, c] - g["Force_Density" + nm][k]).max() <= \
+                float(g["tol_force_density"])
+        assert np.abs(r["theta"] - g["Dilatation"][k]).max() <= float(g["tol_dilatation"])
+        assert np.abs(r["m"] - g["Weighted_Volume"][k]).max() <= float(g["tol_weighted_volume"])
+
+
 def test_critical_time_step_closed_form():
     x = np.array([[0, 0, 0], [0.7, 0, 0]], float)
     vol = np.array([0.3, 0.5])
