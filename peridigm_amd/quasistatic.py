@@ -92,9 +92,13 @@ def end_dofs(left, right):
     return (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
 
 
-def run_tension(ctx: Context, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8):
+def run_tension(ctx: Context, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8,
+                keep_entry=False):
     """Tensile load stepping of the qs_bar problem: end layers displaced by
-    -/+ strain*Lx/2 * s (y, z held) for s = k/load_steps."""
+    -/+ strain*Lx/2 * s (y, z held) for s = k/load_steps. Each step's stats
+    hold the Newton and linear iteration counts, the final residual norm and
+    the tolerance multiplier (the reaction norm at the step's entry state);
+    with keep_entry, also that entry state (u_entry)."""
     N = ctx.N
     set_fixed(ctx, end_dofs(p["left"], p["right"]))
     Lx = p["length"][0]
@@ -107,7 +111,11 @@ def run_tension(ctx: Context, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8)
         u[p["right"], :] = 0.0
         u[p["left"], 0] = -0.5 * p["strain"] * Lx * s
         u[p["right"], 0] = 0.5 * p["strain"] * Lx * s
+        entry = u.copy() if keep_entry else None
         u, st = newton(ctx, u, tol=tol, precond=precond, lin_tol=lin_tol)
+        st["multiplier"] = tolerance_multiplier(ctx)
+        if keep_entry:
+            st["u_entry"] = entry
         stats.append(st)
     wall = time.perf_counter() - t0
     return u, stats, wall

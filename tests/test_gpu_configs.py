@@ -131,21 +131,26 @@ def test_qs_bar_60x20x20_ten_load_steps(precond):
             R, E, _ = Lr.generate_pairs(ctx, p, nsamples=100, seed=11)
             Lr.init(ctx, 5)
             Lr.train(ctx, R, E, epochs=20, lr=2e-3, seed=3)
-        u, stats, wall = Q.run_tension(ctx, p, precond=precond, tol=tol, lin_tol=1e-8)
-        mult = Q.tolerance_multiplier(ctx)
+        u, stats, wall = Q.run_tension(ctx, p, precond=precond, tol=tol, lin_tol=1e-8,
+                                       keep_entry=True)
         m = ctx.initialize()
     print(f"QS bar {precond}: {wall:.3f} s, Newton {sum(s['newton'] for s in stats)}, "
           f"linear {sum(s['linear'] for s in stats)}")
     assert len(stats) == 10
-    f = O.force_at(x, u, m, vol, np.zeros(nl.size - N), nl, p["K"], p["mu"], p["horizon"])
-    r = f * vol[:, None]
+    bd0 = np.zeros(nl.size - N)
     free = np.ones(N, bool)
     free[p["left"]] = False
     free[p["right"]] = False
-    reac = np.linalg.norm(r[~free])
-    rf = r[free]
-    # the last load step's tolerance was scaled by the reactions at its entry
-    assert 0.5 * reac <= mult <= 2.0 * reac
+    # the last load step's tolerance is tol times the reaction norm (f V on
+    # the kinematic DOFs) at that step's entry state: the previous solution
+    # with the new end displacements (src/core/Peridigm.cpp:2716-2732)
+    last = stats[-1]
+    fe = O.force_at(x, last["u_entry"], m, vol, bd0, nl, p["K"], p["mu"], p["horizon"])
+    reac = np.linalg.norm((fe * vol[:, None])[~free])
+    mult = last["multiplier"]
+    assert abs(mult - reac) <= 1e-8 * reac
+    f = O.force_at(x, u, m, vol, bd0, nl, p["K"], p["mu"], p["horizon"])
+    rf = (f * vol[:, None])[free]
     assert np.linalg.norm(rf) + 20 * np.abs(rf).max() <= tol * mult * (1 + 1e-6)
     Lx = p["length"][0]
     assert np.allclose(u[p["right"], 0], 0.5 * p["strain"] * Lx, rtol=0, atol=1e-15)

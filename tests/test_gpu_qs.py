@@ -163,7 +163,11 @@ def test_newton_tolerance_scales_with_reactions_below_one():
     from peridigm_amd import Context
     from peridigm_amd import quasistatic as Q
     from peridigm_amd.problems import qs_bar
-    p = qs_bar(nx=16, ny=6, nz=6, strain=1e-12)
+    # a soft material (K = 130 Pa, mu = 78 Pa) at 1e-3 strain: reactions far
+    # below 1 N with a well-resolved residual (a tiny strain instead would put
+    # the FP64 extension |Y| - |xi| at its round-off floor)
+    p = qs_bar(nx=16, ny=6, nz=6, strain=1e-3)
+    p["K"], p["mu"] = 130.0, 78.0
     xo, vo, nl = O.quickgrid(*p["n"], p["origin"], p["length"], p["horizon"])
     N = len(vo)
     with Context(0) as ctx:
