@@ -146,7 +146,9 @@ int pdg_verlet_run(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEv
 int pdg_verlet_run_persistent(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEval,
                               int* gridSize);
 
-/* Stable time step (ComputeCriticalTimeStep, constant horizon) for density rho. */
+/* Stable time step (ComputeCriticalTimeStep, constant horizon,
+ * src/core/Peridigm_CriticalTimeStep.cpp:54-126) for density rho, computed on
+ * the device: min over nodes of sqrt(2 rho / sum_j V_j 18K/(pi delta^4) / |xi|). */
 int pdg_critical_time_step(pdg_ctx* ctx, double density, double* dtOut);
 
 /* Per-kernel timing of the last pdg_verlet_run (ms, HIP events on the

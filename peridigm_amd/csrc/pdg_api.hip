@@ -18,22 +18,23 @@ using namespace pdg;
 
 namespace {
 
+// Records' u = (y - x) / uq from current positions y (reference order);
+// x is the FP64 reference position in internal order.
 __global__ void k_set_u(int n, const double* __restrict__ y, const int* __restrict__ perm,
-                        NodeRec* __restrict__ rec, double L) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 3 * n) return;
-  const int k = q / 3, c = q - 3 * (q / 3);
+                        const double* __restrict__ x, NodeRec* __restrict__ rec, double uq) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int k = t / 3, c = t - 3 * (t / 3);
   const int p = perm[k];
-  const double x = rec[k].x[c] * L;
-  rec[k].u[c] = (y[3 * p + c] - x) * (1.0 / L);
+  rec[k].u[c] = (y[3 * p + c] - x[t]) * (1.0 / uq);
 }
 
 __global__ void k_set_u_disp(int n, const double* __restrict__ u, const int* __restrict__ perm,
-                             NodeRec* __restrict__ rec, double L) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 3 * n) return;
-  const int k = q / 3, c = q - 3 * (q / 3);
-  rec[k].u[c] = u[3 * perm[k] + c] * (1.0 / L);
+                             NodeRec* __restrict__ rec, double uq) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int k = t / 3, c = t - 3 * (t / 3);
+  rec[k].u[c] = u[3 * perm[k] + c] * (1.0 / uq);
 }
 
 // Hilbert index of a 21-bit cell triple (Skilling's transpose algorithm:
@@ -117,14 +118,14 @@ void Ctx::free_device() {
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(d_row); F(d_col); F(d_mw); F(d_mask); F(d_nc); F(d_rec[0]); F(d_rec[1]); F(d_theta);
   F(d_nbroken); F(d_v); F(d_a); F(d_f); F(d_rho); F(d_bcslot); F(d_bcs); F(d_err); F(d_near);
-  F(d_perm); F(d_fout); F(d_y);
+  F(d_perm); F(d_fout); F(d_y); F(d_vol); F(d_x);
   F(d_tile_start); F(d_halo_off); F(d_halo); F(d_own_off); F(d_lcol);
   d_tile_start = d_halo_off = d_halo = d_own_off = nullptr;
   d_lcol = nullptr;
   d_row = d_col = d_mw = d_nbroken = d_bcslot = d_err = d_perm = nullptr;
   d_mask = d_near = nullptr;
   d_nc = nullptr; d_rec[0] = d_rec[1] = nullptr; d_theta = nullptr;
-  d_v = d_a = d_f = d_rho = d_fout = d_y = nullptr;
+  d_v = d_a = d_f = d_rho = d_fout = d_y = d_vol = d_x = nullptr;
   d_bcs = nullptr;
 }
 
@@ -139,6 +140,7 @@ BondArgs Ctx::bond_args() const {
   BondArgs a;
   a.n = N; a.row = d_row; a.col = d_col; a.mw = d_mw; a.mask = d_mask; a.nc = d_nc;
   a.near_rel = 1e-6f; a.near_ctr = d_near; a.influence = infl;
+  a.fq = std::ldexp(1.0f, -qexp);
   return a;
 }
 
@@ -269,7 +271,7 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
       own[t] = (int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin());
       for (int k = n0; k < n1; k++)
         for (int b = row[k]; b < row[k + 1]; b++)
-          lcol[b] = (unsigned short)(8 * (std::lower_bound(H.begin(), H.end(), col[b]) - H.begin()));
+          lcol[b] = (unsigned short)(kLdsSlot * (std::lower_bound(H.begin(), H.end(), col[b]) - H.begin()));
     }
   }, 32);
   if (over) return 1;
@@ -323,7 +325,7 @@ int Ctx::upload_y(const double* y) {
   }
   if (!rc)
     hipLaunchKernelGGL(k_set_u, dim3((3 * N + 255) / 256), dim3(256), 0, stream, N, d_y, d_perm,
-                       d_rec[cur], L);
+                       d_x, d_rec[cur], q);
   return rc;
 }
 
@@ -608,6 +610,34 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   const double vmean = N ? vsum / N : 1.0;
   c->L = std::ldexp(1.0, (int)std::lround(std::log2(horizon)));
   c->S = std::ldexp(1.0, vmean > 0 ? (int)std::lround(std::log2(vmean)) : 0);
+  // Position quantum q = L * 2^-qexp: the longest bond component must stay
+  // below 2^30 q, so that bond differences of the 32-bit fixed-point
+  // positions are exact (|difference| < 2^31). 2^30 quanta over the longest
+  // bond give positions to about 1e-9 of a bond length.
+  {
+    std::vector<double> mx(std::max(1, N), 0.0);
+    parallel_for(N, [&](long long i0, long long i1) {
+      for (int i = (int)i0; i < (int)i1; i++) {
+        double m = 0.0;
+        for (int b = c->row[i]; b < c->row[i + 1]; b++) {
+          const int j = c->perm[c->col[b]], pi = c->perm[i];
+          for (int d = 0; d < 3; d++)
+            m = std::max(m, std::fabs(x[3 * (size_t)j + d] - x[3 * (size_t)pi + d]));
+        }
+        mx[i] = m;
+      }
+    });
+    const double maxlen = *std::max_element(mx.begin(), mx.end()) / c->L;
+    int e = 0;
+    if (maxlen > 0.0) (void)std::frexp(maxlen, &e);  // maxlen < 2^e
+    c->qexp = 30 - e;
+    c->q = std::ldexp(c->L, -c->qexp);
+    double xmax = 0.0;
+    for (size_t t = 0; t < 3 * (size_t)N; t++) xmax = std::max(xmax, std::fabs(x[t]));
+    if (!(xmax / c->q < 4.0e18) || c->qexp > 120 || c->qexp < -120)
+      return c->fail(PDG_E_UNSUPPORTED, "coordinates too large relative to the bond lengths "
+                                        "for 64-bit fixed point");
+  }
 
   c->free_device();
   auto A = [&](void** p, size_t bytes, const char* what) {
@@ -637,6 +667,8 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if ((rc = A((void**)&c->d_err, 4, "alloc err"))) return rc;
   if ((rc = A((void**)&c->d_near, 8, "alloc near"))) return rc;
   if ((rc = A((void**)&c->d_perm, (size_t)N * 4, "alloc perm"))) return rc;
+  if ((rc = A((void**)&c->d_vol, (size_t)N * 8, "alloc vol"))) return rc;
+  if ((rc = A((void**)&c->d_x, (size_t)N * 24, "alloc x"))) return rc;
   if (c->ntiles > 0) {
     if ((rc = A((void**)&c->d_tile_start, 4 * c->tile_start.size(), "alloc tiles"))) return rc;
     if ((rc = A((void**)&c->d_halo_off, 4 * c->halo_off.size(), "alloc halo off"))) return rc;
@@ -672,31 +704,40 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   // Node records and constants (internal order).
   std::vector<NodeRec> rec(N);
   std::vector<NodeConst> nc(N);
-  const double L = c->L, S = c->S, invL = 1.0 / L;
+  const double L = c->L, S = c->S, invq = 1.0 / c->q;
+  const double s2 = std::ldexp(1.0, -2 * c->qexp);  // sums in q units -> L units
+  std::vector<double> x_int(3 * (size_t)N);
   for (int k = 0; k < N; k++) {
     const int p = c->perm[k];
     const double m = c->m[p];
     NodeRec& r = rec[k];
-    for (int d = 0; d < 3; d++) { r.x[d] = x[3 * (size_t)p + d] * invL; r.u[d] = 0.0; }
+    for (int d = 0; d < 3; d++) {
+      const double xd = x[3 * (size_t)p + d];
+      x_int[3 * (size_t)k + d] = xd;
+      r.xq[d] = (unsigned)(unsigned long long)std::llrint(xd * invq);
+      r.u[d] = 0.0;
+    }
     r.vol = (float)(vol[p] / S);
     r.p = 0.0f;
-    r.pad = 0.0f;
+    for (float& z : r.pad) z = 0.0f;
     NodeConst& q = nc[k];
     q.sc = sc > 0.0 ? (float)sc : 0.0f;
-    q.hhat = (float)(horizon * invL);
+    q.hhat = (float)(horizon * invq);
     if (m > 0.0) {
       const double alpha = 15.0 * mu / m;
       const double beta = 3.0 * K / m - alpha / 3.0;
       const double ct = 3.0 * L * L * S / m;
       r.alpha = (float)(alpha * L * S);
-      q.ctheta = (float)ct;
-      q.kp = (float)(ct * beta * L * S);
+      q.ctheta = (float)(ct * s2);
+      q.kp = (float)(ct * beta * L * S * s2);
     } else {
       r.alpha = 0.0f; q.ctheta = 0.0f; q.kp = 0.0f;
     }
   }
   std::vector<int> noslot(N, -1);
   std::vector<double> rho1(N, 1.0);
+  std::vector<double> vol_int(N);
+  for (int k = 0; k < N; k++) vol_int[k] = vol[c->perm[k]];
   rc = c->hip(hipMemcpyAsync(c->d_row, c->row.data(), 4 * (N + 1), hipMemcpyHostToDevice, s), "h2d row");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_col, c->col.data(), 4 * c->B, hipMemcpyHostToDevice, s), "h2d col");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_mw, c->mw.data(), 4 * (N + 1), hipMemcpyHostToDevice, s), "h2d mw");
@@ -712,6 +753,8 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_bcslot, noslot.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d bcslot");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_rho, rho1.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d rho");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_perm, c->perm.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d perm");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_vol, vol_int.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d vol");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_x, x_int.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d x");
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
   if (!rc && c->ntiles > 0) {
@@ -857,13 +900,13 @@ int pdg_state_upload(pdg_ctx* h, const double* u, const double* v, const double*
     rc = c->hip(hipMemcpyAsync(c->d_y, u, 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d u");
     if (!rc && N > 0)
       hipLaunchKernelGGL(k_set_u_disp, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y,
-                         c->d_perm, c->d_rec[c->cur], c->L);
+                         c->d_perm, c->d_rec[c->cur], c->q);
   } else if (N > 0) {
     std::vector<double> z(3 * (size_t)N, 0.0);
     rc = c->hip(hipMemcpyAsync(c->d_y, z.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d u");
     if (!rc)
       hipLaunchKernelGGL(k_set_u_disp, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, c->d_y,
-                         c->d_perm, c->d_rec[c->cur], c->L);
+                         c->d_perm, c->d_rec[c->cur], c->q);
     if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
   }
   if (!rc) rc = c->upload_vec3(v, c->d_v);
@@ -914,7 +957,8 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   BondArgs ba = c->bond_args();
   VerletArgs va{};
   va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
-  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.inv_L = 1.0 / c->L; va.err = c->d_err;
+  va.bcs = c->d_bcs; va.dt = dt; va.uq = c->q; va.inv_uq = 1.0 / c->q;
+  va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
   // near-critical bonds are counted over every damage evaluation of the run
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
@@ -994,7 +1038,8 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   hipStream_t s = c->stream;
   VerletArgs va{};
   va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
-  va.bcs = c->d_bcs; va.dt = dt; va.L = c->L; va.inv_L = 1.0 / c->L; va.err = c->d_err;
+  va.bcs = c->d_bcs; va.dt = dt; va.uq = c->q; va.inv_uq = 1.0 / c->q;
+  va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_near, 0, 8, s), "zero near");
   if (rc) return rc;
@@ -1042,7 +1087,7 @@ int pdg_state_download(pdg_ctx* h, double* u, double* v, double* a, double* bd, 
     if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
     if (rc) return rc;
     for (int k = 0; k < N; k++)
-      for (int d = 0; d < 3; d++) u[3 * (size_t)c->perm[k] + d] = rec[k].u[d] * c->L;
+      for (int d = 0; d < 3; d++) u[3 * (size_t)c->perm[k] + d] = rec[k].u[d] * c->q;
   }
   if (v && (rc = c->download_vec3(c->d_v, v))) return rc;
   if (a && (rc = c->download_vec3(c->d_a, a))) return rc;
@@ -1061,34 +1106,6 @@ int pdg_state_download(pdg_ctx* h, double* u, double* v, double* a, double* bd, 
       if (dilatation) dilatation[c->perm[k]] = (double)th[k];
     }
   }
-  return PDG_OK;
-}
-
-int pdg_critical_time_step(pdg_ctx* h, double density, double* dtOut) {
-  Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !dtOut) return PDG_E_ARG;
-  // src/core/Peridigm_CriticalTimeStep.cpp:54-126 (constant horizon)
-  const double pi = 3.14159265358979323846;
-  const double hz = c->horizon;
-  const double spring = 18.0 * c->K / (pi * hz * hz * hz * hz);
-  double best = 1.0e50;
-  for (int p = 0; p < c->N; p++) {
-    const long long h0 = c->head[p];
-    const int nn = c->nlist[h0];
-    const double* X = &c->x[3 * (size_t)p];
-    double den = 0.0;
-    for (int k = 0; k < nn; k++) {
-      const int j = c->nlist[h0 + 1 + k];
-      const double* Y = &c->x[3 * (size_t)j];
-      const double d = std::sqrt((X[0] - Y[0]) * (X[0] - Y[0]) + (X[1] - Y[1]) * (X[1] - Y[1]) +
-                                 (X[2] - Y[2]) * (X[2] - Y[2]));
-      den += c->vol[j] * spring / d;
-    }
-    double dt = 1.0e50;
-    if (nn > 0) dt = std::sqrt(2.0 * density / den);
-    if (dt < best) best = dt;
-  }
-  *dtOut = best;
   return PDG_OK;
 }
 

@@ -15,8 +15,9 @@
 // group with cross-lane shuffles: no atomics, deterministic results.
 //
 // Bond arithmetic is FP32 with compensated deformed-bond vectors: the bond
-// vector xi = x_j - x_i and the displacement difference eta = u_j - u_i are
-// formed in FP64 from the FP64 records and rounded once to FP32, and the
+// vector xi = x_j - x_i is the exact integer difference of the records'
+// fixed-point positions and the displacement difference eta = u_j - u_i is
+// formed in FP64 from the records; both are rounded once to FP32, and the
 // extension uses the cancellation-free identity
 //     e = |Y| - |xi| = eta . (xi + Y) / (|Y| + |xi|),   Y = xi + eta,
 // so stretches near 1e-4 keep full FP32 relative accuracy. Sums are FP32.
@@ -49,14 +50,15 @@ __global__ __launch_bounds__(256) void k_pass1(BondArgs a, NodeRec* __restrict__
   const bool valid = node < a.n;
 
   int rs = 0, deg = 0, w0 = 0;
-  double xi[3] = {0.0, 0.0, 0.0}, ui[3] = {0.0, 0.0, 0.0};
+  unsigned xi[3] = {0u, 0u, 0u};
+  double ui[3] = {0.0, 0.0, 0.0};
   NodeConst c{0.0f, 0.0f, 0.0f, 1.0f};
   if (valid) {
     rs = a.row[node];
     deg = a.row[node + 1] - rs;
     w0 = a.mw[node];
     const NodeRec me = rec[node];
-    xi[0] = me.x[0]; xi[1] = me.x[1]; xi[2] = me.x[2];
+    xi[0] = me.xq[0]; xi[1] = me.xq[1]; xi[2] = me.xq[2];
     ui[0] = me.u[0]; ui[1] = me.u[1]; ui[2] = me.u[2];
     c = a.nc[node];
   }
@@ -74,8 +76,8 @@ __global__ __launch_bounds__(256) void k_pass1(BondArgs a, NodeRec* __restrict__
       const NodeRec r = rec[j];
       wold = a.mask[w0 + (k >> 6)];
       brk = ((wold >> (k & 63)) & 1ull) != 0ull;
-      const BondGeom b = bond_geom<false>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.x[0],
-                                           r.x[1], r.x[2], r.u[0], r.u[1], r.u[2]);
+      const BondGeom b = bond_geom_q<false>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
+                                            r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]);
       if (DAMAGE) {
         // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0)
         const float lim = c.sc * b.zeta;
@@ -125,14 +127,15 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
   const bool valid = node < a.n;
 
   int rs = 0, deg = 0, w0 = 0;
-  double xi[3] = {0.0, 0.0, 0.0}, ui[3] = {0.0, 0.0, 0.0};
+  unsigned xi[3] = {0u, 0u, 0u};
+  double ui[3] = {0.0, 0.0, 0.0};
   float alpha_i = 0.0f, p_i = 0.0f, hhat = 1.0f;
   if (valid) {
     rs = a.row[node];
     deg = a.row[node + 1] - rs;
     w0 = a.mw[node];
     const NodeRec me = rec[node];
-    xi[0] = me.x[0]; xi[1] = me.x[1]; xi[2] = me.x[2];
+    xi[0] = me.xq[0]; xi[1] = me.xq[1]; xi[2] = me.xq[2];
     ui[0] = me.u[0]; ui[1] = me.u[1]; ui[2] = me.u[2];
     alpha_i = me.alpha;
     p_i = me.p;
@@ -144,8 +147,8 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
     const NodeRec r = rec[j];
     const unsigned long long word = a.mask[w0 + (k >> 6)];
     const bool brk = ((word >> (k & 63)) & 1ull) != 0ull;
-    const BondGeom b = bond_geom<true>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.x[0], r.x[1],
-                                       r.x[2], r.u[0], r.u[1], r.u[2]);
+    const BondGeom b = bond_geom_q<true>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
+                                         r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]);
     const float w = omega_hat(a.influence, b.zeta, hhat);
     float t = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
     t = brk ? 0.0f : w * t * r.vol * b.ry;
@@ -157,7 +160,8 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
   f1 = group_sum<G>(f1);
   f2 = group_sum<G>(f2);
   if (!valid || l >= 3) return;
-  const float fc = l == 0 ? f0 : (l == 1 ? f1 : f2);
+  // the sums are in units of 1/q: back to force density (exact scaling)
+  const float fc = (l == 0 ? f0 : (l == 1 ? f1 : f2)) * a.fq;
   if (!VERLET) {
     fout[3 * node + l] = (double)fc;
     return;
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(256) void k_verlet_prologue(int n, const NodeRec* _
   const int node = q / 3, l = q - 3 * (q / 3);
   const double dt2 = va.dt / 2.0;
   double v = va.v[q];
-  double u = rec[node].u[l] * va.L;
+  double u = rec[node].u[l] * va.uq;
   v = v + dt2 * va.a[q];
   const int slot = va.bc_slot[node];
   if (slot >= 0 && ((va.bcs[slot].mask >> l) & 1)) {
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(256) void k_verlet_prologue(int n, const NodeRec* _
   }
   u = u + va.dt * v;
   va.v[q] = v;
-  rec_next[node].u[l] = u * va.inv_L;
+  rec_next[node].u[l] = u * va.inv_uq;
 }
 
 // Weighted volume in the reference's own summation order (one thread per

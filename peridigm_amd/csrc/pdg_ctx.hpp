@@ -39,6 +39,10 @@ struct Ctx {
 
   // constants
   double K = 0, mu = 0, horizon = 0, sc = 0, L = 1, S = 1;
+  // position quantum of the bond kernels: q = L * 2^-qexp (records hold
+  // u / q and the fixed-point x / q)
+  double q = 1;
+  int qexp = 0;
   int infl = 0;
   std::vector<double> x, vol, m;  // reference order
 
@@ -63,6 +67,8 @@ struct Ctx {
   int* d_err = nullptr;
   unsigned long long* d_near = nullptr;
   int* d_perm = nullptr;
+  double* d_vol = nullptr;  // FP64 volumes (internal order)
+  double* d_x = nullptr;    // FP64 reference positions (internal order, xyz)
 
   // LDS-staged Hilbert tiles (empty if a node's neighbourhood exceeds the
   // halo capacity; the untiled kernels are used then)

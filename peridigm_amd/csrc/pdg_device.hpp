@@ -10,7 +10,8 @@ __device__ __forceinline__ float fsqrt(float v) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
 
-// Influence function in units of L (src/core/Peridigm_InfluenceFunction.hpp:69-88).
+// Influence function (src/core/Peridigm_InfluenceFunction.hpp:69-88) of a
+// bond length and horizon given in the same unit.
 __device__ __forceinline__ float omega_hat(int kind, float zeta, float hhat) {
   if (kind == PDG_INFLUENCE_ONE) return 1.0f;
   if (kind == PDG_INFLUENCE_PARABOLIC_DECAY) {
@@ -78,41 +79,49 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 struct BondGeom {
-  float x0, x1, x2;  // xi (units of L)
+  float x0, x1, x2;  // xi
   float y0, y1, y2;  // Y = xi + eta
   float zeta, ext;   // |xi|, |Y| - |xi|
   float ry;          // 1/|Y| (when requested)
 };
 
-// Compensated bond geometry from FP64 reference positions and displacements:
-// xi and eta are formed in FP64 and rounded once; the extension uses
-// e = eta . (xi + Y) / (|Y| + |xi|), which has no cancellation.
+// Compensated bond geometry, lengths in units of the position quantum q.
+// xi is the exact integer difference of the neighbours' 32-bit fixed-point
+// positions (taken modulo 2^32; every bond component is below 2^31 q), so
+// xi_ij = -xi_ji exactly; eta = u_j - u_i is formed in FP64 from the records
+// and rounded once. Both are rounded to FP32 once, Y = xi + eta, and the
+// extension uses e = eta . (xi + Y) / (|Y| + |xi|), which has no
+// cancellation.
 template <bool NEED_RY>
-__device__ __forceinline__ BondGeom bond_geom(double xi0, double xi1, double xi2, double ui0,
-                                              double ui1, double ui2, double xj0, double xj1,
-                                              double xj2, double uj0, double uj1, double uj2) {
+__device__ __forceinline__ BondGeom bond_geom_q(unsigned xi0, unsigned xi1, unsigned xi2,
+                                                double ui0, double ui1, double ui2, unsigned xj0,
+                                                unsigned xj1, unsigned xj2, double uj0, double uj1,
+                                                double uj2) {
   BondGeom b;
-  b.x0 = (float)(xj0 - xi0);
-  b.x1 = (float)(xj1 - xi1);
-  b.x2 = (float)(xj2 - xi2);
+  b.x0 = (float)(int)(xj0 - xi0);
+  b.x1 = (float)(int)(xj1 - xi1);
+  b.x2 = (float)(int)(xj2 - xi2);
   const float e0 = (float)(uj0 - ui0);
   const float e1 = (float)(uj1 - ui1);
   const float e2 = (float)(uj2 - ui2);
   const float z2 = fmaf(b.x2, b.x2, fmaf(b.x1, b.x1, b.x0 * b.x0));
   b.zeta = fsqrt(z2);
-  // |Y|^2 = |xi|^2 + 2 xi.eta + eta.eta and eta.(xi + Y) = 2 xi.eta + eta.eta
-  // share the two small dot products
-  const float xe = fmaf(e2, b.x2, fmaf(e1, b.x1, e0 * b.x0));
-  const float ee = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
-  const float dot = fmaf(2.0f, xe, ee);
-  const float yy = z2 + dot;
   if (NEED_RY) {
+    // |Y|^2 - |xi|^2 = eta . (xi + Y)
     b.y0 = b.x0 + e0;
     b.y1 = b.x1 + e1;
     b.y2 = b.x2 + e2;
+    const float dot = fmaf(e2, b.x2 + b.y2, fmaf(e1, b.x1 + b.y1, e0 * (b.x0 + b.y0)));
+    const float yy = z2 + dot;
     b.ry = frsq(yy);
     b.ext = dot * frcp(yy * b.ry + b.zeta);
   } else {
+    // |Y|^2 = |xi|^2 + 2 xi.eta + eta.eta and eta.(xi + Y) = 2 xi.eta + eta.eta
+    // share the two small dot products
+    const float xe = fmaf(e2, b.x2, fmaf(e1, b.x1, e0 * b.x0));
+    const float ee = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+    const float dot = fmaf(2.0f, xe, ee);
+    const float yy = z2 + dot;
     b.y0 = b.y1 = b.y2 = 0.0f;
     b.ry = 0.0f;
     b.ext = dot * frcp(fsqrt(yy) + b.zeta);
@@ -138,7 +147,7 @@ __device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, 
 // src/core/Peridigm.cpp:1480-1501, :1569-1577): finishes the current step
 // (v += dt/2 a) and, unless this is the last force evaluation, starts the next
 // one (v += dt/2 a, prescribed-displacement velocity, u += dt v). u_scaled is
-// the node's displacement / L from the current record; the new one goes to
+// the node's displacement / q from the current record; the new one goes to
 // rec_next. `pre` holds the component's inputs from verlet_pre.
 __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, int l, float fc,
                                                 double u_scaled, NodeRec* __restrict__ rec_next,
@@ -150,7 +159,7 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
   acc += 0.0;  // external force (none)
   acc /= pre.rho;
   double v = pre.v;
-  double u = u_scaled * va.L;
+  double u = u_scaled * va.uq;
   const double dt2 = va.dt / 2.0;
   if (va.end_kick) v = v + dt2 * acc;
   if (va.start_step) {
@@ -169,7 +178,7 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
     va.f[q] = f;
   }
   va.v[q] = v;
-  rec_next[node].u[l] = u * va.inv_L;
+  rec_next[node].u[l] = u * va.inv_uq;
 }
 
 __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, int l, float fc,

@@ -14,16 +14,22 @@ namespace pdg {
 // One node's bond-gather record, 64 bytes (half a 128-byte cache line):
 // everything a neighbour needs from node j in either bond pass, so a gather
 // touches exactly one aligned 64-byte segment.
-//   x: reference position / L (FP64), u: displacement / L (FP64),
+//   u: displacement / q (FP64), q = L * 2^-qexp the context's position
+//      quantum (a power of two, so the scaling is exact);
+//   xq: the low 32 bits of round(x / q), the reference position in fixed
+//      point: for any bond the difference xq_j - xq_i, taken modulo 2^32, is
+//      the exact integer (x_j - x_i) / q of the rounded positions, because
+//      every bond component is shorter than 2^31 q; the same difference with
+//      i and j swapped is its exact negative;
 //   vol: V / S, alpha: 15 mu / m * L * S, p: theta * (3K/m - alpha/3) * L * S.
 // L and S are powers of two, so the scalings are exact.
 struct __align__(64) NodeRec {
   double u[3];  // first: the per-step displacement write fills one 32-byte sector
-  double x[3];
+  unsigned xq[3];
   float vol;
   float alpha;
   float p;
-  float pad;
+  float pad[4];
 };
 static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
 
@@ -31,11 +37,14 @@ static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
 //   ctheta = 3 L^2 S / m      (theta = ctheta * sum_j w (1-d) zeta e V)
 //   kp     = ctheta * (3K/m - alpha/3) * L * S   (p = kp * same sum)
 //   sc     = critical stretch (0: damage disabled for this node)
+// The bond kernels measure lengths in units of q (= L * 2^-qexp): their sums
+// are 2^(2 qexp) times those in units of L, so ctheta and kp carry the factor
+// 2^(-2 qexp).
 struct __align__(16) NodeConst {
   float ctheta;
   float kp;
   float sc;
-  float hhat;  // horizon / L (influence functions)
+  float hhat;  // horizon / q (influence functions)
 };
 
 // Boundary-condition slot of a node: u_c(t) = c1[c] * t + c0[c] for the
@@ -58,6 +67,7 @@ struct BondArgs {
   float near_rel;              // near-critical band (relative)
   unsigned long long* near_ctr;
   int influence;               // PDG_INFLUENCE_*
+  float fq;                    // 2^-qexp: pass-2 FP32 force sums are in units of 1/q
 };
 
 struct VerletArgs {
@@ -67,9 +77,10 @@ struct VerletArgs {
   double* f;          // 3n, FP64 force density (written on the last step only)
   const int* bc_slot; // n, -1 if none
   const BcSlot* bcs;
-  double dt, t_cur, t_prev, L;
+  double dt, t_cur, t_prev;
+  double uq;          // record displacement unit q (u = rec.u * uq)
+  double inv_uq;      // 1.0 / uq (exact: a power of two)
   double inv_dtbc;    // 1.0 / (t_cur - t_prev), rounded once like the reference's division
-  double inv_L;       // 1.0 / L
   int end_kick;       // finish the current step: v += dt/2 a
   int start_step;     // begin the next step: v += dt/2 a, BC, u += dt v
   int* err;           // set to 1 on a non-finite force
@@ -77,11 +88,12 @@ struct VerletArgs {
 
 // Hilbert tiles of the LDS-staged kernels: a tile is a contiguous range of
 // internal nodes [tile_start[t], tile_start[t+1]); its halo (sorted internal
-// ids, own nodes included) is staged in LDS; lcol[b] is 8 times the position
-// of bond b's neighbour inside its tile's halo (its byte offset in the 8-byte
-// LDS arrays). Tiling needs every node degree <= 128.
+// ids, own nodes included) is staged in LDS; lcol[b] is kLdsSlot times the
+// position of bond b's neighbour inside its tile's halo (its byte offset in
+// the 16-byte-slot LDS arrays). Tiling needs every node degree <= 128.
 constexpr int kTileHaloMax = 800;
-static_assert(8 * kTileHaloMax <= 65536, "halo byte offsets must fit 16 bits");
+constexpr int kLdsSlot = 16;  // bytes per halo slot in each LDS array
+static_assert(kLdsSlot * kTileHaloMax <= 65536, "halo byte offsets must fit 16 bits");
 constexpr int kTileNodesMax = 128;
 constexpr int kTileThreads = 512;  // threads per tiled-pass workgroup
 constexpr int kMaskPad = 2;    // >= mask words per node in the tiled kernels (degree <= 128)

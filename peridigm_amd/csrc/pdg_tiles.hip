@@ -2,13 +2,14 @@
 //
 // One workgroup (8 waves) owns a tile: a contiguous range of at most 128
 // Hilbert-ordered nodes (32 or 64 on small meshes), ordered by degree inside
-// the tile. It first stages the FP64 reference positions, FP64
+// the tile. It first stages the fixed-point reference positions, FP64
 // displacements and per-node scalars of the tile's halo (the union of the
 // tile's nodes and all their neighbours, at most 800 nodes) in LDS with one
 // 64-byte record load per halo node, then runs group-of-G-lanes bond loops
-// that read neighbours from LDS through 16-bit local indices. Per bond, HBM
-// and L2 see a 2-byte index and one bit of the damage mask instead of a
-// 4-byte index plus a 64-byte record gather.
+// that read neighbours from LDS through 16-bit local indices, three 16-byte
+// LDS reads per neighbour. Per bond, HBM and L2 see a 2-byte index and one
+// bit of the damage mask instead of a 4-byte index plus a 64-byte record
+// gather.
 //
 // A node's damage-mask words (at most 2, degree <= 128) are held in registers
 // for the whole bond loop, the lane's 16-bit neighbour indices are prefetched
@@ -42,8 +43,8 @@ constexpr int HMAX = kTileHaloMax;
 constexpr int WG = kTileThreads;
 // Minimum waves per SIMD the register allocation must allow (the second
 // __launch_bounds__ argument): 4, i.e. two 512-thread workgroups per CU, so at
-// most 128 VGPRs. The 52.7 KB of LDS per workgroup (halo capacity 800) would
-// admit three, but that needs <= 80 VGPRs, which spills and measured slower.
+// most 128 VGPRs. The 39.9 KB of LDS per workgroup (halo capacity 800) would
+// admit four, but three already need <= 80 VGPRs, which spills.
 constexpr int WAVES_PER_SIMD = 4;
 constexpr int MAXDEG = 128;
 static_assert(3 * kTileNodesMax <= WG, "one Verlet component per thread");
@@ -72,10 +73,16 @@ __device__ __forceinline__ float omega_t(float zeta, float hhat) {
   return omega_hat(INFL, zeta, hhat);
 }
 
-// Loads a node's mask words into registers. The mask array is padded by
-// NWORDS words and the index array by MAXDEG entries, so the loads past a
-// row's end stay in bounds and need no branch. Words past the row are
-// replaced by 0.
+// Loads a node's mask words into registers, with the bits of the bonds past
+// the row's end set: such "bonds" read as broken, so the loops need no
+// separate existence test. The mask array is padded by NWORDS words and the
+// index array by MAXDEG entries, so the loads past a row's end stay in bounds
+// and need no branch.
+__device__ __forceinline__ unsigned long long valid_bits(int deg, int w) {
+  const int r = deg - 64 * w;
+  return r >= 64 ? ~0ull : (r <= 0 ? 0ull : ((1ull << r) - 1ull));
+}
+
 __device__ __forceinline__ void load_words(const unsigned long long* __restrict__ mask, int w0,
                                            int deg, unsigned long long (&m)[NWORDS]) {
   const int nw = (deg + 63) >> 6;
@@ -83,16 +90,16 @@ __device__ __forceinline__ void load_words(const unsigned long long* __restrict_
 #pragma unroll
   for (int w = 0; w < NWORDS; w++) {
     const unsigned long long v = p[w];
-    m[w] = w < nw ? v : 0ull;
+    m[w] = (w < nw ? v : 0ull) | ~valid_bits(deg, w);
   }
 }
 
 // Prefetches a lane's local neighbour indices for the whole row: one base
 // address, constant offsets. Entries past the row's end belong to later rows
-// (or the zero padding), so they are byte offsets below 8 * HMAX; the bonds
-// they stand for are masked out, and the LDS slots they address always hold
-// finite values (staged or zero-filled), so those bonds' arithmetic stays
-// finite.
+// (or the zero padding), so they are byte offsets below kLdsSlot * HMAX; the
+// bonds they stand for are masked out, and the LDS slots they address always
+// hold finite values (staged or zero-filled), so those bonds' arithmetic
+// stays finite.
 template <int G>
 __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lcol, int rs, int l,
                                           int (&lj)[MAXDEG / G]) {
@@ -100,47 +107,77 @@ __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lco
 #pragma unroll
   for (int it = 0; it < MAXDEG / G; it++) lj[it] = (int)p[it * G];
 }
-// LDS staging arrays of one workgroup (shared by both passes).
+
+// LDS staging arrays of one workgroup (shared by both passes), 16-byte slots
+// so one byte offset (16 x the halo position) addresses every array and a
+// neighbour takes three 16-byte reads:
+//   sa: fixed-point position and volume, sb: displacement x and y (units of
+//   q), sc: displacement z, alpha and p.
+struct __align__(16) SlotC {
+  double u2;
+  float alpha, p;
+};
 struct TileLds {
-  double sx[3][HMAX];
-  double su[3][HMAX];
-  float2 sva[HMAX];  // (volume, alpha)
-  float2 spx[HMAX];  // (p, unused): 8-byte slots, so one byte offset j * 8
-                     // addresses every array
+  SlotC sc[HMAX];    // (u2, alpha, p); first, so its reads need no address add
+  uint4 sa[HMAX];    // (xq0, xq1, xq2, bits of V/S)
+  double2 sb[HMAX];  // (u0, u1)
   float sf[3 * kTileNodesMax];  // pass 2: the tile's forces, for the Verlet epilogue
 };
+static_assert(sizeof(SlotC) == kLdsSlot && sizeof(uint4) == kLdsSlot && sizeof(double2) == kLdsSlot,
+              "16-byte LDS slots");
 
 // One neighbour's staged data, loaded one bond ahead of its use by the
 // pipelined bond loops so LDS latency overlaps the previous bond's math.
-template <bool FORCE>
 struct Nb {
-  double x0, x1, x2, u0, u1, u2;
-  float vol, alpha, p;
+  unsigned x0, x1, x2;
+  float vol;
+  double u0, u1, u2;
+  float alpha, p;
 };
 
-// jb is the neighbour's byte offset in the 8-byte-slot arrays (local index
-// times 8, as stored in the tile index array).
+// jb is the neighbour's byte offset in the slot arrays (local index times
+// kLdsSlot, as stored in the tile index array).
 template <typename T>
 __device__ __forceinline__ T lds_at(const void* arr, int jb) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(arr) + jb);
 }
 
 template <bool FORCE>
-__device__ __forceinline__ Nb<FORCE> nb_load(const TileLds& L, int jb) {
-  Nb<FORCE> n;
-  n.x0 = lds_at<double>(L.sx[0], jb); n.x1 = lds_at<double>(L.sx[1], jb);
-  n.x2 = lds_at<double>(L.sx[2], jb);
-  n.u0 = lds_at<double>(L.su[0], jb); n.u1 = lds_at<double>(L.su[1], jb);
-  n.u2 = lds_at<double>(L.su[2], jb);
+__device__ __forceinline__ Nb nb_load(const TileLds& L, int jb) {
+  Nb n;
+  const uint4 a = lds_at<uint4>(L.sa, jb);
+  const double2 b = lds_at<double2>(L.sb, jb);
+  n.x0 = a.x; n.x1 = a.y; n.x2 = a.z; n.vol = __uint_as_float(a.w);
+  n.u0 = b.x; n.u1 = b.y;
   if (FORCE) {
-    const float2 va = lds_at<float2>(L.sva, jb);
-    n.vol = va.x; n.alpha = va.y;
-    n.p = lds_at<float>(L.spx, jb);
+    // one 16-byte read: (u2, the bits of alpha and p)
+    const double2 c = lds_at<double2>(L.sc, jb);
+    n.u2 = c.x;
+    n.alpha = __int_as_float(__double2loint(c.y));
+    n.p = __int_as_float(__double2hiint(c.y));
   } else {
-    n.vol = lds_at<float>(L.sva, jb);
+    n.u2 = lds_at<double>(L.sc, jb);
     n.alpha = 0.0f; n.p = 0.0f;
   }
   return n;
+}
+
+// Stages halo slot h from the node record r (zeros when r is null).
+__device__ __forceinline__ void stage_slot(TileLds& L, int h, const NodeRec* r) {
+  if (r) {
+    L.sa[h] = make_uint4(r->xq[0], r->xq[1], r->xq[2], __float_as_uint(r->vol));
+    L.sb[h] = make_double2(r->u[0], r->u[1]);
+    L.sc[h] = SlotC{r->u[2], r->alpha, r->p};
+  } else {
+    L.sa[h] = make_uint4(0u, 0u, 0u, 0u);
+    L.sb[h] = make_double2(0.0, 0.0);
+    L.sc[h] = SlotC{0.0, 0.0f, 0.0f};
+  }
+}
+
+// Component l of the displacement staged in slot h.
+__device__ __forceinline__ double slot_u(const TileLds& L, int h, int l) {
+  return l == 0 ? L.sb[h].x : (l == 1 ? L.sb[h].y : L.sc[h].u2);
 }
 
 // Break bit of bond k = it * G + l of a node (it a compile-time constant):
@@ -154,11 +191,12 @@ __device__ __forceinline__ bool mask_bit(const unsigned long long (&m)[NWORDS], 
   return ((half >> ((base & 31) + l)) & 1u) != 0u;
 }
 
-// A node group's row data: CSR range, mask words, local indices, constants.
+// A node group's row data: CSR range, mask words (bits past the row set),
+// local indices, constants.
 template <int G>
 struct RowData {
   int rs, deg, w0;
-  unsigned long long mo[NWORDS];
+  unsigned long long m[NWORDS];
   int lj[MAXDEG / G];
   NodeConst c;
 };
@@ -176,18 +214,19 @@ __device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, i
     r.w0 = a.mw[node];
     r.c = a.nc[node];
   }
-  load_words(a.mask, r.w0, r.deg, r.mo);
+  load_words(a.mask, r.w0, r.deg, r.m);
   load_lcol<G>(t.lcol, r.rs, l, r.lj);
 }
 
 // Lane masks of the wave (compare results kept in scalar registers):
 // predicate codes of the compare builtins.
-enum { CMP_FOGT = 2, CMP_FOLE = 5, CMP_IEQ = 32, CMP_ISLT = 40 };
+enum { CMP_FOGT = 2, CMP_FOLE = 5, CMP_IEQ = 32 };
 
-// Lanes whose bond k = it * G + l is not marked broken in m.
+// Lanes whose bond k = it * G + l exists and is not marked broken in m (the
+// words carry set bits past the row's end).
 template <int G>
-__device__ __forceinline__ unsigned long long unbroken_lanes(const unsigned long long (&m)[NWORDS],
-                                                             int it, int l) {
+__device__ __forceinline__ unsigned long long intact_lanes(const unsigned long long (&m)[NWORDS],
+                                                           int it, int l) {
   const int base = it * G;
   unsigned v;
   if (G == 64) {
@@ -206,8 +245,6 @@ template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
 __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
                                            NodeRec* __restrict__ rec, float* __restrict__ theta,
                                            int* __restrict__ nbroken, int tile, TileLds& L) {
-  double (&sx)[3][HMAX] = L.sx;
-  double (&su)[3][HMAX] = L.su;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
   constexpr int NG = WG / G;
@@ -219,18 +256,8 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   RowData<G> rd;
   row_load<G>(a, t, n0 + grp, n0 + grp < n1, l, rd);
   // slots past the halo are zero-filled: masked-out bonds may address them
-  for (int h = threadIdx.x; h < HMAX; h += WG) {
-    if (h < hn) {
-      const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
-      sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
-      su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
-      L.sva[h].x = r.vol;
-    } else {
-      sx[0][h] = 0.0; sx[1][h] = 0.0; sx[2][h] = 0.0;
-      su[0][h] = 0.0; su[1][h] = 0.0; su[2][h] = 0.0;
-      L.sva[h].x = 0.0f;
-    }
-  }
+  for (int h = threadIdx.x; h < HMAX; h += WG)
+    stage_slot(L, h, h < hn ? &rec[(unsigned)t.halo[(unsigned)(h0 + h)]] : nullptr);
   __syncthreads();
   const int own = t.own_off[tile];
   for (int base = n0; base < n1; base += NG) {
@@ -238,44 +265,44 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     const bool valid = node < n1;
     if (base != n0) row_load<G>(a, t, node, valid, l, rd);
     const int deg = rd.deg, w0 = rd.w0;
-    const unsigned long long (&mo)[NWORDS] = rd.mo;
+    const unsigned long long (&mo)[NWORDS] = rd.m;
     const int (&lj)[MAXDEG / G] = rd.lj;
     const NodeConst c = rd.c;
-    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
+    unsigned xi0 = 0u, xi1 = 0u, xi2 = 0u;
+    double ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
     if (valid) {
       const int li = own + (node - n0);
-      xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
-      ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
+      const uint4 sa = L.sa[li];
+      xi0 = sa.x; xi1 = sa.y; xi2 = sa.z;
+      ui0 = L.sb[li].x; ui1 = L.sb[li].y; ui2 = L.sc[li].u2;
     }
     unsigned long long mn[NWORDS];
 #pragma unroll
     for (int w = 0; w < NWORDS; w++) mn[w] = 0ull;
     const float scz = c.sc;
     const int nmax = groups_max<G>((deg + G - 1) / G);
-    const int nact = deg - l;  // bond it * G + l exists iff it * G < nact
     float acc = 0.0f;
     int nnear = 0;               // plain loops: per lane
     unsigned nnear_w = 0;        // pipelined loops: per wave (scalar)
     if (PIPE) {
-      // branch-free bond loop: every lane computes, inactive and broken
-      // bonds are selected away; the next bond's LDS data is in flight. The
+      // branch-free bond loop: every lane computes, absent and broken bonds
+      // are selected away; the next bond's LDS data is in flight. The
       // per-iteration ballots of new breaks stay wave-uniform (scalar
       // registers) and are merged into the group's mask words after the loop.
       unsigned long long balv[MAXDEG / G];
-      Nb<false> nb[2];
+      Nb nb[2];
       nb[0] = nb_load<false>(L, lj[0]);
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
           if (it + 1 < MAXDEG / G && it + 1 < nmax)
             nb[(it + 1) & 1] = nb_load<false>(L, lj[(it + 1) % (MAXDEG / G)]);
-          const Nb<false>& cur = nb[it & 1];
+          const Nb& cur = nb[it & 1];
           // lanes whose bond exists and is unbroken; the masks stay in scalar
           // registers, so combining them costs no vector instructions
-          const unsigned long long okm =
-              __builtin_amdgcn_sicmp(it * G, nact, CMP_ISLT) & unbroken_lanes<G>(mo, it, l);
-          const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
-                                               cur.x2, cur.u0, cur.u1, cur.u2);
+          const unsigned long long okm = intact_lanes<G>(mo, it, l);
+          const BondGeom b = bond_geom_q<false>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
+                                                cur.x2, cur.u0, cur.u1, cur.u2);
           unsigned long long brkm = 0ull;  // lanes whose bond breaks now
           if (DAMAGE) {
             const float lim = scz * b.zeta;
@@ -303,8 +330,6 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
               G == 64 ? balv[it] : ((balv[it] >> (g * G)) & ((1ull << (G & 63)) - 1ull));
           mn[(it * G) >> 6] |= gb << ((it * G) & 63);
         }
-#pragma unroll
-        for (int w = 0; w < NWORDS; w++) mn[w] |= mo[w];
       }
     } else {
 #pragma unroll
@@ -314,9 +339,9 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
         const bool act = k < deg;
         bool brk = act && mask_bit<G>(mo, it, l);
         if (act) {
-          const int j = lj[it] >> 3;
-          const BondGeom b = bond_geom<false>(xi0, xi1, xi2, ui0, ui1, ui2, sx[0][j], sx[1][j],
-                                               sx[2][j], su[0][j], su[1][j], su[2][j]);
+          const Nb r = nb_load<false>(L, lj[it]);
+          const BondGeom b = bond_geom_q<false>(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2,
+                                                r.u0, r.u1, r.u2);
           if (DAMAGE) {
             // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0)
             const float lim = scz * b.zeta;
@@ -324,7 +349,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
             brk = brk || (b.ext > lim);
           }
           const float w = omega_t<INFL>(b.zeta, c.hhat);
-          if (!brk) acc = fmaf(w * b.zeta * b.ext, L.sva[j].x, acc);
+          if (!brk) acc = fmaf(w * b.zeta * b.ext, r.vol, acc);
         }
         if (DAMAGE) {
           const unsigned long long bal = __ballot(brk);
@@ -345,8 +370,11 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
         int nb = 0;
 #pragma unroll
         for (int w = 0; w < NWORDS; w++) {
-          if (w < nw && mn[w] != mo[w]) a.mask[(unsigned)(w0 + w)] = mn[w];
-          nb += __popcll(mn[w]);
+          const unsigned long long vb = valid_bits(deg, w);
+          const unsigned long long old = mo[w] & vb;
+          const unsigned long long nw_ = (mo[w] | mn[w]) & vb;
+          if (w < nw && nw_ != old) a.mask[(unsigned)(w0 + w)] = nw_;
+          nb += __popcll(nw_);
         }
         if (nbroken) nbroken[node] = nb;
       }
@@ -376,8 +404,6 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
                                            NodeRec* __restrict__ rec_next,
                                            double* __restrict__ fout, const VerletArgs& va,
                                            int tile, TileLds& L) {
-  double (&sx)[3][HMAX] = L.sx;
-  double (&su)[3][HMAX] = L.su;
   const int n0 = t.tile_start[tile], n1 = t.tile_start[tile + 1];
   const int h0 = t.halo_off[tile], hn = min(t.halo_off[tile + 1] - h0, HMAX);
   constexpr int NG = WG / G;
@@ -393,20 +419,8 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   VerletPre vp{0.0, 0.0, -1};
   if (VERLET && has_q) vp = verlet_pre(va, n0 + q / 3, q - 3 * (q / 3));
   // slots past the halo are zero-filled: masked-out bonds may address them
-  for (int h = threadIdx.x; h < HMAX; h += WG) {
-    if (h < hn) {
-      const NodeRec r = rec[(unsigned)t.halo[(unsigned)(h0 + h)]];
-      sx[0][h] = r.x[0]; sx[1][h] = r.x[1]; sx[2][h] = r.x[2];
-      su[0][h] = r.u[0]; su[1][h] = r.u[1]; su[2][h] = r.u[2];
-      L.sva[h] = make_float2(r.vol, r.alpha);
-      L.spx[h].x = r.p;
-    } else {
-      sx[0][h] = 0.0; sx[1][h] = 0.0; sx[2][h] = 0.0;
-      su[0][h] = 0.0; su[1][h] = 0.0; su[2][h] = 0.0;
-      L.sva[h] = make_float2(0.0f, 0.0f);
-      L.spx[h].x = 0.0f;
-    }
-  }
+  for (int h = threadIdx.x; h < HMAX; h += WG)
+    stage_slot(L, h, h < hn ? &rec[(unsigned)t.halo[(unsigned)(h0 + h)]] : nullptr);
   __syncthreads();
   const int own = t.own_off[tile];
   for (int base = n0; base < n1; base += NG) {
@@ -414,37 +428,38 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     const bool valid = node < n1;
     if (base != n0) row_load<G>(a, t, node, valid, l, rd);
     const int deg = rd.deg;
-    const unsigned long long (&mo)[NWORDS] = rd.mo;
+    const unsigned long long (&mo)[NWORDS] = rd.m;
     const int (&lj)[MAXDEG / G] = rd.lj;
     const float hhat = rd.c.hhat;
-    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
+    unsigned xi0 = 0u, xi1 = 0u, xi2 = 0u;
+    double ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
     float alpha_i = 0.0f, p_i = 0.0f;
     if (valid) {
       const int li = own + (node - n0);
-      xi0 = sx[0][li]; xi1 = sx[1][li]; xi2 = sx[2][li];
-      ui0 = su[0][li]; ui1 = su[1][li]; ui2 = su[2][li];
-      alpha_i = L.sva[li].y;
-      p_i = L.spx[li].x;
+      const uint4 sa = L.sa[li];
+      xi0 = sa.x; xi1 = sa.y; xi2 = sa.z;
+      ui0 = L.sb[li].x; ui1 = L.sb[li].y;
+      const SlotC sc = L.sc[li];
+      ui2 = sc.u2; alpha_i = sc.alpha; p_i = sc.p;
     }
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
       const int nmax = groups_max<G>((deg + G - 1) / G);
-      const int nact = deg - l;  // bond it * G + l exists iff it * G < nact
-      Nb<true> nb[2];
+      Nb nb[2];
       nb[0] = nb_load<true>(L, lj[0]);
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
           if (it + 1 < MAXDEG / G && it + 1 < nmax)
             nb[(it + 1) & 1] = nb_load<true>(L, lj[(it + 1) % (MAXDEG / G)]);
-          const Nb<true>& cur = nb[it & 1];
-          const bool ok = it * G < nact && !mask_bit<G>(mo, it, l);
-          const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
-                                             cur.x2, cur.u0, cur.u1, cur.u2);
+          const Nb& cur = nb[it & 1];
+          const unsigned long long okm = intact_lanes<G>(mo, it, l);
+          const BondGeom b = bond_geom_q<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
+                                               cur.x2, cur.u0, cur.u1, cur.u2);
           const float w = omega_t<INFL>(b.zeta, hhat);
           float tt = fmaf(b.zeta, p_i + cur.p, b.ext * (alpha_i + cur.alpha));
           tt = w * tt * cur.vol * b.ry;
-          tt = ok ? tt : 0.0f;
+          tt = __builtin_amdgcn_inverse_ballot_w64(okm) ? tt : 0.0f;
           f0 = fmaf(tt, b.y0, f0);
           f1 = fmaf(tt, b.y1, f1);
           f2 = fmaf(tt, b.y2, f2);
@@ -456,12 +471,12 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
       const int k = it * G + l;
       if (k < deg) {
         const bool brk = mask_bit<G>(mo, it, l);
-        const int j = lj[it] >> 3;
-        const BondGeom b = bond_geom<true>(xi0, xi1, xi2, ui0, ui1, ui2, sx[0][j], sx[1][j],
-                                           sx[2][j], su[0][j], su[1][j], su[2][j]);
+        const Nb r = nb_load<true>(L, lj[it]);
+        const BondGeom b = bond_geom_q<true>(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2,
+                                             r.u0, r.u1, r.u2);
         const float w = omega_t<INFL>(b.zeta, hhat);
-        float tt = fmaf(b.zeta, p_i + L.spx[j].x, b.ext * (alpha_i + L.sva[j].y));
-        tt = brk ? 0.0f : w * tt * L.sva[j].x * b.ry;
+        float tt = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
+        tt = brk ? 0.0f : w * tt * r.vol * b.ry;
         f0 = fmaf(tt, b.y0, f0);
         f1 = fmaf(tt, b.y1, f1);
         f2 = fmaf(tt, b.y2, f2);
@@ -472,18 +487,19 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     f1 = group_sum<G>(f1);
     f2 = group_sum<G>(f2);
     if (valid && l < 3) {
-      const float fc = l == 0 ? f0 : (l == 1 ? f1 : f2);
+      // the sums are in units of 1/q: back to force density (exact scaling)
+      const float fc = (l == 0 ? f0 : (l == 1 ? f1 : f2)) * a.fq;
       if (!VERLET) fout[3 * node + l] = (double)fc;
       else L.sf[3 * (node - n0) + l] = fc;
     }
   }
   if (VERLET) {
     // Verlet epilogue of the whole tile at once: one component per thread
-    // instead of 3 of every 16 lanes per node round
+    // instead of 3 of every G lanes per node round
     __syncthreads();
     if (has_q) {
       const int k = q / 3, l3 = q - 3 * k;
-      verlet_epilogue(va, n0 + k, l3, L.sf[q], su[l3][own + k], rec_next, vp);
+      verlet_epilogue(va, n0 + k, l3, L.sf[q], slot_u(L, own + k, l3), rec_next, vp);
     }
   }
   __syncthreads();  // LDS is reused by the next tile
