@@ -11,11 +11,17 @@ Prints ONE JSON line: value = directed-bond evaluations per second
 (B bonds per step; each bond is evaluated from both ends, as the CSR holds
 both directions) with all inputs resident in HBM. `roofline` reports the
 dominant kernel (the force+Verlet pass) from HIP events recorded on the
-stream it runs on; `cpu_baseline` times the FP64 CPU oracle (a scalar
-restatement of the reference loops, one core) on a bounded sample.
+stream it runs on. `quasi_static` is the second half of the metric: the
+60x20x20 tensile bar solved to 1e-8 linear residuals with block-Jacobi CG
+and with the learned preconditioner. The cpu_baseline leg times the FP64 CPU
+path (the reference loops, multi-threaded, every host thread OpenMP is
+given) on a bounded sample and checks the GPU against FP64: `accuracy` holds
+the force/dilatation error at the timed run's end state and the displacement
+and velocity error of the sample's trajectory.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plate|bar|cylinder|batched]
-                    [--lanes 4|8|16|32|64] [--persistent] [--qs]
+                    [--lanes 4|8|16|32|64] [--persistent] [--no-qs] [--no-cpu-baseline]
+                    [--no-accuracy]
 """
 from __future__ import annotations
 
@@ -78,26 +84,101 @@ def build(workload):
     return p, nl
 
 
-def cpu_baseline(steps=20):
-    """FP64 oracle on a 200x200x4 sample of the plate workload (13.7M bonds),
-    one core: per-step time from a run of `steps` steps minus a run of zero
-    steps (both include the initial evaluation and setup)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(target_s=12.0):
+    """Multi-threaded FP64 CPU baseline (oracle/pd_baseline.cpp: the reference
+    loops in reference order, OpenMP over owned points, per-thread force
+    buffers) on a 200x200x4 sample of the plate workload (13.7M bonds), on
+    every thread OpenMP is given (OMP_NUM_THREADS). The step count is chosen
+    for about target_s seconds of stepping; per-step time is the run minus a
+    zero-step run (setup and initial evaluation). Returns the baseline record
+    and the sample's final state (for the trajectory error)."""
     import oracle as O
     from peridigm_amd import problems
     p = problems.plate(nx=200, ny=200, nz=4, notch_cells=20)
     nl = O.filter_planes(p["x"], O.radius_search(p["x"], p["horizon"]), p["planes"])
     B = nl.size - len(p["x"])
-    args = (p["x"], p["volume"], p["density"], nl, p["K"], p["mu"], p["horizon"],
-            p["critical_stretch"], p["dt"])
-    t0 = time.perf_counter()
-    O.explicit_run(*args, 0, bcs=p["bcs"])
-    t1 = time.perf_counter()
-    O.explicit_run(*args, steps, bcs=p["bcs"])
-    t2 = time.perf_counter()
-    per_step = ((t2 - t1) - (t1 - t0)) / steps
-    return dict(value=B / per_step, unit="bond_evals/s", cores=1, kind="port",
-                sample=f"plate 200x200x4 (N={len(p['x'])}, B={B}), {steps} FP64 Verlet steps "
-                       f"(damage, dilatation, scatter force), {per_step*1e3:.1f} ms/step")
+    m = O.weighted_volume(p["x"], p["volume"], nl, p["horizon"])
+    threads = O.baseline_threads()
+
+    def run(n):
+        t = time.perf_counter()
+        r = O.baseline_explicit_run(p["x"], p["volume"], p["density"], nl, m, p["K"], p["mu"],
+                                    p["critical_stretch"], p["dt"], n, bcs=p["bcs"])
+        return time.perf_counter() - t, r
+
+    t_zero, _ = run(0)
+    t_probe, _ = run(5)
+    per = max((t_probe - t_zero) / 5, 1e-6)
+    steps = int(min(2000, max(20, target_s / per)))
+    t_run, r = run(steps)
+    per_step = (t_run - t_zero) / steps
+    rec = dict(value=B / per_step, unit="bond_evals/s", cores=threads, kind="port",
+               cpu=_cpu_model(),
+               sample=f"plate 200x200x4 (N={len(p['x'])}, B={B}), {steps} FP64 Verlet steps "
+                      f"(damage, dilatation, scatter force with per-thread buffers) on {threads} "
+                      f"threads, {per_step*1e3:.2f} ms/step")
+    return rec, dict(p=p, nl=nl, steps=steps, state=r)
+
+
+def trajectory_error(sample):
+    """The GPU run of the CPU baseline's sample (same steps) against the
+    baseline's FP64 end state: max and RMS displacement and velocity error
+    (relative to max / RMS of the FP64 values) and bond-mask mismatches."""
+    from peridigm_amd import Context
+    p, nl, steps, r = sample["p"], sample["nl"], sample["steps"], sample["state"]
+    N = len(p["volume"])
+    with Context(0) as ctx:
+        ctx.set_topology(nl, N)
+        ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0,
+                          p["critical_stretch"])
+        ctx.set_density(np.full(N, p["density"]))
+        ctx.set_bcs(p["bcs"])
+        ctx.upload_state()
+        ctx.verlet_run(steps, p["dt"])
+        st = ctx.download_state()
+
+    def err(a, b):
+        return (float(np.abs(a - b).max() / np.abs(b).max()),
+                float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))))
+
+    eu, ev = err(st["u"], r["u"]), err(st["v"], r["v"])
+    return {"sample": f"plate 200x200x4, {steps} steps, GPU vs the FP64 CPU run",
+            "u_max": eu[0], "u_rms": eu[1], "v_max": ev[0], "v_rms": ev[1],
+            "broken_fp64": int(r["bond_damage"].sum()),
+            "mask_mismatches": int(np.count_nonzero(st["bond_damage"] != r["bond_damage"]))}
+
+
+def force_error(p, nl, st, ctx_force):
+    """Force and dilatation of the GPU drop-in call at the bench run's final
+    state (its displacements and broken bonds) against the FP64 oracle at the
+    same state: max error over max |f| and RMS error over RMS |f|."""
+    import oracle as O
+    x = p["x"]
+    y = x + st["u"]
+    m = O.weighted_volume(x, p["volume"], nl, p["horizon"])
+    th_o, f_o = O.compute_force(x, y, m, p["volume"], st["bond_damage"], nl, p["K"], p["mu"],
+                                p["horizon"])
+    th_g, f_g = ctx_force
+
+    def err(a, b):
+        return (float(np.abs(a - b).max() / np.abs(b).max()),
+                float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))))
+
+    ef, et = err(f_g, f_o), err(th_g, th_o)
+    return {"state": "end of the timed run (same displacements and broken bonds)",
+            "force_max": ef[0], "force_rms": ef[1], "dilatation_max": et[0],
+            "dilatation_rms": et[1]}
 
 
 def copy_bandwidth(mib=1024, reps=10):
@@ -172,8 +253,10 @@ def main():
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--variant", type=int, default=0, help="0: LDS-staged tiles, 1: untiled, 2: tiles with plain loops")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--qs", action="store_true",
-                    help="also measure the quasi-static solve (block-Jacobi CG vs learned FCG)")
+    ap.add_argument("--no-qs", action="store_true",
+                    help="skip the quasi-static solve figure (block-Jacobi CG vs learned FCG)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the force and trajectory errors against the FP64 CPU path")
     ap.add_argument("--persistent", action="store_true",
                     help="time the steps as one persistent cooperative launch")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -228,6 +311,7 @@ def main():
         ctx.verlet_run(args.steps, dt, t0=t_start, initial_eval=False)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    near_timed = ctx.near_critical_count()
     t_start += args.steps * dt
 
     # per-kernel durations from HIP events on the library's stream
@@ -238,6 +322,9 @@ def main():
     p1_avg = p1_ms / ksteps * 1e-3
     p2_avg = p2_ms / ksteps * 1e-3
     st = ctx.download_state()
+    fg = None
+    if not args.no_accuracy:
+        fg = ctx.compute_force(p["x"] + st["u"], st["bond_damage"])
     ctx.close()
 
     copy_gbs = copy_bandwidth()
@@ -289,10 +376,17 @@ def main():
             "copy_gbs": copy_gbs, "frac_of_copy": a2 / copy_gbs,
         },
     }
-    if args.qs:
+    out["config"]["near_critical_bonds_timed"] = near_timed
+    if not args.no_qs:
         out["quasi_static"] = qs_figure()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        # the cpu_baseline leg: the FP64 CPU path timed on the host, and the
+        # GPU results checked against it
+        rec, sample = cpu_baseline()
+        out["cpu_baseline"] = rec
+        if not args.no_accuracy:
+            out["accuracy"] = {"end_state": force_error(p, nl, st, fg),
+                               "trajectory": trajectory_error(sample)}
     print(json.dumps(out))
 
 

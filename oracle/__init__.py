@@ -228,6 +228,52 @@ def explicit_run(x, vol, rho, nlist, K, mu, horizon, sc, dt, nsteps, u0=None, v0
     return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f, m=m)
 
 
+_BLIB = None
+
+
+def baseline_lib():
+    """libbaseline.so: the multi-threaded CPU baseline (pd_baseline.cpp)."""
+    global _BLIB
+    if _BLIB is None:
+        path = os.path.join(_HERE, "libbaseline.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.bl_explicit_run.restype = C.c_int
+        L.bl_max_threads.restype = C.c_int
+        _BLIB = L
+    return _BLIB
+
+
+def baseline_explicit_run(x, vol, rho, nlist, m, K, mu, sc, dt, nsteps, u0=None, v0=None,
+                          bond_damage=None, bcs=(), t0=0.0, threads=0):
+    """Multi-threaded FP64 velocity-Verlet run ("One" influence function,
+    scalar critical stretch, weighted volume m given). threads <= 0: all the
+    threads OpenMP allows. Returns the final state like explicit_run."""
+    x = _c(x, np.float64); vol = _c(vol, np.float64); m = _c(m, np.float64)
+    N = x.shape[0]
+    rho = _c(np.broadcast_to(rho, (N,)), np.float64)
+    nlist = _c(nlist, np.int32)
+    B = num_bonds(nlist, N)
+    u = np.zeros((N, 3)) if u0 is None else _c(u0, np.float64).copy()
+    v = np.zeros((N, 3)) if v0 is None else _c(v0, np.float64).copy()
+    bd = np.zeros(B) if bond_damage is None else _c(bond_damage, np.float64).copy()
+    a = np.zeros((N, 3)); th = np.zeros(N); f = np.zeros((N, 3)); dmg = np.zeros(N)
+    ptr, nodes, coord, c0, c1 = _bc_arrays(bcs)
+    rc = baseline_lib().bl_explicit_run(
+        C.c_int(threads), C.c_int(N), _p(x), _p(vol), _p(rho), _p(nlist), _p(m), C.c_double(K),
+        C.c_double(mu), C.c_double(sc), C.c_double(dt), C.c_double(t0), C.c_int(nsteps),
+        C.c_int(len(bcs)), _p(ptr), _p(nodes), _p(coord), _p(c0), _p(c1), _p(u), _p(v), _p(a),
+        _p(bd), _p(dmg), _p(th), _p(f))
+    if rc != 0:
+        raise FloatingPointError("baseline explicit run: non-finite force")
+    return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f)
+
+
+def baseline_threads():
+    return int(baseline_lib().bl_max_threads())
+
+
 def force_at(x, u, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):
     x = _c(x, np.float64); u = _c(u, np.float64); m = _c(m, np.float64)
     vol = _c(vol, np.float64); bd = _c(bond_damage, np.float64); nlist = _c(nlist, np.int32)
