@@ -150,6 +150,15 @@ int pdg_verlet_run_persistent(pdg_ctx* ctx, int nsteps, double dt, double t0, in
  * src/core/Peridigm_CriticalTimeStep.cpp:54-126) for density rho, computed on
  * the device: min over nodes of sqrt(2 rho / sum_j V_j 18K/(pi delta^4) / |xi|). */
 int pdg_critical_time_step(pdg_ctx* ctx, double density, double* dtOut);
+/* The same time step without allocating: written to *dtOutDevice (device
+ * memory, 1 double) before the call returns, with the per-workgroup partial
+ * minima in a fixed device array of the library (128 KiB per device, kept
+ * until the process ends). Blocks of at most 4,194,304 nodes
+ * (PDG_E_UNSUPPORTED beyond). Calls that overlap on one device race on that
+ * array. No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). */
+int pdg_critical_time_step_noalloc(pdg_ctx* ctx, double density, double* dtOutDevice);
 
 /* Per-kernel timing of the last pdg_verlet_run (ms, HIP events on the
  * context's stream): total, damage+dilatation pass, force+Verlet pass. */

@@ -62,6 +62,7 @@ SIGNATURES = {
     "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),
     "pdg_verlet_run_persistent": (_i, [_vp, _i, _d, _d, _i, C.POINTER(_i)]),
     "pdg_critical_time_step": (_i, [_vp, _d, C.POINTER(_d)]),
+    "pdg_critical_time_step_noalloc": (_i, [_vp, _d, _vp]),
     "pdg_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
     "pdg_set_per_pass_timing": (_i, [_vp, _i]),
     "pdg_near_critical_count": (_i, [_vp, C.POINTER(_ll)]),

@@ -427,6 +427,13 @@ int pdg_ctx_create(int device, pdg_ctx** out) {
     return PDG_E_HIP;
   }
   for (int i = 0; i < 4; i++) (void)hipEventCreate(&c->ev[i]);
+  // this device's address of the allocation-free entry point's scratch array
+  if (dt_scratch_init(device) != hipSuccess) {
+    for (int i = 0; i < 4; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PDG_E_HIP;
+  }
   *out = reinterpret_cast<pdg_ctx*>(c);
   return PDG_OK;
 }

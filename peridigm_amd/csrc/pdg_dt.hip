@@ -9,8 +9,14 @@
 // internal one, so a sum may differ from the reference's list-order sum in
 // its last bits. Each workgroup writes the minimum of its nodes to a partial
 // array; one more workgroup reduces the partials.
+//
+// pdg_critical_time_step allocates the partial array for the call;
+// pdg_critical_time_step_noalloc uses `scratch`, a file-scope device array of
+// fixed size (one copy per device, held until the process ends), and
+// allocates nothing.
 
 #include <cmath>
+#include <mutex>
 
 #include "pdg_ctx.hpp"
 
@@ -19,6 +25,25 @@ namespace pdg {
 namespace {
 
 constexpr int kDtThreads = 256;
+
+// Partial minima of the allocation-free entry point: one per workgroup of
+// kDtThreads nodes, so it serves blocks of up to kDtScratchMax * kDtThreads
+// = 4,194,304 nodes (128 KiB per device).
+constexpr int kDtScratchMax = 16384;
+static __device__ __align__(256) double scratch[kDtScratchMax];
+
+// Device address of each device's copy of `scratch`, from
+// hipGetSymbolAddress (the array's name is not a device address in host
+// code).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+double* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch[device];
+}
 
 __device__ __forceinline__ double min_keep(double a, double b) { return b < a ? b : a; }
 
@@ -93,6 +118,18 @@ hipError_t launch_dt(const Ctx* c, double density, double* partial, double* out)
 
 }  // namespace
 
+// Called by pdg_ctx_create with `device` current, once per device.
+hipError_t dt_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[device] = static_cast<double*>(p);
+  return hipSuccess;
+}
+
 }  // namespace pdg
 
 using namespace pdg;
@@ -117,6 +154,24 @@ int pdg_critical_time_step(pdg_ctx* h, double density, double* dtOut) {
   if (rc) return rc;
   *dtOut = v;
   return PDG_OK;
+}
+
+// Same result as pdg_critical_time_step, written to device memory, with the
+// partials in this device's `scratch`: no allocation. Calls that overlap on
+// one device share the array and race.
+int pdg_critical_time_step_noalloc(pdg_ctx* h, double density, double* dtOutDevice) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !dtOutDevice) return PDG_E_ARG;
+  if (!(density > 0.0)) return c->fail(PDG_E_ARG, "density must be positive");
+  if ((long long)c->N > (long long)kDtScratchMax * kDtThreads)
+    return c->fail(PDG_E_UNSUPPORTED, "pdg_critical_time_step_noalloc: more than 4194304 nodes "
+                                      "(its scratch holds 16384 partial minima of 256 nodes)");
+  double* part = scratch_ptr(c->device);
+  if (!part) return c->fail(PDG_E_HIP, "scratch address of this device unknown");
+  (void)hipSetDevice(c->device);
+  int rc = c->hip(launch_dt(c, density, part, dtOutDevice), "critical time step");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
 }
 
 }  // extern "C"

@@ -139,4 +139,8 @@ hipError_t launch_weighted_volume(int n, const double* x, const double* vol, con
                                   const long long* head, double horizon, int influence, double* m,
                                   hipStream_t s);
 
+// Looks up this device's address of the scratch array behind the
+// allocation-free entry point (pdg_dt.hip); the device must be current.
+hipError_t dt_scratch_init(int device);
+
 }  // namespace pdg
