@@ -210,6 +210,53 @@ int pdg_qs_newton(pdg_ctx* ctx, double* u, double tol, int maxNewton, int precon
 int pdg_qs_set_absolute_tolerance(pdg_ctx* ctx, int on);
 /* The tolerance multiplier (reaction norm, or 1) of the last pdg_qs_newton. */
 int pdg_qs_tolerance_multiplier(const pdg_ctx* ctx, double* multiplier);
+/* Controls of the quasi-static load-stepping driver, named after the
+ * reference's "Solver"/"QuasiStatic" parameters (src/core/Peridigm.cpp:2483-2630);
+ * pdg_qs_params_default fills the reference's defaults. */
+typedef struct {
+  int maxSolverIterations;     /* "Maximum Solver Iterations" (10) */
+  double tolerance;            /* "Relative Tolerance" (1e-6), or the absolute one */
+  int absoluteTolerance;       /* 1: `tolerance` is "Absolute Tolerance" (no reaction scaling) */
+  int disableHeuristics;       /* "Disable Heuristics": no predictor, line search or damping */
+  double dampedNewtonDiagonalScaleFactor; /* "Damped Newton Diagonal Scale Factor" (1.0001) */
+  double dampedNewtonDiagonalShiftFactor; /* "Damped Newton Diagonal Shift Factor" (1e-5) */
+  int adaptiveLoadStepping;    /* the "Adaptive Load Stepping" sublist is present */
+  int maxLoadStepReductionsInOneStep; /* "Maximum load step reductions in one step" */
+  int maxTotalLoadStepReductions;     /* "Maximum total load step reductions" */
+  int reduceAllRemainingLoadSteps;    /* "Reduce all remaining load steps" */
+  int precond;                 /* PDG_PRECOND_* of the linear solves (the reference: none) */
+  double linearTolerance;      /* "Belos Relative Tolerance" (1e-4) */
+  int maxLinearIterations;     /* "Belos Maximum Iterations" (0: 3N) */
+} pdg_qs_params;
+
+/* One accepted load step of pdg_qs_run. */
+typedef struct {
+  double timePrevious, timeCurrent;
+  int newtonIterations;        /* solver iterations, the predictor step included */
+  int linearIterations;
+  double residualNorm;         /* ||r||_2 + 20 ||r||_inf at the end of the step */
+  double toleranceMultiplier;  /* reaction norm at the step's entry, or 1 */
+  int status;                  /* 0 converged; 1 accepted below 100x the tolerance */
+  int dampedNewton;            /* 1 if the step switched to damped Newton */
+  double lastAlpha;            /* line-search step of the last iteration */
+} pdg_qs_step;
+
+int pdg_qs_params_default(pdg_qs_params* params);
+
+/* Peridigm::executeQuasiStatic (src/core/Peridigm.cpp:2470-3043) from the
+ * displacement u (3N, in: initial, out: final): load steps at times[0..n-1];
+ * prescribed displacements u_c(t) = c1 t + c0 on the bcs' node sets applied
+ * as increments per load step (DirichletIncrementBC, :2674-2686); the
+ * reaction-scaled Newton tolerance (:2716-2732); the previous step's velocity
+ * as the first iterate (:2764-2769); damped Newton after 50 iterations or on a
+ * failed linear solve, retried without preconditioner (:2772-2822); the
+ * candidate-alpha line search (:3364-3618); acceptance below 100x the
+ * tolerance, and adaptive halving of failed load steps (:2880-2934). The
+ * tangent is applied matrix-free at the current state. Returns
+ * PDG_E_NOCONV when the run aborts (u then holds the last accepted step). */
+int pdg_qs_run(pdg_ctx* ctx, const pdg_bc* bcs, int numBcs, const double* times, int numTimes,
+               const pdg_qs_params* params, double* u, pdg_qs_step* steps, int maxSteps,
+               int* numSteps);
 /* 1: linear solves replay each CG iteration's two fixed launch sequences as
  * HIP graphs, captured once per context; 0 (default): every kernel is
  * launched directly. Same arithmetic, bitwise. */

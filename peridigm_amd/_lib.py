@@ -33,6 +33,24 @@ class pdg_bc(C.Structure):
                 ("c0", C.c_double), ("c1", C.c_double)]
 
 
+class pdg_qs_params(C.Structure):
+    _fields_ = [("maxSolverIterations", C.c_int), ("tolerance", C.c_double),
+                ("absoluteTolerance", C.c_int), ("disableHeuristics", C.c_int),
+                ("dampedNewtonDiagonalScaleFactor", C.c_double),
+                ("dampedNewtonDiagonalShiftFactor", C.c_double),
+                ("adaptiveLoadStepping", C.c_int), ("maxLoadStepReductionsInOneStep", C.c_int),
+                ("maxTotalLoadStepReductions", C.c_int), ("reduceAllRemainingLoadSteps", C.c_int),
+                ("precond", C.c_int), ("linearTolerance", C.c_double),
+                ("maxLinearIterations", C.c_int)]
+
+
+class pdg_qs_step(C.Structure):
+    _fields_ = [("timePrevious", C.c_double), ("timeCurrent", C.c_double),
+                ("newtonIterations", C.c_int), ("linearIterations", C.c_int),
+                ("residualNorm", C.c_double), ("toleranceMultiplier", C.c_double),
+                ("status", C.c_int), ("dampedNewton", C.c_int), ("lastAlpha", C.c_double)]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _ll = C.c_longlong
@@ -59,6 +77,9 @@ SIGNATURES = {
     "pdg_state_upload": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "pdg_state_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pdg_set_bcs": (_i, [_vp, C.POINTER(pdg_bc), _i]),
+    "pdg_qs_params_default": (_i, [C.POINTER(pdg_qs_params)]),
+    "pdg_qs_run": (_i, [_vp, C.POINTER(pdg_bc), _i, _vp, _i, C.POINTER(pdg_qs_params), _vp,
+                        C.POINTER(pdg_qs_step), _i, C.POINTER(_i)]),
     "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),
     "pdg_verlet_run_persistent": (_i, [_vp, _i, _d, _d, _i, C.POINTER(_i)]),
     "pdg_critical_time_step": (_i, [_vp, _d, C.POINTER(_d)]),

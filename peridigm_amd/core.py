@@ -135,8 +135,9 @@ class Context:
                                                _p(th), _p(f)))
         return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f)
 
-    def set_bcs(self, bcs):
-        """bcs: iterable of (node_ids, coord, c0, c1): u_coord(t) = c1*t + c0."""
+    def bc_array(self, bcs):
+        """ctypes pdg_bc array of (node_ids, coord, c0, c1): u_coord(t) = c1*t + c0;
+        the node arrays are kept alive on the context."""
         arr = (pdg_bc * max(1, len(bcs)))()
         self._keep = []
         for k, (ids, coord, c0, c1) in enumerate(bcs):
@@ -147,7 +148,11 @@ class Context:
             arr[k].coord = int(coord)
             arr[k].c0 = float(c0)
             arr[k].c1 = float(c1)
-        self.check(self.lib.pdg_set_bcs(self.h, arr, len(bcs)))
+        return arr
+
+    def set_bcs(self, bcs):
+        """bcs: iterable of (node_ids, coord, c0, c1): u_coord(t) = c1*t + c0."""
+        self.check(self.lib.pdg_set_bcs(self.h, self.bc_array(bcs), len(bcs)))
 
     def verlet_run(self, nsteps, dt, t0=0.0, initial_eval=True):
         self.check(self.lib.pdg_verlet_run(self.h, int(nsteps), float(dt), float(t0),

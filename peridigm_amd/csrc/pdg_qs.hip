@@ -54,6 +54,14 @@ struct QsState {
   unsigned gepoch = 0;
   int fallbacks = 0;  // learned-preconditioner solves handed over to block-Jacobi CG
   std::vector<signed char> hfixed;
+  // diagonal of K (3n, from the block-Jacobi pass) and the damped-Newton
+  // diagonal shift added to every tangent apply while `damped` is set
+  double *diag = nullptr, *dampv = nullptr;
+  bool damped = false;
+  // load-stepping driver vectors (3n each, allocated by its first run): the
+  // displacement at the load step's start, Newton increment, predictor,
+  // line-search base state and residual
+  double* rv[5] = {};
 };
 
 namespace {
@@ -102,6 +110,7 @@ struct QsArgs {
   const signed char* fixed;
   double* out;
   double* dinv;
+  double* diag;
   double horizon;
   int influence;
 };
@@ -152,7 +161,8 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
   }
 }
 
-// MODE 0: residual r = f V (0 on fixed DOFs); MODE 1: K w = -V df (w on fixed)
+// MODE 0: residual r = f V (0 on fixed DOFs); MODE 1: K w = -V df (w on fixed);
+// MODE 2: f V on every DOF (the reactions are its kinematic entries)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
   const int lane = threadIdx.x & 63;
@@ -181,7 +191,7 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
       const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
       const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
       const double vj = a.vol[j];
-      if (MODE == 0) {
+      if (MODE != 1) {
         f0 += T * n0 * vj; f1 += T * n1 * vj; f2 += T * n2 * vj;
       } else {
         const double* wi = a.w + 3 * node;
@@ -201,6 +211,7 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
   const double f = l == 0 ? f0 : (l == 1 ? f1 : f2);
   const double vi = a.vol[node];
   if (MODE == 0) a.out[q] = a.fixed[q] ? 0.0 : f * vi;
+  else if (MODE == 2) a.out[q] = f * vi;
   else a.out[q] = a.fixed[q] ? a.w[q] : -f * vi;
 }
 
@@ -252,6 +263,7 @@ __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
   for (int c = 0; c < 3; c++)
     if (a.fixed[3 * node + c])
       for (int t = 0; t < 3; t++) { K[3 * c + t] = c == t ? 1.0 : 0.0; K[3 * t + c] = c == t ? 1.0 : 0.0; }
+  for (int c = 0; c < 3; c++) a.diag[3 * node + c] = K[4 * c];
   // closed-form inverse by cofactors
   const double c00 = K[4] * K[8] - K[5] * K[7], c01 = K[5] * K[6] - K[3] * K[8], c02 = K[3] * K[7] - K[4] * K[6];
   const double det = K[0] * c00 + K[1] * c01 + K[2] * c02;
@@ -435,6 +447,77 @@ __global__ __launch_bounds__(RB) void k_sub(int n, const double* a, const double
   if (i < n) c[i] = a[i] - b[i];
 }
 
+// out += s * w (damped-Newton diagonal shift)
+__global__ __launch_bounds__(RB) void k_damp_add(int n, const double* sdiag, const double* w,
+                                                 double* out) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) out[i] += sdiag[i] * w[i];
+}
+
+// s = (f - 1) d + g: the change of the diagonal d scaled by f and shifted by g
+__global__ __launch_bounds__(RB) void k_damp_vec(int n, const double* d, double f, double g,
+                                                 double* sdiag) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) sdiag[i] = (f * d[i] + g) - d[i];
+}
+
+// per-block partial sums of squares and maxima of |v| (optionally only where
+// mask != 0): part[b] and part[nb + b]
+__global__ __launch_bounds__(RB) void k_norm_part(int n, const double* v, const signed char* mask,
+                                                  double* part) {
+  __shared__ double sh[RB / 64];
+  __shared__ double shm[RB / 64];
+  double s = 0.0, m = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+    if (mask && !mask[i]) continue;
+    const double x = v[i];
+    s += x * x;
+    m = fmax(m, fabs(x));
+  }
+  s = block_sum(s, sh);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < RB / 64; k++) m = fmax(m, shm[k]);
+    part[blockIdx.x] = s;
+    part[gridDim.x + blockIdx.x] = m;
+  }
+}
+
+// out[0] = sum of the nb partial sums, out[1] = max of the nb partial maxima
+__global__ __launch_bounds__(RB) void k_norm_final(int nb, const double* part, double* out) {
+  __shared__ double sh[RB / 64];
+  __shared__ double shm[RB / 64];
+  double s = 0.0, m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    s += part[i];
+    m = fmax(m, part[nb + i]);
+  }
+  s = block_sum(s, sh);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < RB / 64; k++) m = fmax(m, shm[k]);
+    out[0] = s;
+    out[1] = m;
+  }
+}
+
+// y = a x + b z (z may alias y)
+__global__ __launch_bounds__(RB) void k_lin2(int n, double a, const double* x, double b,
+                                             const double* z, double* y) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * z[i];
+}
+
+// v = 0 on the kinematic DOFs
+__global__ __launch_bounds__(RB) void k_zero_fixed(int n, const signed char* fixed, double* v) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n && fixed[i]) v[i] = 0.0;
+}
+
 inline unsigned qblocks(int n) { return (unsigned)((n + 4 * (64 / QG) - 1) / (4 * (64 / QG))); }
 
 }  // namespace
@@ -450,7 +533,8 @@ static QsArgs args(Ctx* c) {
   a.n = c->N; a.row = c->d_row; a.col = c->d_col; a.mw = c->d_mw; a.mask = c->d_mask;
   a.x = S->x; a.u = S->u; a.w = S->w; a.vol = S->vol; a.m = S->m; a.alpha = S->alpha;
   a.beta = S->beta; a.theta = S->theta; a.dtheta = S->dtheta; a.g = S->g; a.fixed = S->fixed;
-  a.out = S->out; a.dinv = S->dinv; a.horizon = c->horizon; a.influence = c->infl;
+  a.out = S->out; a.dinv = S->dinv; a.diag = S->diag; a.horizon = c->horizon;
+  a.influence = c->infl;
   return a;
 }
 
@@ -464,7 +548,8 @@ static int ensure(Ctx* c) {
   S->nblk = std::min(1024, std::max(1, (3 * N + RB - 1) / RB));
   auto A = [&](void** p, size_t bytes) { return c->hip(hipMalloc(p, std::max<size_t>(bytes, 64)), "qs alloc"); };
   int rc = 0;
-  double** vecs3[] = {&S->x, &S->u, &S->w, &S->g, &S->out, &S->xs, &S->r, &S->z, &S->p, &S->q, &S->b};
+  double** vecs3[] = {&S->x,  &S->u, &S->w, &S->g, &S->out, &S->xs,   &S->r,
+                      &S->z,  &S->p, &S->q, &S->b, &S->diag, &S->dampv};
   for (double** v : vecs3) if (!rc) rc = A((void**)v, 24 * (size_t)N);
   double** vecs1[] = {&S->vol, &S->m, &S->alpha, &S->beta, &S->theta, &S->dtheta};
   for (double** v : vecs1) if (!rc) rc = A((void**)v, 8 * (size_t)N);
@@ -534,6 +619,10 @@ static int tangent(Ctx* c, const double* w, double* out, bool theta_ready) {
   (void)theta_ready;  // the dual pass recomputes theta with its derivative
   hipLaunchKernelGGL((k_qs_theta<true, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_qs_force<1>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  if (c->qs->damped) {
+    const int n3 = 3 * c->N;
+    hipLaunchKernelGGL(k_damp_add, dim3((n3 + RB - 1) / RB), dim3(RB), 0, s, n3, c->qs->dampv, w, out);
+  }
   return c->hip(hipGetLastError(), "qs tangent");
 }
 
@@ -552,6 +641,44 @@ static int dot(Ctx* c, const double* x, const double* y, double* out) {
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, c->stream, S->nblk, S->part, 1, S->scal);
   int rc = c->hip(hipMemcpyAsync(out, S->scal, 8, hipMemcpyDeviceToHost, c->stream), "d2h");
   if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
+// ||v||_2 and ||v||_inf over all entries (mask null) or the masked ones.
+static int norms(Ctx* c, const double* v, const signed char* mask, double* n2, double* ninf) {
+  QsState* S = c->qs;
+  const int n = 3 * c->N;
+  hipLaunchKernelGGL(k_norm_part, dim3(S->nblk), dim3(RB), 0, c->stream, n, v, mask, S->part);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(RB), 0, c->stream, S->nblk, S->part, S->scal + kScal - 2);
+  double h[2] = {0.0, 0.0};
+  int rc = c->hip(hipMemcpyAsync(h, S->scal + kScal - 2, 16, hipMemcpyDeviceToHost, c->stream), "d2h");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  *n2 = std::sqrt(h[0]);
+  *ninf = h[1];
+  return rc;
+}
+
+// Residual at the device state S->u into S->out and its norm
+// ||r||_2 + 20 ||r||_inf (src/core/Peridigm.cpp:4306-4316).
+static int residual_norm(Ctx* c, double* norm) {
+  int rc = residual(c);
+  double n2 = 0.0, ninf = 0.0;
+  if (!rc) rc = norms(c, c->qs->out, nullptr, &n2, &ninf);
+  *norm = n2 + 20.0 * ninf;
+  return rc;
+}
+
+// ||f V||_2 over the kinematic DOFs at S->u: the reaction norm that scales
+// the Newton tolerance (src/core/Peridigm.cpp:2716-2732). Leaves S->out
+// holding f V on every DOF.
+static int reaction_norm(Ctx* c, double* norm) {
+  QsArgs a = args(c);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_qs_force<2>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  double ninf = 0.0;
+  int rc = c->hip(hipGetLastError(), "qs reactions");
+  if (!rc) rc = norms(c, c->qs->out, c->qs->fixed, norm, &ninf);
   return rc;
 }
 
@@ -589,9 +716,12 @@ void Ctx::free_extra() {
   tev.clear();
   if (qs) {
     QsState* S = qs;
-    double* ptrs[] = {S->x, S->u, S->w, S->vol, S->m, S->alpha, S->beta, S->theta, S->dtheta, S->g,
-                      S->out, S->dinv, S->xs, S->r, S->z, S->p, S->q, S->b, S->part, S->scal};
+    double* ptrs[] = {S->x,   S->u,  S->w,    S->vol,  S->m,      S->alpha, S->beta,
+                      S->theta, S->dtheta, S->g, S->out, S->dinv, S->xs,    S->r,
+                      S->z,   S->p,  S->q,    S->b,    S->part,   S->scal,  S->diag,
+                      S->dampv};
     for (double* p : ptrs) if (p) (void)hipFree(p);
+    for (double* p : S->rv) if (p) (void)hipFree(p);
     for (int k = 0; k < 4; k++) {
       if (S->ph[k]) (void)hipFree(S->ph[k]);
       if (S->qh[k]) (void)hipFree(S->qh[k]);
@@ -723,7 +853,8 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     }
     return c->hip(hipGetLastError(), "cg direction");
   };
-  const bool graphs = S->graphs;
+  // the damped tangent adds a launch: no graphs while damping
+  const bool graphs = S->graphs && !S->damped;
   // graphs captured for other vectors or older network buffers are stale
   if (graphs && (S->gx != x || S->gb != b || (flexible && S->gepoch != c->gnn_epoch))) {
     qs::drop_graphs(c);
@@ -977,6 +1108,341 @@ int pdg_qs_tolerance_multiplier(const pdg_ctx* h, double* multiplier) {
   if (!c || !multiplier) return PDG_E_ARG;
   *multiplier = c->qs_tol_multiplier;
   return PDG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Quasi-static load stepping (Peridigm::executeQuasiStatic,
+// src/core/Peridigm.cpp:2470-3043): per load step the prescribed-displacement
+// increments, the reaction-scaled tolerance, the Newton loop with the
+// previous step's velocity as predictor, the damped-Newton switch and retry,
+// the candidate-alpha line search (:3364-3618), and adaptive halving of the
+// load step (:2889-2934). The tangent is applied matrix-free at the current
+// state in every iteration; the reference reassembles it every iteration
+// too, except in damped Newton, where it keeps one for 8 iterations.
+// ---------------------------------------------------------------------------
+namespace pdg {
+namespace {
+
+__global__ __launch_bounds__(RB) void k_div(int n, const double* x, double d, double* y) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) y[i] = x[i] / d;
+}
+
+struct QsRun {
+  Ctx* c;
+  QsState* S;
+  int n3;
+  unsigned vb;
+  double *ub, *du, *pred, *tmp, *rs;  // S->rv
+  double* lhs;                        // S->xs
+
+  // S->u = ub + du + a * lhs (the reference's y = x + u + deltaU)
+  void state(double a) {
+    hipStream_t s = c->stream;
+    if (a == 0.0) {
+      hipLaunchKernelGGL(k_lin2, dim3(vb), dim3(RB), 0, s, n3, 1.0, ub, 1.0, du, S->u);
+    } else {
+      hipLaunchKernelGGL(k_lin2, dim3(vb), dim3(RB), 0, s, n3, 1.0, du, a, lhs, tmp);
+      hipLaunchKernelGGL(k_lin2, dim3(vb), dim3(RB), 0, s, n3, 1.0, ub, 1.0, tmp, S->u);
+    }
+  }
+
+  // residual norm with the increment du + a * lhs
+  int trial(double a, double* norm) {
+    state(a);
+    return qs::residual_norm(c, norm);
+  }
+
+  // quasiStaticsLineSearch (src/core/Peridigm.cpp:3364-3618)
+  int line_search(double* alpha_out) {
+    double r0 = 0.0;
+    int rc = trial(0.0, &r0);
+    if (rc) return rc;
+    if (!std::isfinite(r0)) return c->fail(PDG_E_NONFINITE, "NaN in the residual of the line search");
+    if (r0 == 0.0) { *alpha_out = 0.0; return PDG_OK; }
+    rc = c->hip(hipMemcpyAsync(rs, S->out, 8 * (size_t)n3, hipMemcpyDeviceToDevice, c->stream), "copy");
+    if (rc) return rc;
+    double best = 1.0, bestres = 1.0e50;
+    std::vector<double> cand;
+    // the secant estimate from a small perturbation
+    const double eps = 1.0e-4;
+    double rp = 0.0, SR = 0.0, SPR = 0.0;
+    if ((rc = trial(eps, &rp))) return rc;
+    if ((rc = qs::dot(c, lhs, rs, &SR))) return rc;
+    if ((rc = qs::dot(c, lhs, S->out, &SPR))) return rc;
+    const double ta = -1.0 * eps * SR / (SPR - SR);
+    if (ta > -0.1 && ta < 10.0) cand.push_back(ta);
+    for (double a : {0.1, 0.2, 0.3, 0.5, 0.75, 1.0}) cand.push_back(a);
+    for (double a : cand) {
+      double r = 0.0;
+      if ((rc = trial(a, &r))) return rc;
+      if (std::isfinite(r) && r < bestres) { best = a; bestres = r; }
+    }
+    if ((r0 - bestres) / r0 > 0.02) { *alpha_out = best; return PDG_OK; }
+    // brute force
+    cand.clear();
+    for (int i = 0; i < 25; ++i) cand.push_back(0.002 * (i + 1));
+    for (int i = 0; i < 15; ++i) cand.push_back(0.01 * (i + 1) + 0.05);
+    for (int i = 0; i < 6; ++i) cand.push_back(0.05 * (i + 1) + 0.2);
+    for (double a : {0.75, 1.0, 1.5, 2.0, 5.0, 10.0}) cand.push_back(a);
+    std::vector<std::pair<double, double>> up;  // (alpha, relative increase)
+    for (double a : cand) {
+      double r = 0.0;
+      if ((rc = trial(a, &r))) return rc;
+      if (std::isfinite(r)) {
+        if (r < bestres) { best = a; bestres = r; }
+        else if (r > r0) up.push_back({a, (r - r0) / r0});
+      }
+    }
+    if ((r0 - bestres) / r0 > 0.0001) { *alpha_out = best; return PDG_OK; }
+    // no reduction: the alpha whose increase is closest to 10%
+    double bestdev = 1.0e50;
+    for (const auto& pr : up) {
+      if (std::fabs(pr.second - 0.1) < bestdev) { best = pr.first; bestdev = std::fabs(pr.second - 0.1); }
+    }
+    *alpha_out = best;
+    return PDG_OK;
+  }
+
+  // damped-Newton diagonal (quasiStaticsDampTangent, :3302-3319) from the
+  // diagonal of K at the current state: d' = f d + g ||f d||_inf
+  int damp(double scale, double shift) {
+    double n2 = 0.0, ninf = 0.0;
+    int rc = qs::norms(c, S->diag, nullptr, &n2, &ninf);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_damp_vec, dim3(vb), dim3(RB), 0, c->stream, n3, S->diag, scale,
+                       shift * (scale * ninf), S->dampv);
+    S->damped = true;
+    return c->hip(hipGetLastError(), "damp");
+  }
+};
+
+}  // namespace
+}  // namespace pdg
+
+extern "C" {
+
+int pdg_qs_params_default(pdg_qs_params* p) {
+  if (!p) return PDG_E_ARG;
+  p->maxSolverIterations = 10;
+  p->tolerance = 1.0e-6;
+  p->absoluteTolerance = 0;
+  p->disableHeuristics = 0;
+  p->dampedNewtonDiagonalScaleFactor = 1.0001;
+  p->dampedNewtonDiagonalShiftFactor = 0.00001;
+  p->adaptiveLoadStepping = 0;
+  p->maxLoadStepReductionsInOneStep = 0;
+  p->maxTotalLoadStepReductions = 0;
+  p->reduceAllRemainingLoadSteps = 0;
+  p->precond = PDG_PRECOND_NONE;
+  p->linearTolerance = 1.0e-4;
+  p->maxLinearIterations = 0;
+  return PDG_OK;
+}
+
+int pdg_qs_run(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* times, int numTimes,
+               const pdg_qs_params* prm, double* u, pdg_qs_step* steps, int maxSteps,
+               int* numSteps) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u || !prm || !times || numTimes < 1 || numBcs < 0 ||
+      (numBcs > 0 && !bcs) || prm->precond < 0 || prm->precond > 2)
+    return PDG_E_ARG;
+  if (prm->adaptiveLoadStepping &&
+      prm->maxTotalLoadStepReductions < prm->maxLoadStepReductionsInOneStep)
+    return c->fail(PDG_E_ARG, "'Maximum total load step reductions' cannot be smaller than "
+                              "'Maximum load step reductions in one step'");
+  for (int k = 1; k < numTimes; k++)
+    if (!(times[k] > times[k - 1])) return c->fail(PDG_E_ARG, "load-step times must increase");
+  (void)hipSetDevice(c->device);
+  int rc = qs::ensure(c);
+  if (rc) return rc;
+  QsState* S = c->qs;
+  const int N = c->N, n3 = 3 * N;
+  hipStream_t s = c->stream;
+  // kinematic DOFs (internal order) with their u(t) = c1 t + c0
+  std::vector<int> kidx(n3, -1), kdof;
+  std::vector<double> kc0, kc1;
+  for (int b = 0; b < numBcs; b++) {
+    if (bcs[b].coord < 0 || bcs[b].coord > 2 || bcs[b].numNodes < 0 || (bcs[b].numNodes && !bcs[b].nodes))
+      return c->fail(PDG_E_ARG, "invalid boundary condition");
+    for (int q = 0; q < bcs[b].numNodes; q++) {
+      const int p = bcs[b].nodes[q];
+      if (p < 0 || p >= N) return c->fail(PDG_E_ARG, "boundary-condition node out of range");
+      const int d = 3 * c->iperm[p] + bcs[b].coord;
+      if (kidx[d] < 0) {
+        kidx[d] = (int)kdof.size();
+        kdof.push_back(d);
+        kc0.push_back(0.0);
+        kc1.push_back(0.0);
+      }
+      kc0[kidx[d]] = bcs[b].c0;
+      kc1[kidx[d]] = bcs[b].c1;
+    }
+  }
+  S->hfixed.assign(n3, 0);
+  for (int d : kdof) S->hfixed[d] = 1;
+  rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), n3, hipMemcpyHostToDevice, s), "h2d fixed");
+  for (int k = 0; k < 5 && !rc; k++)
+    if (!S->rv[k]) rc = c->hip(hipMalloc((void**)&S->rv[k], std::max<size_t>(8 * (size_t)n3, 64)), "qs alloc");
+  if (rc) return rc;
+  QsRun R{c, S, n3, (unsigned)((n3 + RB - 1) / RB), S->rv[0], S->rv[1], S->rv[2], S->rv[3], S->rv[4], S->xs};
+  if ((rc = qs::to_dev(c, u, R.ub))) return rc;
+  rc = c->hip(hipMemsetAsync(R.pred, 0, 8 * (size_t)n3, s), "zero predictor");
+  std::vector<double> hub(n3), hdu(n3);
+  if (!rc) rc = qs::to_host(c, R.ub, u);  // reference order, for the zero-increment rule
+  if (rc) return rc;
+  for (int k = 0; k < N; k++)
+    for (int q = 0; q < 3; q++) hub[3 * (size_t)k + q] = u[3 * (size_t)c->perm[k] + q];
+  const bool adaptive = prm->adaptiveLoadStepping != 0, noheur = prm->disableHeuristics != 0;
+  const int maxLin = prm->maxLinearIterations > 0 ? prm->maxLinearIterations : n3;
+  std::vector<double> ts(times, times + numTimes);
+  const double tfinal = ts.back();
+  double tc = ts[0], tp = tc;
+  bool failed = false, failed_in_step = false;
+  int fail_step = 0, fail_total = 0, nrec = 0;
+  int result = PDG_OK;
+  for (int step = 1; step < (int)ts.size(); step++) {
+    if (!adaptive || !failed) tp = tc;
+    failed = false;
+    tc = ts[step];
+    const double dt = tc - tp;
+    // deltaU: the prescribed increments (DirichletIncrementBC with coefficient
+    // 1, src/core/Peridigm_BoundaryCondition.cpp:202-241; a zero increment
+    // takes the field's own value as the previous one, :136-143)
+    std::fill(hdu.begin(), hdu.end(), 0.0);
+    if (tc != tp) {
+      for (size_t k = 0; k < kdof.size(); k++) {
+        const double cur = kc1[k] * tc + kc0[k];
+        double prev = kc1[k] * tp + kc0[k];
+        if (cur - prev == 0.0) prev = hub[kdof[k]];
+        hdu[kdof[k]] = 1.0 * (cur - prev) + 0.0 * (cur - prev) * (1.0 / dt);
+      }
+    }
+    rc = c->hip(hipMemcpyAsync(R.du, hdu.data(), 8 * (size_t)n3, hipMemcpyHostToDevice, s), "h2d du");
+    if (rc) return rc;
+    R.state(0.0);
+    S->damped = false;
+    double mult = 1.0, rnorm = 0.0;
+    if (!prm->absoluteTolerance && (rc = qs::reaction_norm(c, &mult))) return rc;
+    if ((rc = qs::residual_norm(c, &rnorm))) return rc;
+    c->qs_tol_multiplier = mult;
+    const double tol = prm->tolerance * mult;
+    int it = 1, lits = 0;
+    bool damped = false;
+    int prec = prm->precond;
+    double alpha = 0.0;
+    while (rnorm > tol && it <= prm->maxSolverIterations) {
+      bool conv = true;
+      if (it == 1 && step > 1 && !noheur) {
+        // predictor: the previous load step's velocity times this increment
+        hipLaunchKernelGGL(k_lin2, dim3(R.vb), dim3(RB), 0, s, n3, dt, R.pred, 0.0, R.pred, R.lhs);
+        hipLaunchKernelGGL(k_zero_fixed, dim3(R.vb), dim3(RB), 0, s, n3, S->fixed, R.lhs);
+      } else {
+        if (it > 50 && !damped && !noheur) damped = true;
+        // tangent at the current state: diagonal blocks (preconditioner and
+        // damping), then the linear solve K lhs = r
+        if ((rc = qs::block_jacobi(c))) return rc;
+        if (damped && (rc = R.damp(prm->dampedNewtonDiagonalScaleFactor,
+                                   prm->dampedNewtonDiagonalShiftFactor)))
+          return rc;
+        if ((rc = qs::residual(c))) return rc;
+        auto solve = [&]() -> int {
+          int r2 = c->hip(hipMemcpyAsync(S->b, S->out, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
+          if (!r2) r2 = c->hip(hipMemsetAsync(R.lhs, 0, 8 * (size_t)n3, s), "zero");
+          if (r2) return r2;
+          int li = 0;
+          double rel = 0.0;
+          r2 = qs_solve(c, R.lhs, S->b, prec, prm->linearTolerance, maxLin, &li, &rel);
+          lits += li;
+          return r2;
+        };
+        rc = solve();
+        if (rc == PDG_E_NOCONV) {
+          conv = false;
+          if (!noheur) {
+            // damp the tangent, drop the preconditioner and try again
+            if (!damped && (rc = R.damp(prm->dampedNewtonDiagonalScaleFactor,
+                                        prm->dampedNewtonDiagonalShiftFactor)))
+              return rc;
+            damped = true;
+            prec = PDG_PRECOND_NONE;
+            rc = solve();
+            conv = rc == PDG_OK;
+            if (rc == PDG_E_NOCONV) rc = PDG_OK;
+          } else {
+            rc = PDG_OK;
+          }
+        }
+        if (rc) return rc;
+      }
+      if (!conv) { rnorm = 1.0e50; break; }
+      hipLaunchKernelGGL(k_zero_fixed, dim3(R.vb), dim3(RB), 0, s, n3, S->fixed, R.lhs);
+      alpha = 1.0;
+      if (!noheur && (rc = R.line_search(&alpha))) return rc;
+      hipLaunchKernelGGL(k_lin2, dim3(R.vb), dim3(RB), 0, s, n3, 1.0, R.du, alpha, R.lhs, R.du);
+      R.state(0.0);
+      if ((rc = qs::residual_norm(c, &rnorm))) return rc;
+      it++;
+    }
+    S->damped = false;
+    int status = 0;
+    if (rnorm > tol) {
+      if (rnorm < 100.0 * tol) {
+        status = 1;  // accepted with a warning
+      } else if (adaptive) {
+        fail_step++;
+        fail_total++;
+        if (fail_total > prm->maxTotalLoadStepReductions ||
+            fail_step > prm->maxLoadStepReductionsInOneStep) {
+          result = c->fail(PDG_E_NOCONV, "quasi-static solve failed: load-step reductions exhausted");
+          break;
+        }
+        failed = true;
+        failed_in_step = true;
+      } else {
+        result = c->fail(PDG_E_NOCONV, "quasi-static solve failed to converge");
+        break;
+      }
+    }
+    if (adaptive && failed) {
+      ts.insert(ts.begin() + step, tp + dt / 2.0);
+      step--;
+    }
+    if (adaptive && prm->reduceAllRemainingLoadSteps && !failed && failed_in_step) {
+      ts.erase(ts.begin() + step + 1, ts.end());
+      for (int i = 0; ts[step + i] < tfinal; i++) ts.push_back(tc + (i + 1) * dt);
+    }
+    if (!adaptive || !failed) {
+      failed_in_step = false;
+      fail_step = 0;
+      // u += deltaU; predictor = deltaU / dt
+      hipLaunchKernelGGL(k_div, dim3(R.vb), dim3(RB), 0, s, n3, R.du, dt, R.pred);
+      hipLaunchKernelGGL(k_lin2, dim3(R.vb), dim3(RB), 0, s, n3, 1.0, R.ub, 1.0, R.du, R.ub);
+      rc = c->hip(hipMemcpyAsync(hub.data(), R.ub, 8 * (size_t)n3, hipMemcpyDeviceToHost, s), "d2h");
+      if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+      if (rc) return rc;
+      if (steps && nrec < maxSteps) {
+        pdg_qs_step& st = steps[nrec];
+        st.timePrevious = tp;
+        st.timeCurrent = tc;
+        st.newtonIterations = it - 1;
+        st.linearIterations = lits;
+        st.residualNorm = rnorm;
+        st.toleranceMultiplier = mult;
+        st.status = status;
+        st.dampedNewton = damped ? 1 : 0;
+        st.lastAlpha = alpha;
+      }
+      nrec++;
+    }
+  }
+  S->damped = false;
+  if (numSteps) *numSteps = nrec;
+  rc = qs::to_host(c, R.ub, u);
+  if (rc) return rc;
+  return result;
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@ import time
 
 import numpy as np
 
+from ._lib import pdg_qs_params, pdg_qs_step
 from .core import Context, _f64, _i32, _p
 
 PRECOND = {"none": 0, "block_jacobi": 1, "learned": 2}
@@ -119,3 +120,41 @@ def run_tension(ctx: Context, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8,
         stats.append(st)
     wall = time.perf_counter() - t0
     return u, stats, wall
+
+
+def default_params(**overrides):
+    """pdg_qs_params with the reference's defaults, then the given fields
+    (maxSolverIterations, tolerance, absoluteTolerance, disableHeuristics,
+    adaptiveLoadStepping, ..., precond as a name or code, linearTolerance)."""
+    prm = pdg_qs_params()
+    from ._lib import load
+    load().pdg_qs_params_default(C.byref(prm))
+    for k, v in overrides.items():
+        if k == "precond" and isinstance(v, str):
+            v = PRECOND[v]
+        setattr(prm, k, v)
+    return prm
+
+
+def run(ctx: Context, bcs, times, u0=None, **params):
+    """Peridigm::executeQuasiStatic on the device (pdg_qs_run): load steps at
+    `times`, prescribed displacements bcs = [(node_ids, coord, c0, c1)] with
+    u(t) = c1 t + c0, parameters as default_params. Returns (u, steps) with
+    one dict per accepted load step; raises PdgError if the run aborts."""
+    N = ctx.N
+    u = np.zeros((N, 3)) if u0 is None else _f64(u0, (N, 3)).copy()
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    prm = default_params(**params)
+    cap = 4 * len(t) + 64
+    steps = (pdg_qs_step * cap)()
+    ns = C.c_int()
+    arr = ctx.bc_array(bcs)
+    rc = ctx.lib.pdg_qs_run(ctx.h, arr, len(bcs), _p(t), t.size, C.byref(prm), _p(u), steps, cap,
+                            C.byref(ns))
+    out = [dict(t0=st.timePrevious, t1=st.timeCurrent, newton=st.newtonIterations,
+                linear=st.linearIterations, residual=st.residualNorm,
+                multiplier=st.toleranceMultiplier, status=st.status,
+                damped=bool(st.dampedNewton), alpha=st.lastAlpha)
+           for st in steps[:min(ns.value, cap)]]
+    ctx.check(rc)
+    return u, out

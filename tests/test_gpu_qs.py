@@ -191,3 +191,93 @@ def test_newton_tolerance_scales_with_reactions_below_one():
         u2, st2 = Q.newton(ctx, u, tol=1.0, precond="block_jacobi")
         assert Q.tolerance_multiplier(ctx) == 1.0
         assert st2["newton"] == 0  # the entry residual is already below an absolute 1.0
+
+
+def _run_case(eps, n=(10, 4, 4), h=1e-3):
+    x, vol, nl = O.quickgrid(*n, [0, 0, 0], [k * h for k in n], 3.015 * h)
+    i = np.rint(x[:, 0] / h - 0.5).astype(int)
+    left, right = np.nonzero(i <= 1)[0], np.nonzero(i >= n[0] - 2)[0]
+    Lx = n[0] * h
+    bcs = []
+    for c in range(3):
+        bcs.append((left, c, 0.0, -0.5 * eps * Lx if c == 0 else 0.0))
+        bcs.append((right, c, 0.0, 0.5 * eps * Lx if c == 0 else 0.0))
+    return x, vol, nl, bcs
+
+
+def _gpu_run(x, vol, nl, bcs, times, **prm):
+    from peridigm_amd import Context
+    from peridigm_amd import quasistatic as Q
+    with Context(0) as ctx:
+        ctx.set_topology(nl, len(vol))
+        ctx.set_constants(x, vol, 130e9, 78e9, 3.015e-3, 0, 0.0)
+        return Q.run(ctx, bcs, times, **prm)
+
+
+@pytest.mark.parametrize("heuristics", [True, False])
+def test_load_stepping_driver_matches_oracle(heuristics):
+    """pdg_qs_run against the oracle's restatement of executeQuasiStatic
+    (oracle/qs_driver.py, dense exact solves): tension bar, 4 load steps.
+    With the linear solves at 1e-12 both take the same Newton iterations
+    (the predictor step included), line-search steps and tolerance
+    multipliers, and end at the same displacement (1e-8 relative)."""
+    from oracle import qs_driver as D
+    x, vol, nl, bcs = _run_case(1e-3)
+    times = np.linspace(0.0, 1.0, 5)
+    flags = dict(disableHeuristics=0 if heuristics else 1)
+    u, st = _gpu_run(x, vol, nl, bcs, times, precond="block_jacobi", linearTolerance=1e-12,
+                     **flags)
+    uo, so, ab = D.qs_run(x, vol, nl, 130e9, 78e9, 3.015e-3, bcs, times,
+                          disableHeuristics=not heuristics)
+    print([(s["newton"], s["alpha"]) for s in st], [(s["newton"], s["alpha"]) for s in so])
+    assert not ab and len(st) == len(so) == 4
+    for g, o in zip(st, so):
+        assert g["t1"] == o["t1"] and g["newton"] == o["newton"] and g["status"] == o["status"]
+        assert abs(g["multiplier"] - o["multiplier"]) <= 1e-9 * o["multiplier"]
+        assert abs(g["alpha"] - o["alpha"]) <= 1e-6
+    if heuristics:
+        assert all(s["newton"] >= 1 for s in st)
+    else:
+        assert all(s["alpha"] == 1.0 for s in st)
+    assert np.abs(u - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+def test_load_stepping_adaptive_halving_matches_oracle():
+    """Adaptive load stepping: at 5% strain in two load steps with at most
+    two Newton iterations and a 1e-9 tolerance, failed steps are halved
+    (insertions at the midpoint, :2919-2924) and steps within 100x the
+    tolerance are accepted; the GPU run takes the same time points, Newton
+    counts and acceptance states as the oracle's."""
+    from oracle import qs_driver as D
+    x, vol, nl, bcs = _run_case(5e-2)
+    times = np.linspace(0.0, 1.0, 3)
+    prm = dict(tolerance=1e-9, maxSolverIterations=2, adaptiveLoadStepping=1,
+               maxLoadStepReductionsInOneStep=3, maxTotalLoadStepReductions=6)
+    u, st = _gpu_run(x, vol, nl, bcs, times, precond="block_jacobi", linearTolerance=1e-12, **prm)
+    uo, so, ab = D.qs_run(x, vol, nl, 130e9, 78e9, 3.015e-3, bcs, times,
+                          **{k: bool(v) if k == "adaptiveLoadStepping" else v for k, v in prm.items()})
+    print([s["t1"] for s in st])
+    assert not ab
+    assert [s["t1"] for s in st] == [s["t1"] for s in so]
+    assert len(st) > 2  # some load steps were halved
+    assert [s["newton"] for s in st] == [s["newton"] for s in so]
+    assert [s["status"] for s in st] == [s["status"] for s in so]
+    assert np.abs(u - uo).max() <= 1e-7 * np.abs(uo).max()
+
+
+def test_load_stepping_aborts_when_reductions_run_out():
+    """One Newton iteration per step at a 1e-8 tolerance cannot converge the
+    first load step even after three halvings: the run aborts (PDG_E_NOCONV),
+    as the oracle's does, with no step accepted."""
+    from oracle import qs_driver as D
+    from peridigm_amd._lib import PdgError
+    x, vol, nl, bcs = _run_case(1e-3)
+    times = np.linspace(0.0, 1.0, 3)
+    prm = dict(tolerance=1e-8, maxSolverIterations=1, adaptiveLoadStepping=1,
+               maxLoadStepReductionsInOneStep=3, maxTotalLoadStepReductions=6)
+    _, so, ab = D.qs_run(x, vol, nl, 130e9, 78e9, 3.015e-3, bcs, times,
+                         **{k: bool(v) if k == "adaptiveLoadStepping" else v for k, v in prm.items()})
+    assert ab and so == []
+    with pytest.raises(PdgError) as e:
+        _gpu_run(x, vol, nl, bcs, times, precond="block_jacobi", linearTolerance=1e-12, **prm)
+    assert e.value.code == -5

@@ -430,3 +430,27 @@ def test_gnn_backward_matches_finite_differences():
         lp = np.sum(O.gnn_forward(rp, col, ew, dims, pp, feat)[0] * gout)
         lm = np.sum(O.gnn_forward(rp, col, ew, dims, pm, feat)[0] * gout)
         assert abs((lp - lm) / (2 * eps) - dp[k]) <= 1e-6 * max(1.0, abs(dp[k]))
+
+
+def test_qs_driver_linear_regime_closed_form():
+    """The oracle's load-stepping driver in the linear regime (strain 1e-8,
+    where the LPS force is linear in u to round-off): the first load step
+    takes one exact Newton solve with line-search step 1, and every later
+    step is solved by the predictor alone (the previous step's velocity
+    times the increment is the exact increment for a linear ramp), so each
+    step reports one solver iteration and alpha = 1."""
+    from oracle import qs_driver as D
+    n, h = (6, 3, 3), 1e-3
+    x, vol, nl = O.quickgrid(*n, [0, 0, 0], [k * h for k in n], 3.015 * h)
+    i = np.rint(x[:, 0] / h - 0.5).astype(int)
+    left, right = np.nonzero(i <= 1)[0], np.nonzero(i >= n[0] - 2)[0]
+    eps = 1e-8
+    bcs = [(left, c, 0.0, -0.5 * eps * n[0] * h if c == 0 else 0.0) for c in range(3)] + \
+          [(right, c, 0.0, 0.5 * eps * n[0] * h if c == 0 else 0.0) for c in range(3)]
+    u, st, ab = D.qs_run(x, vol, nl, 130e9, 78e9, 3.015 * h, bcs, np.linspace(0, 1, 4),
+                         tolerance=1e-6)
+    assert not ab and len(st) == 3
+    assert [s["newton"] for s in st] == [1, 1, 1]
+    assert all(abs(s["alpha"] - 1.0) <= 1e-6 for s in st)
+    # the ends hold the prescribed displacements
+    assert np.allclose(u[right, 0], 0.5 * eps * n[0] * h, rtol=1e-12, atol=0)
