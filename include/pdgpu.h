@@ -191,6 +191,11 @@ int pdg_qs_residual(pdg_ctx* ctx, const double* u, double* residual, double* nor
 /* K w = -V (df/du) w on free DOFs (forward-mode directional derivative of
  * the force), w on kinematic DOFs. */
 int pdg_qs_tangent_apply(pdg_ctx* ctx, const double* u, const double* w, double* Kw);
+/* FP64 internal force density f(u) on every DOF (3N, kinematic ones
+ * included) and dilatation (N, optional) at displacement u, from the
+ * quasi-static kernels (src/materials/elastic.cxx:55-146,
+ * src/materials/material_utilities.cxx:235-288 with y = x + u). */
+int pdg_qs_force_dilatation(pdg_ctx* ctx, const double* u, double* force, double* dilatation);
 /* Inverses of the 3x3 diagonal blocks of K at u (9N, row-major per node). */
 int pdg_qs_block_diagonal(pdg_ctx* ctx, const double* u, double* blocksInv);
 /* Solve K(u) x = rhs (x: initial guess in, solution out) to ||r||/||rhs|| <=

@@ -78,6 +78,7 @@ SIGNATURES = {
     "pdg_state_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pdg_set_bcs": (_i, [_vp, C.POINTER(pdg_bc), _i]),
     "pdg_qs_params_default": (_i, [C.POINTER(pdg_qs_params)]),
+    "pdg_qs_force_dilatation": (_i, [_vp, _vp, _vp, _vp]),
     "pdg_qs_run": (_i, [_vp, C.POINTER(pdg_bc), _i, _vp, _i, C.POINTER(pdg_qs_params), _vp,
                         C.POINTER(pdg_qs_step), _i, C.POINTER(_i)]),
     "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),

@@ -1096,6 +1096,33 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
   return PDG_OK;
 }
 
+int pdg_qs_force_dilatation(pdg_ctx* h, const double* u, double* force, double* dilatation) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || !u || !force) return PDG_E_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = qs::ensure(c);
+  QsState* S = c->qs;
+  if (!rc) rc = qs::to_dev(c, u, S->u);
+  if (rc) return rc;
+  QsArgs a = qs::args(c);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_qs_force<2>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  rc = c->hip(hipGetLastError(), "qs force");
+  if (!rc) rc = qs::to_host(c, S->out, force);
+  if (rc) return rc;
+  for (int p = 0; p < c->N; p++)
+    for (int d = 0; d < 3; d++) force[3 * (size_t)p + d] /= c->vol[p];
+  if (dilatation) {
+    std::vector<double> th(c->N);
+    rc = c->hip(hipMemcpyAsync(th.data(), S->theta, 8 * (size_t)c->N, hipMemcpyDeviceToHost, s), "d2h");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    for (int k = 0; k < c->N; k++) dilatation[c->perm[k]] = th[k];
+  }
+  return PDG_OK;
+}
+
 int pdg_qs_set_absolute_tolerance(pdg_ctx* h, int on) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;

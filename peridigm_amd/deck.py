@@ -1,0 +1,356 @@
+"""Input decks: Peridigm's XML parameter lists run on the device.
+
+`run_deck(path)` reads a deck in Peridigm's Teuchos XML format and runs it
+through the C ABI: the discretization (Exodus mesh with bond filters, or a
+PdQuickGrid tensor-product lattice), one block of the "Elastic" (LPS)
+material with an optional "Critical Stretch" damage model, the initial and
+boundary conditions, the "Verlet" or "QuasiStatic" solver, and the ExodusII
+output frames (src/core/Peridigm.cpp:1290-1593 and :2470-3043 for the
+solvers, src/io/Peridigm_OutputManager_ExodusII.cpp:210-470 for the output).
+
+Supported subset (anything else raises DeckError):
+  - boundary-condition values are expressions of x, y, z and t (C syntax:
+    numbers, + - * /, parentheses, and sin cos tan exp log sqrt fabs pow);
+    prescribed displacements must be linear in t (u = c1(x) t + c0(x)), as the
+    device integrator applies them;
+  - elastic constants: any two of bulk, shear, Young's modulus and Poisson's
+    ratio (src/materials/Peridigm_Material.cpp:335-433);
+  - one block; influence function "One", "Parabolic Decay" or "Gaussian".
+"""
+from __future__ import annotations
+
+import ast
+import math
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from . import exodus as EX
+from ._lib import INFLUENCE
+from .core import Context
+from .discretization import quickgrid, radius_search
+
+
+class DeckError(ValueError):
+    pass
+
+
+# ------------------------------------------------------------------ parsing
+def _convert(typ, val):
+    typ = (typ or "string").lower()
+    if typ in ("double", "float"):
+        return float(val)
+    if typ == "int":
+        return int(val)
+    if typ == "bool":
+        return val.strip().lower() in ("true", "1", "yes")
+    return val
+
+
+def _plist(elem):
+    out = {}
+    for ch in elem:
+        if ch.tag == "ParameterList":
+            out[ch.get("name")] = _plist(ch)
+        elif ch.tag == "Parameter":
+            out[ch.get("name")] = _convert(ch.get("type"), ch.get("value"))
+    return out
+
+
+def parse_deck(path):
+    """The deck as nested dicts (ParameterList -> dict, Parameter -> value)."""
+    return _plist(ET.parse(path).getroot())
+
+
+# -------------------------------------------------------------- expressions
+_FUNCS = {"sin": np.sin, "cos": np.cos, "tan": np.tan, "exp": np.exp, "log": np.log,
+          "sqrt": np.sqrt, "fabs": np.abs, "abs": np.abs, "pow": np.power}
+_BIN = {ast.Add: np.add, ast.Sub: np.subtract, ast.Mult: np.multiply, ast.Div: np.divide}
+
+
+def evaluate(expr, x, y, z, t):
+    """Value of a boundary-condition expression at points (x, y, z) and time t.
+    Parsed with Python's ast and evaluated node by node: only numbers, the
+    variables x y z t, the four operators, unary minus and the listed
+    functions are accepted."""
+    s = expr.strip().rstrip(";")
+    if "=" in s:
+        lhs, s = s.split("=", 1)
+        if lhs.strip() != "value":
+            raise DeckError("expression must assign 'value': %r" % expr)
+    try:
+        tree = ast.parse(s.strip(), mode="eval")
+    except SyntaxError as e:
+        raise DeckError("cannot parse expression %r: %s" % (expr, e))
+    env = {"x": x, "y": y, "z": z, "t": t, "pi": math.pi, "M_PI": math.pi}
+
+    def ev(n):
+        if isinstance(n, ast.Expression):
+            return ev(n.body)
+        if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)):
+            return float(n.value)
+        if isinstance(n, ast.Name) and n.id in env:
+            return env[n.id]
+        if isinstance(n, ast.BinOp) and type(n.op) in _BIN:
+            return _BIN[type(n.op)](ev(n.left), ev(n.right))
+        if isinstance(n, ast.UnaryOp) and isinstance(n.op, (ast.USub, ast.UAdd)):
+            v = ev(n.operand)
+            return -v if isinstance(n.op, ast.USub) else v
+        if isinstance(n, ast.Call) and isinstance(n.func, ast.Name) and n.func.id in _FUNCS \
+                and not n.keywords:
+            return _FUNCS[n.func.id](*[ev(a) for a in n.args])
+        raise DeckError("unsupported expression element in %r" % expr)
+
+    return np.broadcast_to(np.asarray(ev(tree), dtype=np.float64), np.shape(x)).copy()
+
+
+def linear_in_t(expr, x, y, z):
+    """(c0, c1) with value = c1 t + c0 at every point, checked at t = 0.5, 2."""
+    c0 = evaluate(expr, x, y, z, 0.0)
+    c1 = evaluate(expr, x, y, z, 1.0) - c0
+    for t in (0.5, 2.0):
+        v = evaluate(expr, x, y, z, t)
+        if not np.allclose(v, c0 + c1 * t, rtol=1e-12, atol=1e-300 + 1e-14 * np.abs(c0 + c1 * t).max()):
+            raise DeckError("prescribed displacement %r is not linear in t" % expr)
+    return c0, c1
+
+
+# ---------------------------------------------------------------- material
+def elastic_constants(mat):
+    """(K, mu) from exactly two of bulk modulus, shear modulus, Young's
+    modulus and Poisson's ratio, by the reference's formulas
+    (src/materials/Peridigm_Material.cpp:335-433)."""
+    K = mat.get("Bulk Modulus")
+    G = mat.get("Shear Modulus")
+    E = mat.get("Young's Modulus")
+    nu = mat.get("Poisson's Ratio")
+    if sum(v is not None for v in (K, G, E, nu)) != 2:
+        raise DeckError("exactly two elastic constants must be given")
+    if K is not None:
+        k = K
+    elif E is not None and G is not None:
+        k = (E * G) / (3.0 * (3.0 * G - E))
+    elif E is not None and nu is not None:
+        if mat.get("Plane Strain", False):
+            k = E / (2.0 * (1.0 - nu - 2.0 * nu * nu))
+        elif mat.get("Plane Stress", False):
+            k = E / (2.0 * (1.0 - nu))
+        else:
+            k = E / (3.0 * (1.0 - 2.0 * nu))
+    else:
+        k = (2.0 * G * (1.0 + nu)) / (3.0 * (1.0 - 2.0 * nu))
+    if G is not None:
+        g = G
+    elif K is not None and E is not None:
+        g = (3.0 * K * E) / (9.0 * K - E)
+    elif K is not None and nu is not None:
+        g = (3.0 * K * (1.0 - 2.0 * nu)) / (2.0 * (1.0 + nu))
+    else:
+        g = E / (2.0 * (1.0 + nu))
+    return float(k), float(g)
+
+
+# ----------------------------------------------------------- discretization
+_PLANE_KEYS = ["Normal_X", "Normal_Y", "Normal_Z", "Lower_Left_Corner_X", "Lower_Left_Corner_Y",
+               "Lower_Left_Corner_Z", "Bottom_Unit_Vector_X", "Bottom_Unit_Vector_Y",
+               "Bottom_Unit_Vector_Z", "Bottom_Length", "Side_Length"]
+
+
+def discretize(deck, base_dir, horizon):
+    """(x, volume, neighbour list, point sets) of the deck's discretization."""
+    d = deck["Discretization"]
+    typ = d.get("Type")
+    planes = []
+    for flt in d.get("Bond Filters", {}).values():
+        if flt.get("Type") != "Rectangular_Plane":
+            raise DeckError("bond filter type %r not supported" % flt.get("Type"))
+        planes.append([float(flt[k]) for k in _PLANE_KEYS])
+    planes = np.array(planes) if planes else None
+    if typ == "Exodus":
+        mesh = EX.read_mesh(os.path.join(base_dir, d["Input Mesh File"]))
+        x, vol, blk, sets = EX.discretize(mesh)
+        if len(mesh["blocks"]) != 1:
+            raise DeckError("one block is supported")
+        nl = radius_search(x, horizon, planes)
+        return x, vol, nl, sets
+    if typ == "PdQuickGrid":
+        g = d["TensorProduct3DMeshGenerator"]
+        n = [g["Number Points X"], g["Number Points Y"], g["Number Points Z"]]
+        o = [g["X Origin"], g["Y Origin"], g["Z Origin"]]
+        L = [g["X Length"], g["Y Length"], g["Z Length"]]
+        if planes is not None:
+            raise DeckError("PdQuickGrid does not take bond filters (as in the reference)")
+        sph = d.get("NeighborhoodType", "Parallelepiped") == "Spherical"
+        x, vol, nl = quickgrid(n, o, L, horizon, spherical=sph)
+        return x, vol, nl, {}
+    raise DeckError("discretization type %r not supported" % typ)
+
+
+# ---------------------------------------------------------------- the run
+_COORD = {"x": 0, "y": 1, "z": 2}
+
+
+def _bcs(deck, x, sets):
+    """(v0, u0, prescribed [(ids, coord, c0, c1)]) from the deck's conditions."""
+    N = x.shape[0]
+    v0 = np.zeros((N, 3))
+    u0 = np.zeros((N, 3))
+    presc = []
+    for name, bc in deck.get("Boundary Conditions", {}).items():
+        if not isinstance(bc, dict):
+            continue
+        typ = bc.get("Type")
+        ns = bc.get("Node Set")
+        if ns not in sets:
+            raise DeckError("node set %r of %r not found" % (ns, name))
+        ids = sets[ns]
+        c = _COORD[bc["Coordinate"].strip().lower()]
+        px, py, pz = x[ids, 0], x[ids, 1], x[ids, 2]
+        if typ == "Initial Velocity":
+            v0[ids, c] = evaluate(bc["Value"], px, py, pz, 0.0)
+        elif typ == "Initial Displacement":
+            u0[ids, c] = evaluate(bc["Value"], px, py, pz, 0.0)
+        elif typ == "Prescribed Displacement":
+            c0, c1 = linear_in_t(bc["Value"], px, py, pz)
+            pairs = np.stack([c0, c1], 1)
+            uniq, inv = np.unique(pairs, axis=0, return_inverse=True)
+            for k, (a0, a1) in enumerate(uniq):
+                presc.append((ids[inv.ravel() == k], c, float(a0), float(a1)))
+        else:
+            raise DeckError("boundary condition type %r not supported" % typ)
+    return v0, u0, presc
+
+
+def run_deck(path, output=None, lanes=4):
+    """Runs a deck. Returns dict(times, frames {variable: (frames, N)}, x,
+    volume, neighbour list, info); with `output` a file name, also writes the
+    frames there in ExodusII."""
+    deck = parse_deck(path)
+    base = os.path.dirname(os.path.abspath(path))
+    blocks = list(deck["Blocks"].values())
+    if len(blocks) != 1:
+        raise DeckError("one block is supported")
+    blk = blocks[0]
+    horizon = float(blk["Horizon"])
+    mat = deck["Materials"][blk["Material"]]
+    if mat.get("Material Model") != "Elastic":
+        raise DeckError("material model %r not supported" % mat.get("Material Model"))
+    K, mu = elastic_constants(mat)
+    rho = float(mat["Density"])
+    infl = INFLUENCE[blk.get("Influence Function", "One")]
+    sc = 0.0
+    if "Damage Model" in blk:
+        dm = deck["Damage Models"][blk["Damage Model"]]
+        if dm.get("Damage Model") != "Critical Stretch":
+            raise DeckError("damage model %r not supported" % dm.get("Damage Model"))
+        sc = float(dm["Critical Stretch"])
+    x, vol, nl, sets = discretize(deck, base, horizon)
+    N = x.shape[0]
+    v0, u0, presc = _bcs(deck, x, sets)
+    solver = deck["Solver"]
+    t0 = float(solver.get("Initial Time", 0.0))
+    t1 = float(solver.get("Final Time", 1.0))
+    out = deck.get("Output", {})
+    freq = int(out.get("Output Frequency", 1))
+    want = {k for k, v in out.get("Output Variables", {}).items() if v}
+    ctx = Context(0, lanes=lanes)
+    try:
+        ctx.set_topology(nl, N)
+        ctx.set_constants(x, vol, K, mu, horizon, infl, sc)
+        m = ctx.initialize()
+        deg = _degrees(nl, N)
+        frames = {}
+        times = []
+
+        def record(t, st):
+            times.append(t)
+            f = st["force"]
+            fields = {
+                "Displacement": st["u"], "Velocity": st["v"], "Force_Density": f,
+                "Force": f * vol[:, None],
+            }
+            for name, arr in fields.items():
+                if name in want:
+                    for c, ax in enumerate("XYZ"):
+                        frames.setdefault(name + ax, []).append(arr[:, c].copy())
+            scal = {"Dilatation": st["dilatation"], "Weighted_Volume": m,
+                    "Damage": st["damage"], "Number_Of_Neighbors": deg.astype(float),
+                    "Volume": vol, "Element_Id": np.arange(1, N + 1, dtype=float),
+                    "Proc_Num": np.zeros(N)}
+            for name, arr in scal.items():
+                if name in want:
+                    frames.setdefault(name, []).append(np.asarray(arr, float).copy())
+
+        info = {}
+        if "Verlet" in solver:
+            vp = solver["Verlet"]
+            dt = vp["Fixed dt"] if "Fixed dt" in vp else ctx.critical_time_step(rho)
+            dt *= vp.get("Safety Factor", 1.0)
+            nsteps = int(math.floor((t1 - t0) / dt))
+            ctx.set_density(np.full(N, rho))
+            ctx.set_bcs(presc)
+            ctx.upload_state(u=u0, v=v0)
+            done = 0
+            ctx.verlet_run(0, dt, t0=t0, initial_eval=True)
+            record(t0, _explicit_state(ctx))
+            while done < nsteps:
+                k = min(freq, nsteps - done)
+                ctx.verlet_run(k, dt, t0=t0 + done * dt, initial_eval=False)
+                done += k
+                if done % freq == 0 or done == nsteps:
+                    record(t0 + done * dt, _explicit_state(ctx))
+            info = dict(solver="Verlet", dt=dt, steps=nsteps)
+        elif "QuasiStatic" in solver:
+            from . import quasistatic as Q
+            qp = solver["QuasiStatic"]
+            nls = int(qp["Number of Load Steps"])
+            ts = [t0 + (i + 1) * (t1 - t0) / nls for i in range(nls)]
+            ts = [t0] + ts
+            prm = dict(maxSolverIterations=int(qp.get("Maximum Solver Iterations", 10)),
+                       tolerance=float(qp.get("Relative Tolerance", 1e-6)),
+                       disableHeuristics=int(bool(solver.get("Disable Heuristics", False))),
+                       linearTolerance=float(qp.get("Belos Relative Tolerance", 1e-4)))
+            if "Absolute Tolerance" in qp:
+                prm.update(absoluteTolerance=1, tolerance=float(qp["Absolute Tolerance"]))
+            record(t0, _qs_state(ctx, u0, Q))
+            u, steps = Q.run(ctx, presc, ts, u0=u0, **prm)
+            st = _qs_state(ctx, u, Q)
+            record(ts[-1], st)
+            info = dict(solver="QuasiStatic", steps=steps)
+        else:
+            raise DeckError("solver not supported (Verlet or QuasiStatic)")
+    finally:
+        ctx.close()
+    res = dict(times=np.array(times), frames={k: np.array(v) for k, v in frames.items()},
+               x=x, volume=vol, nlist=nl, info=info)
+    if output:
+        nodal = {k: v for k, v in res["frames"].items()
+                 if k[:-1] in ("Displacement", "Velocity", "Force_Density", "Force")}
+        elem = {k: v for k, v in res["frames"].items() if k not in nodal}
+        EX.write_results(output, x, res["times"], nodal, elem)
+    return res
+
+
+def _degrees(nl, N):
+    out = np.zeros(N, np.int64)
+    p = 0
+    for i in range(N):
+        out[i] = nl[p]
+        p += nl[p] + 1
+    return out
+
+
+def _explicit_state(ctx):
+    st = ctx.download_state()
+    return dict(u=st["u"], v=st["v"], force=st["force"], dilatation=st["theta"],
+                damage=st["damage"])
+
+
+def _qs_state(ctx, u, Q):
+    """FP64 state of a quasi-static frame: displacement, force density and
+    dilatation at u (the quasi-static kernels)."""
+    f, th = Q.force_and_dilatation(ctx, u)
+    N = ctx.N
+    return dict(u=np.asarray(u, float).reshape(N, 3), v=np.zeros((N, 3)), force=f,
+                dilatation=th, damage=np.zeros(N))

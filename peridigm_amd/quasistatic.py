@@ -61,6 +61,13 @@ def newton(ctx: Context, u, tol=1e-6, max_newton=20, precond="block_jacobi", lin
     return u, dict(newton=ni.value, linear=li.value, residual=rn.value)
 
 
+def force_and_dilatation(ctx: Context, u):
+    """FP64 force density (N,3, every DOF) and dilatation (N) at u."""
+    u = _f64(u, (ctx.N, 3)); f = np.zeros((ctx.N, 3)); th = np.zeros(ctx.N)
+    ctx.check(ctx.lib.pdg_qs_force_dilatation(ctx.h, _p(u), _p(f), _p(th)))
+    return f, th
+
+
 def set_absolute_tolerance(ctx: Context, on=True):
     """Newton tolerance absolute (multiplier 1) instead of scaled by the
     reaction norm (the reference's "Absolute Tolerance")."""

@@ -13,6 +13,9 @@ Sources (paths relative to the reference root):
   * src/io/discretization/unit_test/utPeridigm_PdQuickGridDiscretization.cpp:70-201
     (2x2x2 lattice, horizon 0.501, spherical, 3 neighbours, centres, volume).
   * src/io/pdneigh/unit_test/utFinitePlane.cxx:52-104 (plane/bond intersection).
+  * test/regression/PrecrackedPlate/PrecrackedPlate_gold.e: the two output
+    frames of the quasi-static run, with the .comp tolerances; the WaveInBar
+    and PrecrackedPlate decks and meshes are copied to decks/ unchanged.
   * test/verification/BondBreakingInitialVelocity/BondBreakingInitialVelocity.xml
     and _gold.e (netCDF-3 read with scipy): the deck parameters and the nine
     output frames of displacement, velocity, dilatation, weighted volume and
@@ -222,11 +225,46 @@ def wave_in_bar(ref):
     return out
 
 
+def precracked_plate(ref):
+    """PrecrackedPlate: Exodus hex mesh (640 elements) with a rectangular-plane
+    bond filter, one quasi-static load step to 1% strain (absolute tolerance
+    1.0); the gold frames at t = 0 and t = 1 with the .comp tolerances."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/regression/PrecrackedPlate")
+    f = netcdf_file(os.path.join(d, "PrecrackedPlate_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-12), tol_force=np.array(1.0),
+               tol_number_of_neighbors=np.array(1e-15), tol_dilatation=np.array(1e-13),
+               tol_default=np.array(1e-9))
+    for k, v in _exodus_frames(f, list(range(len(time)))).items():
+        out[k] = v
+    f.close()
+    return out
+
+
+def copy_decks(ref, here):
+    """The input decks and meshes of the WaveInBar and PrecrackedPlate
+    regression tests (data files, copied unchanged)."""
+    import shutil
+    dst = os.path.join(here, "decks")
+    os.makedirs(dst, exist_ok=True)
+    for rel in ("test/regression/WaveInBar/WaveInBar.xml", "test/regression/WaveInBar/WaveInBar.g",
+                "test/regression/PrecrackedPlate/PrecrackedPlate.xml",
+                "test/regression/PrecrackedPlate/Plate.g"):
+        shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, os.path.basename(rel)))
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     here = os.path.dirname(os.path.abspath(__file__))
     np.savez_compressed(os.path.join(here, "wave_in_bar.npz"), **wave_in_bar(ref))
     print("wrote wave_in_bar.npz")
+    np.savez_compressed(os.path.join(here, "precracked_plate.npz"), **precracked_plate(ref))
+    print("wrote precracked_plate.npz")
+    copy_decks(ref, here)
+    print("copied decks")
     fx = {
         "elastic_unit_tests.json": elastic_unit_tests(),
         "neighborhood_list_bar.json": neighborhood_list_bar(),

@@ -454,3 +454,21 @@ def test_qs_driver_linear_regime_closed_form():
     assert all(abs(s["alpha"] - 1.0) <= 1e-6 for s in st)
     # the ends hold the prescribed displacements
     assert np.allclose(u[right, 0], 0.5 * eps * n[0] * h, rtol=1e-12, atol=0)
+
+
+def test_precracked_plate_neighbour_counts_against_gold(golden_dir):
+    """PrecrackedPlate's gold Number_Of_Neighbors: the oracle's radius search
+    (<= R^2) over the Exodus points, then its finite-plane filter with the
+    deck's Rectangular_Plane, gives every point's count exactly."""
+    import os
+    from peridigm_amd import deck as Dk
+    from peridigm_amd import exodus as EX
+    g = np.load(os.path.join(golden_dir, "precracked_plate.npz"))
+    d = Dk.parse_deck(os.path.join(golden_dir, "decks", "PrecrackedPlate.xml"))
+    x, vol, blk, sets = EX.discretize(EX.read_mesh(os.path.join(golden_dir, "decks", "Plate.g")))
+    flt = list(d["Discretization"]["Bond Filters"].values())[0]
+    plane = np.array([[float(flt[k]) for k in Dk._PLANE_KEYS]])
+    nl = O.filter_planes(x, O.radius_search(x, 3.015), plane)
+    deg = O.degrees(nl, len(vol))
+    assert np.array_equal(deg, g["Number_Of_Neighbors"][1].astype(int))
+    assert deg.min() < deg.max()
