@@ -41,6 +41,12 @@ extern "C" {
 #define PDG_E_NONFINITE -4 /* NaN/Inf in a force evaluation */
 #define PDG_E_NOCONV -5    /* iterative solve did not converge */
 
+/* material models: "Elastic" (the linear peridynamic solid,
+ * src/materials/Peridigm_ElasticMaterial.cpp) and "Elastic Bond Based"
+ * (src/materials/Peridigm_ElasticBondBasedMaterial.cpp) */
+#define PDG_MATERIAL_ELASTIC 0
+#define PDG_MATERIAL_ELASTIC_BOND_BASED 1
+
 /* influence functions (src/core/Peridigm_InfluenceFunction.hpp:69-88) */
 #define PDG_INFLUENCE_ONE 0
 #define PDG_INFLUENCE_PARABOLIC_DECAY 1
@@ -93,6 +99,29 @@ int pdg_set_constants(pdg_ctx* ctx, const double* x, const double* volume,
  * must have been enabled by a positive criticalStretch in pdg_set_constants.
  * A bond i->j uses node i's value. */
 int pdg_set_critical_stretch(pdg_ctx* ctx, const double* criticalStretch);
+/* Material model for the next pdg_set_constants (default
+ * PDG_MATERIAL_ELASTIC). The bond-based model (pair force
+ * t = 0.5 (1 - d) c s with c = 18K/(pi delta^4),
+ * src/materials/elastic_bond_based.cxx:55-130) takes the bulk modulus only;
+ * planar = 1 (plane strain) or 2 (plane stress) uses c = 12K/(pi h delta^3)
+ * with thickness `height` (:80-84). Its dilatation output is zero and the
+ * quasi-static calls reject it. */
+int pdg_set_material_model(pdg_ctx* ctx, int model, int planar, double height);
+
+/* Critical stretch used by every damage-enabled node from the next damage
+ * evaluation on (> 0), or back to the per-node values (<= 0): the current
+ * value of "Time Dependent Critical Stretch"
+ * (src/damage/Peridigm_UserDefinedTimeDependentCriticalStretchDamageModel.cpp:93-131),
+ * which the caller evaluates at each step's time. */
+int pdg_set_critical_stretch_value(pdg_ctx* ctx, double criticalStretch);
+/* The same as a function of time for pdg_verlet_run: at each damage
+ * evaluation (time t0 + k dt) the critical stretch is the function's value,
+ * linear between the n points (times increasing), constant beyond them; a
+ * point time gives its value exactly (src/core/Peridigm.cpp:1447-1463
+ * evaluates it at every step). n = 0 removes it. */
+int pdg_set_critical_stretch_function(pdg_ctx* ctx, int n, const double* times,
+                                      const double* values);
+
 /* Density per node (N) for the explicit integrator. */
 int pdg_set_density(pdg_ctx* ctx, const double* density);
 

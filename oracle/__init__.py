@@ -39,6 +39,7 @@ def lib():
         L.orc_critical_time_step.restype = C.c_double
         L.orc_explicit_run.restype = C.c_int
         L.orc_explicit_run_sc.restype = C.c_int
+        L.orc_explicit_run_ext.restype = C.c_int
         L.orc_plane_cuts.restype = C.c_int
         _LIB = L
     return _LIB
@@ -272,6 +273,46 @@ def baseline_explicit_run(x, vol, rho, nlist, m, K, mu, sc, dt, nsteps, u0=None,
 
 def baseline_threads():
     return int(baseline_lib().bl_max_threads())
+
+
+def compute_force_pmb(x, y, vol, bond_damage, nlist, K, horizon, planar=0, height=0.0):
+    """Elastic bond-based force density (src/materials/elastic_bond_based.cxx:55-130)."""
+    x = _c(x, np.float64); y = _c(y, np.float64); vol = _c(vol, np.float64)
+    bd = _c(bond_damage, np.float64); nlist = _c(nlist, np.int32)
+    f = np.zeros((x.shape[0], 3))
+    lib().orc_compute_force_pmb(C.c_int(x.shape[0]), _p(x), _p(y), _p(vol), _p(bd), _p(nlist),
+                                C.c_double(K), C.c_double(horizon), C.c_int(planar),
+                                C.c_double(height), _p(f))
+    return f
+
+
+def explicit_run_ext(x, vol, rho, nlist, K, mu, horizon, sc, dt, nsteps, material=0, planar=0,
+                     height=0.0, sc_eval=None, u0=None, v0=None, bond_damage=None, bcs=(),
+                     t0=0.0, infl=0):
+    """explicit_run with a material model (0 elastic LPS, 1 elastic bond
+    based) and optional per-evaluation critical stretches sc_eval (nsteps+1)."""
+    x = _c(x, np.float64); vol = _c(vol, np.float64)
+    N = x.shape[0]
+    rho = _c(np.broadcast_to(rho, (N,)), np.float64)
+    nlist = _c(nlist, np.int32)
+    B = num_bonds(nlist, N)
+    u = np.zeros((N, 3)) if u0 is None else _c(u0, np.float64).copy()
+    v = np.zeros((N, 3)) if v0 is None else _c(v0, np.float64).copy()
+    bd = np.zeros(B) if bond_damage is None else _c(bond_damage, np.float64).copy()
+    a = np.zeros((N, 3)); th = np.zeros(N); f = np.zeros((N, 3)); dmg = np.zeros(N); m = np.zeros(N)
+    ptr, nodes, coord, c0, c1 = _bc_arrays(bcs)
+    sce = None if sc_eval is None else _c(sc_eval, np.float64)
+    if sce is not None:
+        assert sce.size == nsteps + 1
+    rc = lib().orc_explicit_run_ext(
+        C.c_int(N), _p(x), _p(vol), _p(rho), _p(nlist), C.c_int(material), C.c_int(planar),
+        C.c_double(height), C.c_double(K), C.c_double(mu), C.c_double(horizon), C.c_int(infl),
+        C.c_double(sc), None if sce is None else _p(sce), C.c_double(dt), C.c_double(t0),
+        C.c_int(nsteps), C.c_int(len(bcs)), _p(ptr), _p(nodes), _p(coord), _p(c0), _p(c1),
+        _p(u), _p(v), _p(a), _p(bd), _p(dmg), _p(th), _p(f), _p(m))
+    if rc != 0:
+        raise FloatingPointError("oracle explicit run: non-finite force")
+    return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f, m=m)
 
 
 def force_at(x, u, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):

@@ -144,6 +144,48 @@ void force_t(int numOwned, const double* x, const T* y, const double* m,
   }
 }
 
+// src/materials/elastic_bond_based.cxx:55-130 computeInternalForceElasticBondBased
+// (owner-side gather plus neighbour-side scatter; f zeroed first as in
+// src/materials/Peridigm_ElasticBondBasedMaterial.cpp:119). planar != 0: the
+// plane strain / plane stress constant with thickness `height`.
+void force_pmb(int numOwned, const double* x, const double* y, const double* vol,
+               const double* bondDamage, double* f, const int* nlist, double K, double horizon,
+               int planar, double height) {
+  const double pi = 3.14159265358979323846;
+  double constant = 0.0;
+  if (planar)
+    constant = 12.0 * K / (pi * height * horizon * horizon * horizon);
+  else
+    constant = 18.0 * K / (pi * horizon * horizon * horizon * horizon);
+  int li = 0, b = 0;
+  for (int p = 0; p < numOwned; p++) {
+    const double X0 = x[p * 3], X1 = x[p * 3 + 1], X2 = x[p * 3 + 2];
+    const double Y0 = y[p * 3], Y1 = y[p * 3 + 1], Y2 = y[p * 3 + 2];
+    const double volume = vol[p];
+    const int nn = nlist[li++];
+    for (int n = 0; n < nn; n++) {
+      const int j = nlist[li++];
+      const double nX0 = x[j * 3], nX1 = x[j * 3 + 1], nX2 = x[j * 3 + 2];
+      const double nY0 = y[j * 3], nY1 = y[j * 3 + 1], nY2 = y[j * 3 + 2];
+      const double neighborVolume = vol[j];
+      const double init = std::sqrt((nX0 - X0) * (nX0 - X0) + (nX1 - X1) * (nX1 - X1) +
+                                    (nX2 - X2) * (nX2 - X2));
+      const double cur = std::sqrt((nY0 - Y0) * (nY0 - Y0) + (nY1 - Y1) * (nY1 - Y1) +
+                                   (nY2 - Y2) * (nY2 - Y2));
+      const double stretch = (cur - init) / init;
+      const double dmg = bondDamage[b++];
+      const double t = 0.5 * (1.0 - dmg) * stretch * constant;
+      const double fx = t * (nY0 - Y0) / cur, fy = t * (nY1 - Y1) / cur, fz = t * (nY2 - Y2) / cur;
+      f[3 * p + 0] += fx * neighborVolume;
+      f[3 * p + 1] += fy * neighborVolume;
+      f[3 * p + 2] += fz * neighborVolume;
+      f[3 * j + 0] -= fx * volume;
+      f[3 * j + 1] -= fy * volume;
+      f[3 * j + 2] -= fz * volume;
+    }
+  }
+}
+
 // src/damage/Peridigm_CriticalStretchDamageModel.cpp:108-184 computeDamage
 // (bondDamageNP1 must hold bondDamageN on entry, as at :131).
 void damage_impl(int numOwned, const double* x, const double* y,
@@ -601,6 +643,66 @@ int orc_explicit_run_sc(int N, const double* x, const double* vol, const double*
     for (int i = 0; i < 3 * N; i++) y[i] = x[i] + u[i] + dt * v[i];
     for (int i = 0; i < 3 * N; i++) u[i] = u[i] + dt * v[i];
     eval();
+    for (int i = 0; i < 3 * N; i++) {
+      if (!std::isfinite(force[i])) return -1;
+    }
+    for (int i = 0; i < 3 * N; i++) { a[i] = force[i]; a[i] += 0.0; a[i] /= rho[i / 3]; }
+    for (int i = 0; i < 3 * N; i++) v[i] = v[i] + dt2 * a[i];
+  }
+  return 0;
+}
+
+// Bond-based force density (theta is not part of the model).
+void orc_compute_force_pmb(int N, const double* x, const double* y, const double* vol,
+                           const double* bondDamage, const int* nlist, double K, double horizon,
+                           int planar, double height, double* f) {
+  for (int i = 0; i < 3 * N; i++) f[i] = 0.0;
+  force_pmb(N, x, y, vol, bondDamage, f, nlist, K, horizon, planar, height);
+}
+
+// Explicit run as orc_explicit_run_sc with a material model (0: elastic
+// LPS, 1: elastic bond based) and optionally one critical stretch per force
+// evaluation (sc_eval[k] for the evaluation that ends step k, k = 0..nsteps:
+// "Time Dependent Critical Stretch" evaluated at every step,
+// src/core/Peridigm.cpp:1447-1463); sc > 0 enables damage.
+int orc_explicit_run_ext(int N, const double* x, const double* vol, const double* rho,
+                         const int* nlist, int material, int planar, double height, double K,
+                         double mu, double horizon, int infl, double sc, const double* sc_eval,
+                         double dt, double t0, int nsteps, int nbc, const int* bc_ptr,
+                         const int* bc_nodes, const int* bc_coord, const double* bc_c0,
+                         const double* bc_c1, double* u, double* v, double* a,
+                         double* bond_damage, double* damage, double* theta, double* force,
+                         double* m) {
+  BCs bc{nbc, bc_ptr, bc_nodes, bc_coord, bc_c0, bc_c1};
+  std::vector<double> y(3 * N);
+  orc_weighted_volume(N, x, vol, nlist, horizon, infl, m);
+  for (int i = 0; i < 3 * N; i++) y[i] = x[i] + u[i];
+  for (int i = 0; i < N; i++) damage[i] = 0.0;
+  auto eval = [&](int k) {
+    if (sc > 0.0)
+      damage_impl(N, x, y.data(), bond_damage, damage, nlist, sc_eval ? sc_eval[k] : sc, nullptr,
+                  0.0, nullptr);
+    if (material == 1) {
+      for (int i = 0; i < N; i++) theta[i] = 0.0;
+      orc_compute_force_pmb(N, x, y.data(), vol, bond_damage, nlist, K, horizon, planar, height,
+                            force);
+    } else {
+      orc_compute_force(N, x, y.data(), m, vol, bond_damage, nlist, K, mu, horizon, infl, theta,
+                        force);
+    }
+  };
+  eval(0);
+  for (int i = 0; i < 3 * N; i++) { a[i] = force[i]; a[i] += 0.0; a[i] /= rho[i / 3]; }
+  double dt2 = dt / 2.0;
+  double tc = t0, tp = t0;
+  for (int step = 1; step <= nsteps; step++) {
+    tp = tc;
+    tc = t0 + (step * dt);
+    for (int i = 0; i < 3 * N; i++) v[i] = v[i] + dt2 * a[i];
+    apply_velocity_bc(bc, tc, tp, u, v);
+    for (int i = 0; i < 3 * N; i++) y[i] = x[i] + u[i] + dt * v[i];
+    for (int i = 0; i < 3 * N; i++) u[i] = u[i] + dt * v[i];
+    eval(step);
     for (int i = 0; i < 3 * N; i++) {
       if (!std::isfinite(force[i])) return -1;
     }

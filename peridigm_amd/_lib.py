@@ -77,6 +77,9 @@ SIGNATURES = {
     "pdg_state_upload": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "pdg_state_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pdg_set_bcs": (_i, [_vp, C.POINTER(pdg_bc), _i]),
+    "pdg_set_material_model": (_i, [_vp, _i, _i, _d]),
+    "pdg_set_critical_stretch_value": (_i, [_vp, _d]),
+    "pdg_set_critical_stretch_function": (_i, [_vp, _i, _vp, _vp]),
     "pdg_qs_params_default": (_i, [C.POINTER(pdg_qs_params)]),
     "pdg_qs_force_dilatation": (_i, [_vp, _vp, _vp, _vp]),
     "pdg_qs_run": (_i, [_vp, C.POINTER(pdg_bc), _i, _vp, _i, C.POINTER(pdg_qs_params), _vp,

@@ -94,6 +94,26 @@ class Context:
         v = _f64(np.broadcast_to(np.asarray(sc, float), (self.N,)))
         self.check(self.lib.pdg_set_critical_stretch(self.h, _p(v)))
 
+    def set_material_model(self, model, planar=0, height=0.0):
+        """model: "Elastic" (0) or "Elastic Bond Based" (1); planar: 0 3D,
+        1 plane strain, 2 plane stress (bond-based only). Call before
+        set_constants."""
+        codes = {"Elastic": 0, "Elastic Bond Based": 1}
+        m = codes.get(model, model)
+        self.check(self.lib.pdg_set_material_model(self.h, int(m), int(planar), float(height)))
+
+    def set_critical_stretch_value(self, sc):
+        """Uniform critical stretch for damage-enabled nodes (<= 0: per-node values)."""
+        self.check(self.lib.pdg_set_critical_stretch_value(self.h, float(sc)))
+
+    def set_critical_stretch_function(self, times, values):
+        """Critical stretch as a function of time for explicit runs
+        (piecewise linear through the points; empty removes it)."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._sc_keep = (t, v)
+        self.check(self.lib.pdg_set_critical_stretch_function(self.h, t.size, _p(t), _p(v)))
+
     def set_density(self, rho):
         r = _f64(np.broadcast_to(np.asarray(rho, float), (self.N,)))
         self.check(self.lib.pdg_set_density(self.h, _p(r)))

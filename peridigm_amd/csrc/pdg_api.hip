@@ -139,9 +139,25 @@ hipError_t Ctx::pass1(NodeRec* rec, int* nbroken, bool damage, bool count_near) 
 BondArgs Ctx::bond_args() const {
   BondArgs a;
   a.n = N; a.row = d_row; a.col = d_col; a.mw = d_mw; a.mask = d_mask; a.nc = d_nc;
-  a.near_rel = 1e-6f; a.near_ctr = d_near; a.influence = infl;
+  a.near_rel = 1e-6f; a.near_ctr = d_near;
+  a.influence = material == PDG_MATERIAL_ELASTIC_BOND_BASED ? kPairModelPmb : infl;
   a.fq = std::ldexp(1.0f, -qexp);
+  a.pmb_c = pmb_c;
+  a.sc_now = sc_cur > 0.0 ? (float)sc_cur : 0.0f;
   return a;
+}
+
+// The critical-stretch function at time t: linear between its points,
+// constant beyond the ends; an exact point time gives its value.
+double Ctx::sc_at(double t) const {
+  const size_t n = sc_t.size();
+  if (n == 0) return sc_value;
+  if (t <= sc_t[0]) return sc_v[0];
+  if (t >= sc_t[n - 1]) return sc_v[n - 1];
+  const size_t k = (size_t)(std::upper_bound(sc_t.begin(), sc_t.end(), t) - sc_t.begin());
+  const double t0 = sc_t[k - 1], t1 = sc_t[k];
+  if (t == t0) return sc_v[k - 1];
+  return sc_v[k - 1] + (sc_v[k] - sc_v[k - 1]) * ((t - t0) / (t1 - t0));
 }
 
 TileArgs Ctx::tile_args() const {
@@ -730,7 +746,10 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
     NodeConst& q = nc[k];
     q.sc = sc > 0.0 ? (float)sc : 0.0f;
     q.hhat = (float)(horizon * invq);
-    if (m > 0.0) {
+    if (c->material == PDG_MATERIAL_ELASTIC_BOND_BASED) {
+      // no dilatation term: theta and p stay zero
+      r.alpha = 0.0f; q.ctheta = 0.0f; q.kp = 0.0f;
+    } else if (m > 0.0) {
       const double alpha = 15.0 * mu / m;
       const double beta = 3.0 * K / m - alpha / 3.0;
       const double ct = 3.0 * L * L * S / m;
@@ -740,6 +759,15 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
     } else {
       r.alpha = 0.0f; q.ctheta = 0.0f; q.kp = 0.0f;
     }
+  }
+  // bond-based pair constant (src/materials/elastic_bond_based.cxx:80-84) in
+  // the kernels' units: volumes scaled by S, lengths by q
+  c->pmb_c = 0.0f;
+  if (c->material == PDG_MATERIAL_ELASTIC_BOND_BASED) {
+    const double pi = 3.14159265358979323846;
+    const double cc = c->planar ? 12.0 * K / (pi * c->height * horizon * horizon * horizon)
+                                : 18.0 * K / (pi * horizon * horizon * horizon * horizon);
+    c->pmb_c = (float)std::ldexp(cc * S, c->qexp);
   }
   std::vector<int> noslot(N, -1);
   std::vector<double> rho1(N, 1.0);
@@ -794,6 +822,41 @@ int pdg_set_critical_stretch(pdg_ctx* h, const double* sc) {
   rc = c->hip(hipMemcpyAsync(c->d_nc, nc.data(), c->N * sizeof(NodeConst), hipMemcpyHostToDevice, c->stream), "h2d nc");
   if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
   return rc;
+}
+
+int pdg_set_material_model(pdg_ctx* h, int model, int planar, double height) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (model != PDG_MATERIAL_ELASTIC && model != PDG_MATERIAL_ELASTIC_BOND_BASED)
+    return c->fail(PDG_E_ARG, "unknown material model");
+  if (planar < 0 || planar > 2 || (planar && model != PDG_MATERIAL_ELASTIC_BOND_BASED))
+    return c->fail(PDG_E_ARG, "plane strain/stress is a variant of the bond-based model");
+  if (planar && !(height > 0.0)) return c->fail(PDG_E_ARG, "plane strain/stress needs a height");
+  c->material = model;
+  c->planar = planar;
+  c->height = height;
+  return PDG_OK;
+}
+
+int pdg_set_critical_stretch_value(pdg_ctx* h, double sc) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (!(sc <= 0.0) && !(sc > 0.0)) return c->fail(PDG_E_ARG, "critical stretch is NaN");
+  c->sc_value = sc > 0.0 ? sc : 0.0;
+  c->sc_cur = c->sc_value;
+  return PDG_OK;
+}
+
+int pdg_set_critical_stretch_function(pdg_ctx* h, int n, const double* times, const double* values) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || n < 0 || (n > 0 && (!times || !values))) return PDG_E_ARG;
+  for (int k = 0; k < n; k++) {
+    if (!(values[k] > 0.0)) return c->fail(PDG_E_ARG, "critical stretch values must be positive");
+    if (k > 0 && !(times[k] > times[k - 1])) return c->fail(PDG_E_ARG, "times must increase");
+  }
+  c->sc_t.assign(times, times + n);
+  c->sc_v.assign(values, values + n);
+  return PDG_OK;
 }
 
 int pdg_set_density(pdg_ctx* h, const double* rho) {
@@ -989,6 +1052,8 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   auto eval = [&](int k, bool end_kick) -> int {
     int r = 0;
     if (nev) r = c->hip(hipEventRecord(c->tev[3 * nevals], s), "event");
+    // a time-dependent critical stretch at the time this evaluation ends
+    c->sc_cur = c->sc_at(step_time(k));
     if (!r) r = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, damage, damage), "pass1");
     if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals + 1], s), "event");
     va.end_kick = end_kick ? 1 : 0;
@@ -1014,6 +1079,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
     c->cur ^= 1;
   }
   for (int k = 1; k <= nsteps && !rc; k++) rc = eval(k, true);
+  c->sc_cur = c->sc_value;
   if (rc) return rc;
   rc = c->hip(hipEventRecord(c->ev[3], s), "event");
   if (!rc) rc = c->hip(hipEventSynchronize(c->ev[3]), "sync");
@@ -1037,6 +1103,8 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
   if (!c->tiled()) return c->fail(PDG_E_UNSUPPORTED, "persistent run needs the tiled kernels");
+  if (!c->sc_t.empty())
+    return c->fail(PDG_E_UNSUPPORTED, "persistent run with a time-dependent critical stretch");
   const int tl = c->tile_lanes();
   if (tl != 4 && tl != 8 && tl != 16)
     return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 4, 8 or 16 lanes per node");

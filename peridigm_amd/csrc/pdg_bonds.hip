@@ -79,8 +79,9 @@ __global__ __launch_bounds__(256) void k_pass1(BondArgs a, NodeRec* __restrict__
       const BondGeom b = bond_geom_q<false>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
                                             r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]);
       if (DAMAGE) {
-        // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0)
-        const float lim = c.sc * b.zeta;
+        // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0); a time-dependent
+        // critical stretch replaces every damage-enabled node's
+        const float lim = ((a.sc_now > 0.0f && c.sc > 0.0f) ? a.sc_now : c.sc) * b.zeta;
         if (NEAR && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
         brk = brk || (b.ext > lim);
       }
@@ -149,9 +150,15 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
     const bool brk = ((word >> (k & 63)) & 1ull) != 0ull;
     const BondGeom b = bond_geom_q<true>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
                                          r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]);
-    const float w = omega_hat(a.influence, b.zeta, hhat);
-    float t = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
-    t = brk ? 0.0f : w * t * r.vol * b.ry;
+    float t;
+    if (a.influence == kPairModelPmb) {
+      // bond-based: (t_ij + t_ji) = c s with s = e / zeta
+      t = brk ? 0.0f : a.pmb_c * b.ext * frcp(b.zeta) * r.vol * b.ry;
+    } else {
+      const float w = omega_hat(a.influence, b.zeta, hhat);
+      t = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
+      t = brk ? 0.0f : w * t * r.vol * b.ry;
+    }
     f0 = fmaf(t, b.y0, f0);
     f1 = fmaf(t, b.y1, f1);
     f2 = fmaf(t, b.y2, f2);

@@ -39,6 +39,16 @@ struct Ctx {
 
   // constants
   double K = 0, mu = 0, horizon = 0, sc = 0, L = 1, S = 1;
+  // material model (PDG_MATERIAL_*), planar variant of the bond-based model
+  // (0: 3D, 1: plane strain, 2: plane stress) and its thickness
+  int material = PDG_MATERIAL_ELASTIC, planar = 0;
+  double height = 0.0;
+  float pmb_c = 0.0f;
+  // critical stretch applied to every damage-enabled node (> 0), and its
+  // piecewise-linear time function for explicit runs (empty: none)
+  double sc_value = 0.0, sc_cur = 0.0;
+  std::vector<double> sc_t, sc_v;
+  double sc_at(double t) const;
   // position quantum of the bond kernels: q = L * 2^-qexp (records hold
   // u / q and the fixed-point x / q)
   double q = 1;

@@ -13,7 +13,7 @@ __device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v)
 // Influence function (src/core/Peridigm_InfluenceFunction.hpp:69-88) of a
 // bond length and horizon given in the same unit.
 __device__ __forceinline__ float omega_hat(int kind, float zeta, float hhat) {
-  if (kind == PDG_INFLUENCE_ONE) return 1.0f;
+  if (kind == PDG_INFLUENCE_ONE || kind == kPairModelPmb) return 1.0f;
   if (kind == PDG_INFLUENCE_PARABOLIC_DECAY) {
     if (zeta > hhat) return 0.0f;
     float s = zeta / hhat;

@@ -66,9 +66,16 @@ struct BondArgs {
   const NodeConst* nc;         // per-node constants
   float near_rel;              // near-critical band (relative)
   unsigned long long* near_ctr;
-  int influence;               // PDG_INFLUENCE_*
+  int influence;               // PDG_INFLUENCE_*, or kPairModelPmb for the bond-based model
   float fq;                    // 2^-qexp: pass-2 FP32 force sums are in units of 1/q
+  float pmb_c;                 // bond-based model: c * S * 2^qexp (c = 18K/(pi delta^4) in 3D)
+  float sc_now;                // > 0: the critical stretch of every damage-enabled node
 };
+
+// The bond kernels' influence-function slot doubles as the pair-model switch:
+// this value selects the elastic bond-based (PMB) force, whose bonds carry no
+// influence function and no dilatation term.
+constexpr int kPairModelPmb = 3;
 
 struct VerletArgs {
   const double* rho;  // n

@@ -539,6 +539,8 @@ static QsArgs args(Ctx* c) {
 }
 
 static int ensure(Ctx* c) {
+  if (c->material != PDG_MATERIAL_ELASTIC)
+    return c->fail(PDG_E_UNSUPPORTED, "the quasi-static path implements the elastic (LPS) model");
   if (c->qs && c->qs->n == c->N) return PDG_OK;
   if (c->qs) c->free_extra();
   QsState* S = new QsState();

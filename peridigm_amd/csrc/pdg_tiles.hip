@@ -279,7 +279,8 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     unsigned long long mn[NWORDS];
 #pragma unroll
     for (int w = 0; w < NWORDS; w++) mn[w] = 0ull;
-    const float scz = c.sc;
+    // a time-dependent critical stretch replaces every damage-enabled node's
+    const float scz = (a.sc_now > 0.0f && c.sc > 0.0f) ? a.sc_now : c.sc;
     const int nmax = groups_max<G>((deg + G - 1) / G);
     float acc = 0.0f;
     int nnear = 0;               // plain loops: per lane
@@ -456,9 +457,15 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
           const unsigned long long okm = intact_lanes<G>(mo, it, l);
           const BondGeom b = bond_geom_q<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
                                                cur.x2, cur.u0, cur.u1, cur.u2);
-          const float w = omega_t<INFL>(b.zeta, hhat);
-          float tt = fmaf(b.zeta, p_i + cur.p, b.ext * (alpha_i + cur.alpha));
-          tt = w * tt * cur.vol * b.ry;
+          float tt;
+          if (INFL == kPairModelPmb) {
+            // bond-based: (t_ij + t_ji) = c s with s = e / zeta
+            tt = a.pmb_c * b.ext * frcp(b.zeta) * cur.vol * b.ry;
+          } else {
+            const float w = omega_t<INFL>(b.zeta, hhat);
+            tt = fmaf(b.zeta, p_i + cur.p, b.ext * (alpha_i + cur.alpha));
+            tt = w * tt * cur.vol * b.ry;
+          }
           tt = __builtin_amdgcn_inverse_ballot_w64(okm) ? tt : 0.0f;
           f0 = fmaf(tt, b.y0, f0);
           f1 = fmaf(tt, b.y1, f1);
@@ -474,9 +481,14 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
         const Nb r = nb_load<true>(L, lj[it]);
         const BondGeom b = bond_geom_q<true>(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2,
                                              r.u0, r.u1, r.u2);
-        const float w = omega_t<INFL>(b.zeta, hhat);
-        float tt = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
-        tt = brk ? 0.0f : w * tt * r.vol * b.ry;
+        float tt;
+        if (INFL == kPairModelPmb) {
+          tt = brk ? 0.0f : a.pmb_c * b.ext * frcp(b.zeta) * r.vol * b.ry;
+        } else {
+          const float w = omega_t<INFL>(b.zeta, hhat);
+          tt = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
+          tt = brk ? 0.0f : w * tt * r.vol * b.ry;
+        }
         f0 = fmaf(tt, b.y0, f0);
         f1 = fmaf(tt, b.y1, f1);
         f2 = fmaf(tt, b.y2, f2);
@@ -595,7 +607,8 @@ void p2_pipe(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* 
 template <int G>
 void p1_infl(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, int* nbroken,
              bool damage, bool count_near, bool pipe, hipStream_t s) {
-  if (a.influence == PDG_INFLUENCE_ONE) p1_pipe<G, 0>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
+  if (a.influence == PDG_INFLUENCE_ONE || a.influence == kPairModelPmb)
+    p1_pipe<G, 0>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p1_pipe<G, 1>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
   else p1_pipe<G, 2>(a, t, rec, theta, nbroken, damage, count_near, pipe, s);
 }
@@ -603,7 +616,8 @@ void p1_infl(const BondArgs& a, const TileArgs& t, NodeRec* rec, float* theta, i
 template <int G, bool VERLET>
 void p2_infl(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* rec_next, double* f,
              const VerletArgs& va, bool pipe, hipStream_t s) {
-  if (a.influence == PDG_INFLUENCE_ONE) p2_pipe<G, VERLET, 0>(a, t, rec, rec_next, f, va, pipe, s);
+  if (a.influence == kPairModelPmb) p2_pipe<G, VERLET, kPairModelPmb>(a, t, rec, rec_next, f, va, pipe, s);
+  else if (a.influence == PDG_INFLUENCE_ONE) p2_pipe<G, VERLET, 0>(a, t, rec, rec_next, f, va, pipe, s);
   else if (a.influence == PDG_INFLUENCE_PARABOLIC_DECAY) p2_pipe<G, VERLET, 1>(a, t, rec, rec_next, f, va, pipe, s);
   else p2_pipe<G, VERLET, 2>(a, t, rec, rec_next, f, va, pipe, s);
 }
@@ -613,6 +627,8 @@ void p2_infl(const BondArgs& a, const TileArgs& t, const NodeRec* rec, NodeRec* 
 namespace {
 template <int G>
 void* persistent_fn(int influence, bool damage) {
+  if (influence == kPairModelPmb)
+    return damage ? (void*)k_persistent<G, true, kPairModelPmb> : (void*)k_persistent<G, false, kPairModelPmb>;
   if (influence == PDG_INFLUENCE_ONE)
     return damage ? (void*)k_persistent<G, true, 0> : (void*)k_persistent<G, false, 0>;
   if (influence == PDG_INFLUENCE_PARABOLIC_DECAY)

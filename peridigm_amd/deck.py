@@ -9,13 +9,19 @@ output frames (src/core/Peridigm.cpp:1290-1593 and :2470-3043 for the
 solvers, src/io/Peridigm_OutputManager_ExodusII.cpp:210-470 for the output).
 
 Supported subset (anything else raises DeckError):
-  - boundary-condition values are expressions of x, y, z and t (C syntax:
-    numbers, + - * /, parentheses, and sin cos tan exp log sqrt fabs pow);
+  - materials "Elastic" and "Elastic Bond Based" (3D, plane strain, plane
+    stress); damage models "Critical Stretch" and "Time Dependent Critical
+    Stretch";
+  - boundary-condition and damage values are C-like statements of x, y, z
+    and t (`value = expr;`, if / else if / else with comparisons, && || !;
+    expressions with numbers, + - * /, parentheses, and sin cos tan exp log
+    sqrt fabs pow);
     prescribed displacements must be linear in t (u = c1(x) t + c0(x)), as the
     device integrator applies them;
   - elastic constants: any two of bulk, shear, Young's modulus and Poisson's
     ratio (src/materials/Peridigm_Material.cpp:335-433);
-  - one block; influence function "One", "Parabolic Decay" or "Gaussian".
+  - one block group (all the mesh's blocks, one material and horizon);
+    influence function "One", "Parabolic Decay" or "Gaussian".
 """
 from __future__ import annotations
 
@@ -58,8 +64,29 @@ def _plist(elem):
     return out
 
 
+def _numbers(d):
+    """YAML 1.1 reads 2.0e9 (no exponent sign) as a string: numbers back."""
+    out = {}
+    for k, v in d.items():
+        if isinstance(v, dict):
+            v = _numbers(v)
+        elif isinstance(v, str):
+            try:
+                v = float(v) if any(ch in v for ch in ".eE") else int(v)
+            except ValueError:
+                pass
+        out[k] = v
+    return out
+
+
 def parse_deck(path):
-    """The deck as nested dicts (ParameterList -> dict, Parameter -> value)."""
+    """The deck as nested dicts: an XML ParameterList (ParameterList -> dict,
+    Parameter -> typed value) or a YAML deck (its "Peridigm" mapping)."""
+    if path.endswith((".yaml", ".yml")):
+        import yaml
+        with open(path) as fh:
+            d = yaml.safe_load(fh)
+        return _numbers(d.get("Peridigm", d))
     return _plist(ET.parse(path).getroot())
 
 
@@ -105,12 +132,129 @@ def evaluate(expr, x, y, z, t):
     return np.broadcast_to(np.asarray(ev(tree), dtype=np.float64), np.shape(x)).copy()
 
 
+def _tokens(code):
+    import re
+    tok = re.findall(r"\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?|[A-Za-z_]\w*|"
+                     r"&&|\|\||<=|>=|==|!=|[-+*/(){};=<>!,]", code)
+    return tok
+
+
+def evaluate_program(code, x, y, z, t):
+    """Value of a boundary-condition or damage function written as C-like
+    statements: `value = expr;` and `if (cond) { ... } else { ... }` (else if
+    chains included), with comparisons, && || ! in the conditions. Each
+    expression and condition goes through the ast-based evaluator."""
+    s = code.replace("\\{", "{").replace("\\}", "}").replace("\\", "")
+    if "if" not in _tokens(s):
+        return evaluate(s, x, y, z, t)
+    toks = _tokens(s)
+    pos = [0]
+
+    def peek():
+        return toks[pos[0]] if pos[0] < len(toks) else None
+
+    def take(v=None):
+        tk = peek()
+        if v is not None and tk != v:
+            raise DeckError("expected %r in %r" % (v, code))
+        pos[0] += 1
+        return tk
+
+    def until(stop):
+        depth, out = 0, []
+        while True:
+            tk = peek()
+            if tk is None:
+                raise DeckError("unterminated statement in %r" % code)
+            if depth == 0 and tk in stop:
+                return " ".join(out)
+            if tk == "(":
+                depth += 1
+            elif tk == ")":
+                depth -= 1
+            out.append(tk)
+            take()
+
+    state = {"value": np.zeros(np.shape(x))}
+
+    def cond_value(txt):
+        py = txt.replace("&&", " and ").replace("||", " or ").replace("!", " not ").replace(" not =", "!=")
+        return _eval_condition(py, x, y, z, t)
+
+    def stmt(active):
+        tk = peek()
+        if tk == "if":
+            take("if"); take("(")
+            c = cond_value(until(")"))
+            take(")")
+            block(active & c)
+            if peek() == "else":
+                take("else")
+                if peek() == "if":
+                    stmt(active & ~c)
+                else:
+                    block(active & ~c)
+        elif tk == "value":
+            take("value"); take("=")
+            v = evaluate(until(";"), x, y, z, t)
+            take(";")
+            state["value"] = np.where(active, v, state["value"])
+        elif tk == ";":
+            take(";")
+        else:
+            raise DeckError("unsupported statement in %r" % code)
+
+    def block(active):
+        if peek() == "{":
+            take("{")
+            while peek() != "}":
+                stmt(active)
+            take("}")
+        else:
+            stmt(active)
+
+    active = np.ones(np.shape(x), bool)
+    while peek() is not None:
+        stmt(active)
+    return state["value"]
+
+
+_CMP = {ast.Lt: np.less, ast.LtE: np.less_equal, ast.Gt: np.greater, ast.GtE: np.greater_equal,
+        ast.Eq: np.equal, ast.NotEq: np.not_equal}
+
+
+def _eval_condition(txt, x, y, z, t):
+    try:
+        tree = ast.parse(txt.strip(), mode="eval")
+    except SyntaxError as e:
+        raise DeckError("cannot parse condition %r: %s" % (txt, e))
+
+    def ev(n):
+        if isinstance(n, ast.Expression):
+            return ev(n.body)
+        if isinstance(n, ast.BoolOp):
+            vals = [ev(v) for v in n.values]
+            out = vals[0]
+            for v in vals[1:]:
+                out = (out & v) if isinstance(n.op, ast.And) else (out | v)
+            return out
+        if isinstance(n, ast.UnaryOp) and isinstance(n.op, ast.Not):
+            return ~ev(n.operand)
+        if isinstance(n, ast.Compare) and len(n.ops) == 1 and type(n.ops[0]) in _CMP:
+            a = evaluate(ast.unparse(n.left), x, y, z, t)
+            b = evaluate(ast.unparse(n.comparators[0]), x, y, z, t)
+            return _CMP[type(n.ops[0])](a, b)
+        raise DeckError("unsupported condition %r" % txt)
+
+    return np.broadcast_to(ev(tree), np.shape(x)).copy()
+
+
 def linear_in_t(expr, x, y, z):
     """(c0, c1) with value = c1 t + c0 at every point, checked at t = 0.5, 2."""
-    c0 = evaluate(expr, x, y, z, 0.0)
-    c1 = evaluate(expr, x, y, z, 1.0) - c0
+    c0 = evaluate_program(expr, x, y, z, 0.0)
+    c1 = evaluate_program(expr, x, y, z, 1.0) - c0
     for t in (0.5, 2.0):
-        v = evaluate(expr, x, y, z, t)
+        v = evaluate_program(expr, x, y, z, t)
         if not np.allclose(v, c0 + c1 * t, rtol=1e-12, atol=1e-300 + 1e-14 * np.abs(c0 + c1 * t).max()):
             raise DeckError("prescribed displacement %r is not linear in t" % expr)
     return c0, c1
@@ -170,10 +314,8 @@ def discretize(deck, base_dir, horizon):
     if typ == "Exodus":
         mesh = EX.read_mesh(os.path.join(base_dir, d["Input Mesh File"]))
         x, vol, blk, sets = EX.discretize(mesh)
-        if len(mesh["blocks"]) != 1:
-            raise DeckError("one block is supported")
         nl = radius_search(x, horizon, planes)
-        return x, vol, nl, sets
+        return x, vol, nl, sets, [b[0] for b in mesh["blocks"]]
     if typ == "PdQuickGrid":
         g = d["TensorProduct3DMeshGenerator"]
         n = [g["Number Points X"], g["Number Points Y"], g["Number Points Z"]]
@@ -183,7 +325,7 @@ def discretize(deck, base_dir, horizon):
             raise DeckError("PdQuickGrid does not take bond filters (as in the reference)")
         sph = d.get("NeighborhoodType", "Parallelepiped") == "Spherical"
         x, vol, nl = quickgrid(n, o, L, horizon, spherical=sph)
-        return x, vol, nl, {}
+        return x, vol, nl, {}, ["block_1"]
     raise DeckError("discretization type %r not supported" % typ)
 
 
@@ -197,7 +339,13 @@ def _bcs(deck, x, sets):
     v0 = np.zeros((N, 3))
     u0 = np.zeros((N, 3))
     presc = []
-    for name, bc in deck.get("Boundary Conditions", {}).items():
+    sets = dict(sets)
+    bcl = deck.get("Boundary Conditions", {})
+    # node sets listed in the deck itself: 1-based ids, space separated
+    for name, val in bcl.items():
+        if not isinstance(val, dict) and name.startswith("Node Set"):
+            sets[name] = np.array([int(tk) - 1 for tk in str(val).split()], np.int64)
+    for name, bc in bcl.items():
         if not isinstance(bc, dict):
             continue
         typ = bc.get("Type")
@@ -208,11 +356,11 @@ def _bcs(deck, x, sets):
         c = _COORD[bc["Coordinate"].strip().lower()]
         px, py, pz = x[ids, 0], x[ids, 1], x[ids, 2]
         if typ == "Initial Velocity":
-            v0[ids, c] = evaluate(bc["Value"], px, py, pz, 0.0)
+            v0[ids, c] = evaluate_program(str(bc["Value"]), px, py, pz, 0.0)
         elif typ == "Initial Displacement":
-            u0[ids, c] = evaluate(bc["Value"], px, py, pz, 0.0)
+            u0[ids, c] = evaluate_program(str(bc["Value"]), px, py, pz, 0.0)
         elif typ == "Prescribed Displacement":
-            c0, c1 = linear_in_t(bc["Value"], px, py, pz)
+            c0, c1 = linear_in_t(str(bc["Value"]), px, py, pz)
             pairs = np.stack([c0, c1], 1)
             uniq, inv = np.unique(pairs, axis=0, return_inverse=True)
             for k, (a0, a1) in enumerate(uniq):
@@ -228,35 +376,58 @@ def run_deck(path, output=None, lanes=4):
     frames there in ExodusII."""
     deck = parse_deck(path)
     base = os.path.dirname(os.path.abspath(path))
-    blocks = list(deck["Blocks"].values())
-    if len(blocks) != 1:
-        raise DeckError("one block is supported")
-    blk = blocks[0]
+    groups = list(deck["Blocks"].values())
+    if len(groups) != 1:
+        raise DeckError("one block group (one material and horizon) is supported")
+    blk = groups[0]
     horizon = float(blk["Horizon"])
     mat = deck["Materials"][blk["Material"]]
-    if mat.get("Material Model") != "Elastic":
-        raise DeckError("material model %r not supported" % mat.get("Material Model"))
-    K, mu = elastic_constants(mat)
+    model = mat.get("Material Model")
+    planar, height = 0, 0.0
+    if model == "Elastic":
+        K, mu = elastic_constants(mat)
+    elif model == "Elastic Bond Based":
+        # one constant; the second is Poisson's ratio 1/4 (1/3 in plane
+        # stress), src/materials/Peridigm_ElasticBondBasedMaterial.cpp:59-82
+        planar = 1 if mat.get("Plane Strain", False) else (2 if mat.get("Plane Stress", False) else 0)
+        height = float(mat.get("Height", 0.0))
+        one = {k: mat[k] for k in ("Bulk Modulus", "Young's Modulus") if k in mat}
+        if len(one) != 1 or "Shear Modulus" in mat or "Poisson's Ratio" in mat:
+            raise DeckError("the bond-based model takes only the bulk or Young's modulus")
+        one["Poisson's Ratio"] = 1.0 / 3.0 if planar == 2 else 0.25
+        one["Plane Stress"] = planar == 2
+        K, mu = elastic_constants(one)
+    else:
+        raise DeckError("material model %r not supported" % model)
     rho = float(mat["Density"])
     infl = INFLUENCE[blk.get("Influence Function", "One")]
-    sc = 0.0
+    sc, sc_fn = 0.0, None
     if "Damage Model" in blk:
         dm = deck["Damage Models"][blk["Damage Model"]]
-        if dm.get("Damage Model") != "Critical Stretch":
+        if dm.get("Damage Model") == "Critical Stretch":
+            sc = float(dm["Critical Stretch"])
+        elif dm.get("Damage Model") == "Time Dependent Critical Stretch":
+            sc_fn = str(dm["Time Dependent Critical Stretch"])
+            sc = 1.0  # enables the damage model; the function sets the value
+        else:
             raise DeckError("damage model %r not supported" % dm.get("Damage Model"))
-        sc = float(dm["Critical Stretch"])
-    x, vol, nl, sets = discretize(deck, base, horizon)
+    x, vol, nl, sets, mesh_blocks = discretize(deck, base, horizon)
+    named = str(blk.get("Block Names", "")).split()
+    if named and set(named) != set(mesh_blocks):
+        raise DeckError("the block group must hold every block of the mesh (%s)" % mesh_blocks)
     N = x.shape[0]
     v0, u0, presc = _bcs(deck, x, sets)
     solver = deck["Solver"]
     t0 = float(solver.get("Initial Time", 0.0))
     t1 = float(solver.get("Final Time", 1.0))
-    out = deck.get("Output", {})
+    out = deck.get("Output", deck.get("Output1", {}))
     freq = int(out.get("Output Frequency", 1))
     want = {k for k, v in out.get("Output Variables", {}).items() if v}
     ctx = Context(0, lanes=lanes)
     try:
         ctx.set_topology(nl, N)
+        if model == "Elastic Bond Based":
+            ctx.set_material_model(model, planar, height)
         ctx.set_constants(x, vol, K, mu, horizon, infl, sc)
         m = ctx.initialize()
         deg = _degrees(nl, N)
@@ -288,6 +459,12 @@ def run_deck(path, output=None, lanes=4):
             dt = vp["Fixed dt"] if "Fixed dt" in vp else ctx.critical_time_step(rho)
             dt *= vp.get("Safety Factor", 1.0)
             nsteps = int(math.floor((t1 - t0) / dt))
+            if sc_fn is not None:
+                # the damage function at every evaluation time t0 + k dt
+                ts = np.array([t0 + (k * dt) for k in range(nsteps + 1)])
+                vals = np.array([float(evaluate_program(sc_fn, np.zeros(1), np.zeros(1),
+                                                        np.zeros(1), tk)[0]) for tk in ts])
+                ctx.set_critical_stretch_function(ts, vals)
             ctx.set_density(np.full(N, rho))
             ctx.set_bcs(presc)
             ctx.upload_state(u=u0, v=v0)

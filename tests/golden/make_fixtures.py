@@ -13,6 +13,10 @@ Sources (paths relative to the reference root):
   * src/io/discretization/unit_test/utPeridigm_PdQuickGridDiscretization.cpp:70-201
     (2x2x2 lattice, horizon 0.501, spherical, 3 neighbours, centres, volume).
   * src/io/pdneigh/unit_test/utFinitePlane.cxx:52-104 (plane/bond intersection).
+  * test/verification/ElasticBondBased/ElasticBondBased3D/ElasticBondBased3D_gold.e
+    and test/verification/BondBreakingInitialVelocity_TimeDependentCS/
+    BondBreakingInitialVelocity_gold.e: their output frames (element variables
+    of all blocks concatenated), decks and meshes copied to decks/.
   * test/regression/PrecrackedPlate/PrecrackedPlate_gold.e: the two output
     frames of the quasi-static run, with the .comp tolerances; the WaveInBar
     and PrecrackedPlate decks and meshes are copied to decks/ unchanged.
@@ -244,16 +248,69 @@ def precracked_plate(ref):
     return out
 
 
+def _exodus_all_blocks(f, frames):
+    """Like _exodus_frames, with element variables concatenated over blocks."""
+    def names(k):
+        return [b"".join(r).strip(b"\x00").decode() for r in f.variables[k][:]]
+    out = {}
+    for i, nme in enumerate(names("name_nod_var")):
+        out[nme] = np.asarray(f.variables["vals_nod_var%d" % (i + 1)][:])[frames]
+    nblk = f.dimensions["num_el_blk"]
+    if "name_elem_var" in f.variables:
+        for i, nme in enumerate(names("name_elem_var")):
+            out[nme] = np.concatenate(
+                [np.asarray(f.variables["vals_elem_var%deb%d" % (i + 1, b + 1)][:])[frames]
+                 for b in range(nblk)], axis=1)
+    return out
+
+
+def elastic_bond_based_3d(ref):
+    """ElasticBondBased3D (verification): 384 hex points in three blocks of
+    the elastic bond-based material, every point moved by the prescribed
+    displacement u_y = 0.001 y t / 1e-8; five Verlet steps of 2e-9, six frames."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/ElasticBondBased/ElasticBondBased3D")
+    f = netcdf_file(os.path.join(d, "ElasticBondBased3D_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-9), tol_velocity=np.array(1e-9), tol_force=np.array(1e-3))
+    out.update(_exodus_all_blocks(f, list(range(len(time)))))
+    f.close()
+    return out
+
+
+def bond_breaking_time_dependent(ref):
+    """BondBreakingInitialVelocity_TimeDependentCS: the four-point deck with a
+    critical stretch of 50 until t = 7e-5 and 0.015 after; 11 frames."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/BondBreakingInitialVelocity_TimeDependentCS")
+    f = netcdf_file(os.path.join(d, "BondBreakingInitialVelocity_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-9), tol_velocity=np.array(1e-9), tol_damage=np.array(1e-3),
+               tol_weighted_volume=np.array(1e-12), tol_dilatation=np.array(1e-12))
+    out.update(_exodus_all_blocks(f, list(range(len(time)))))
+    f.close()
+    return out
+
+
 def copy_decks(ref, here):
     """The input decks and meshes of the WaveInBar and PrecrackedPlate
     regression tests (data files, copied unchanged)."""
     import shutil
     dst = os.path.join(here, "decks")
     os.makedirs(dst, exist_ok=True)
-    for rel in ("test/regression/WaveInBar/WaveInBar.xml", "test/regression/WaveInBar/WaveInBar.g",
-                "test/regression/PrecrackedPlate/PrecrackedPlate.xml",
-                "test/regression/PrecrackedPlate/Plate.g"):
-        shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, os.path.basename(rel)))
+    for rel, name in (("test/regression/WaveInBar/WaveInBar.xml", None),
+                      ("test/regression/WaveInBar/WaveInBar.g", None),
+                      ("test/regression/PrecrackedPlate/PrecrackedPlate.xml", None),
+                      ("test/regression/PrecrackedPlate/Plate.g", None),
+                      ("test/verification/ElasticBondBased/ElasticBondBased3D/ElasticBondBased3D.yaml", None),
+                      ("test/verification/ElasticBondBased/ElasticBondBased3D/ElasticBondBased3D.g", None),
+                      ("test/verification/BondBreakingInitialVelocity_TimeDependentCS/"
+                       "BondBreakingInitialVelocity.xml", "BondBreakingInitialVelocity_TimeDependentCS.xml")):
+        shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, name or os.path.basename(rel)))
 
 
 def main():
@@ -263,6 +320,10 @@ def main():
     print("wrote wave_in_bar.npz")
     np.savez_compressed(os.path.join(here, "precracked_plate.npz"), **precracked_plate(ref))
     print("wrote precracked_plate.npz")
+    np.savez_compressed(os.path.join(here, "elastic_bond_based_3d.npz"), **elastic_bond_based_3d(ref))
+    np.savez_compressed(os.path.join(here, "bond_breaking_time_dependent.npz"),
+                        **bond_breaking_time_dependent(ref))
+    print("wrote elastic_bond_based_3d.npz, bond_breaking_time_dependent.npz")
     copy_decks(ref, here)
     print("copied decks")
     fx = {

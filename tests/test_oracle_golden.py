@@ -472,3 +472,75 @@ def test_precracked_plate_neighbour_counts_against_gold(golden_dir):
     deg = O.degrees(nl, len(vol))
     assert np.array_equal(deg, g["Number_Of_Neighbors"][1].astype(int))
     assert deg.min() < deg.max()
+
+
+def _deck_setup(name):
+    """Points, neighbour list and conditions of a committed reference deck
+    (host-side parsing and discretization of the deck runner)."""
+    import os
+    from peridigm_amd import deck as Dk
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks", name)
+    d = Dk.parse_deck(path)
+    blk = list(d["Blocks"].values())[0]
+    hz = float(blk["Horizon"])
+    disc = d["Discretization"]
+    if disc["Type"] == "Exodus":
+        from peridigm_amd import exodus as EX
+        x, vol, _, sets = EX.discretize(EX.read_mesh(os.path.join(os.path.dirname(path),
+                                                                   disc["Input Mesh File"])))
+        nl = O.radius_search(x, hz)
+    else:
+        g = disc["TensorProduct3DMeshGenerator"]
+        n = [g["Number Points X"], g["Number Points Y"], g["Number Points Z"]]
+        x, vol, nl = O.quickgrid(*n, [g["X Origin"], g["Y Origin"], g["Z Origin"]],
+                                 [g["X Length"], g["Y Length"], g["Z Length"]], hz,
+                                 spherical=disc.get("NeighborhoodType") == "Spherical")
+        sets = {}
+    v0, u0, presc = Dk._bcs(d, x, sets)
+    return d, x, vol, nl, hz, v0, u0, presc
+
+
+def test_elastic_bond_based_3d_oracle_against_gold(golden_dir):
+    """ElasticBondBased3D (verification deck): the oracle's bond-based force
+    (src/materials/elastic_bond_based.cxx:55-130) under the deck's prescribed
+    motion reproduces the six gold frames of displacement, velocity and force
+    at the .comp tolerances."""
+    import os
+    g = np.load(os.path.join(golden_dir, "elastic_bond_based_3d.npz"))
+    d, x, vol, nl, hz, v0, u0, presc = _deck_setup("ElasticBondBased3D.yaml")
+    assert np.abs(x - g["coords"]).max() <= 1e-12
+    assert np.array_equal(O.degrees(nl, len(vol)), g["Number_Of_Neighbors"][0].astype(int))
+    K = d["Materials"]["My Elastic Bond Based Material"]["Bulk Modulus"]
+    dt = d["Solver"]["Verlet"]["Fixed dt"]
+    for k in range(len(g["time"])):
+        r = O.explicit_run_ext(x, vol, 7800.0, nl, K, 0.0, hz, 0.0, dt, k, material=1, bcs=presc)
+        F = r["force"] * vol[:, None]
+        for c, ax in enumerate("XYZ"):
+            assert np.abs(r["u"][:, c] - g["Displacement" + ax][k]).max() <= float(g["tol_displacement"])
+            assert np.abs(r["v"][:, c] - g["Velocity" + ax][k]).max() <= float(g["tol_velocity"])
+            assert np.abs(F[:, c] - g["Force" + ax][k]).max() <= float(g["tol_force"])
+
+
+def test_time_dependent_critical_stretch_oracle_against_gold(golden_dir):
+    """BondBreakingInitialVelocity_TimeDependentCS: with the critical stretch
+    evaluated at every step time (50 until t <= 7e-5, then 0.015), the oracle
+    keeps every bond until step 7000 and breaks them all there, as the gold
+    damage frames show; displacements and velocities at the .comp tolerances."""
+    import os
+    from peridigm_amd import deck as Dk
+    g = np.load(os.path.join(golden_dir, "bond_breaking_time_dependent.npz"))
+    d, x, vol, nl, hz, v0, u0, presc = _deck_setup("BondBreakingInitialVelocity_TimeDependentCS.xml")
+    mat = list(d["Materials"].values())[0]
+    fn = list(d["Damage Models"].values())[0]["Time Dependent Critical Stretch"]
+    dt = d["Solver"]["Verlet"]["Fixed dt"]
+    every = d["Output"]["Output Frequency"]
+    z = np.zeros(1)
+    for f in range(len(g["time"])):
+        n = f * every
+        sce = np.array([float(Dk.evaluate_program(fn, z, z, z, 0.0 + (k * dt))[0]) for k in range(n + 1)])
+        r = O.explicit_run_ext(x, vol, mat["Density"], nl, mat["Bulk Modulus"], mat["Shear Modulus"],
+                               hz, 1.0, dt, n, sc_eval=sce, v0=v0)
+        assert np.array_equal(r["damage"], g["Damage"][f]), f
+        for c, ax in enumerate("XYZ"):
+            assert np.abs(r["u"][:, c] - g["Displacement" + ax][f]).max() <= float(g["tol_displacement"])
+            assert np.abs(r["v"][:, c] - g["Velocity" + ax][f]).max() <= float(g["tol_velocity"])
