@@ -295,6 +295,7 @@ def main():
     ctx.upload_state(v=p["v0"])
     ntiles, halo_total = ctx.tiling()
     tile_nodes, lanes_run = ctx.tile_config()
+    lds_groups, lds_before, lds_after = ctx.lds_schedule()
     dt = p["dt"]
 
     # warmup: initial force evaluation + W steps
@@ -362,6 +363,7 @@ def main():
             "kernels": ("untiled" if (args.variant == 1 or ntiles == 0) else
                         "lds-tiled" + (", plain loops" if args.variant == 2 else ", pipelined loops")),
             "tiles": ntiles, "tile_nodes": tile_nodes, "halo_per_node": halo_total / N,
+            "lds_conflict_cost": [lds_before / max(lds_groups, 1), lds_after / max(lds_groups, 1)],
             "bond_arithmetic": "fp32 compensated", "integration": "fp64",
             "broken_bonds_end": int(st["bond_damage"].sum()),
         },

@@ -80,6 +80,13 @@ int pdg_tile_config(const pdg_ctx* ctx, int* tileNodes, int* lanes);
 int pdg_set_kernel_variant(pdg_ctx* ctx, int variant);
 /* Number of Hilbert tiles (0: tiling unavailable) and total halo entries. */
 int pdg_get_tiling(const pdg_ctx* ctx, int* numTiles, long long* haloTotal);
+/* LDS bank-conflict figures of the tiled bond loops: the number of (16-lane
+ * group, iteration) pairs and the sum over them of the largest number of
+ * distinct halo slots sharing one residue modulo 16 (1 per pair when
+ * conflict-free), for the rows in neighbour order and in the order chosen at
+ * pdg_set_constants. No reference counterpart (a diagnostic). */
+int pdg_lds_schedule(const pdg_ctx* ctx, long long* groups, long long* costBefore,
+                     long long* costAfter);
 
 /* ---- topology and constants -------------------------------------------
  * pdg_set_topology: the block's NeighborhoodData (numOwnedPoints, ownedIDs,

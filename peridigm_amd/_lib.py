@@ -66,6 +66,7 @@ SIGNATURES = {
     "pdg_set_tile_nodes": (_i, [_vp, _i]),
     "pdg_tile_config": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "pdg_get_tiling": (_i, [_vp, C.POINTER(_i), C.POINTER(_ll)]),
+    "pdg_lds_schedule": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
     "pdg_set_topology": (_i, [_vp, _i, _vp, _vp, _ll]),
     "pdg_set_constants": (_i, [_vp, _vp, _vp, _d, _d, _d, _i, _d]),
     "pdg_set_density": (_i, [_vp, _vp]),

@@ -194,6 +194,13 @@ class Context:
         self.check(self.lib.pdg_get_tiling(self.h, C.byref(n), C.byref(t)))
         return n.value, t.value
 
+    def lds_schedule(self):
+        """(16-lane group iterations, LDS conflict cost in neighbour order,
+        cost in the scheduled order); cost = groups when conflict-free."""
+        g = C.c_longlong(); b = C.c_longlong(); a = C.c_longlong()
+        self.check(self.lib.pdg_lds_schedule(self.h, C.byref(g), C.byref(b), C.byref(a)))
+        return g.value, b.value, a.value
+
     def tile_config(self):
         """(nodes per tile, lanes per node) the bond kernels run with; the
         tile size is 0 when the untiled kernels run."""

@@ -301,6 +301,58 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
   return 0;
 }
 
+// Bond order inside the rows of the tiled CSR for conflict-free LDS reads
+// (pdg_sched.hpp), for the current lanes per node; col, src and lcol are
+// permuted together and the reverse-bond index rebuilt.
+void Ctx::order_bonds() {
+  std::vector<int> slot(B), order(B);
+  for (long long b = 0; b < B; b++) slot[b] = lcol[b] / kLdsSlot;
+  std::vector<SchedStats> part(ntiles);
+  const int G = tile_lanes();
+  parallel_for(ntiles, [&](long long t0, long long t1) {
+    for (long long t = t0; t < t1; t++)
+      part[t] = schedule_bonds(G, kTileThreads, 128, tile_start, row, slot, order, t, t + 1);
+  }, 16);
+  sched = SchedStats{};
+  for (const SchedStats& p : part) {
+    sched.groups += p.groups;
+    sched.cost_before += p.cost_before;
+    sched.cost_after += p.cost_after;
+  }
+  {
+    std::vector<int> c2(B), s2(B);
+    std::vector<unsigned short> l2(B);
+    parallel_for(B, [&](long long b0, long long b1) {
+      for (long long b = b0; b < b1; b++) {
+        c2[b] = col[order[b]];
+        s2[b] = src[order[b]];
+        l2[b] = lcol[order[b]];
+      }
+    });
+    col.swap(c2);
+    src.swap(s2);
+    lcol.swap(l2);
+  }
+  // reverse bonds: rows are no longer sorted, so look up through a sorted
+  // copy of each row
+  std::vector<std::pair<int, int>> srt(B);
+  parallel_for(N, [&](long long k0, long long k1) {
+    for (int k = (int)k0; k < (int)k1; k++) {
+      for (int b = row[k]; b < row[k + 1]; b++) srt[b] = {col[b], b};
+      std::sort(srt.begin() + row[k], srt.begin() + row[k + 1]);
+    }
+  });
+  parallel_for(N, [&](long long k0, long long k1) {
+    for (int k = (int)k0; k < (int)k1; k++)
+      for (int b = row[k]; b < row[k + 1]; b++) {
+        const int j = col[b];
+        auto it = std::lower_bound(srt.begin() + row[j], srt.begin() + row[j + 1],
+                                   std::make_pair(k, -1));
+        rev[b] = it->second;
+      }
+  });
+}
+
 // Pinned host buffer of at least `bytes` (hipHostMalloc), kept for later
 // calls; nullptr if it cannot be allocated (callers then copy from the
 // caller's memory directly).
@@ -499,6 +551,15 @@ int pdg_tile_config(const pdg_ctx* h, int* tileNodes, int* lanes) {
   return PDG_OK;
 }
 
+int pdg_lds_schedule(const pdg_ctx* h, long long* groups, long long* before, long long* after) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c) return PDG_E_ARG;
+  if (groups) *groups = c->sched.groups;
+  if (before) *before = c->sched.cost_before;
+  if (after) *after = c->sched.cost_after;
+  return PDG_OK;
+}
+
 int pdg_set_kernel_variant(pdg_ctx* h, int variant) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
@@ -626,6 +687,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   // an untileable mesh (bounds = {0}) leaves the context untiled
   if (c->build_tiles(bounds) != 0 && bounds.size() > 1)
     return c->fail(PDG_E_UNSUPPORTED, "a tile halo outgrew its capacity under the in-tile node order");
+  if (c->ntiles > 0) c->order_bonds();
 
   // Scales: L and S powers of two near the horizon and the mean volume.
   double vsum = 0.0;
@@ -633,10 +695,12 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   const double vmean = N ? vsum / N : 1.0;
   c->L = std::ldexp(1.0, (int)std::lround(std::log2(horizon)));
   c->S = std::ldexp(1.0, vmean > 0 ? (int)std::lround(std::log2(vmean)) : 0);
-  // Position quantum q = L * 2^-qexp: the longest bond component must stay
-  // below 2^30 q, so that bond differences of the 32-bit fixed-point
-  // positions are exact (|difference| < 2^31). 2^30 quanta over the longest
-  // bond give positions to about 1e-9 of a bond length.
+  // Position quantum q = L * 2^-qexp: the longest bond component stays below
+  // 2^27 q, so bond differences of the 32-bit fixed-point positions are exact
+  // integers (rounded once to FP32), rounding the positions to q moves them by
+  // at most 4e-9 of the longest bond component, and the bond kernels' product
+  // |xi|^2 |Y|^2 (below 2^110 (1 + s)^2 in units of q^4) stays finite in FP32
+  // for every stretch s below 300.
   {
     std::vector<double> mx(std::max(1, N), 0.0);
     parallel_for(N, [&](long long i0, long long i1) {
@@ -653,7 +717,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
     const double maxlen = *std::max_element(mx.begin(), mx.end()) / c->L;
     int e = 0;
     if (maxlen > 0.0) (void)std::frexp(maxlen, &e);  // maxlen < 2^e
-    c->qexp = 30 - e;
+    c->qexp = 27 - e;
     c->q = std::ldexp(c->L, -c->qexp);
     double xmax = 0.0;
     for (size_t t = 0; t < 3 * (size_t)N; t++) xmax = std::max(xmax, std::fabs(x[t]));

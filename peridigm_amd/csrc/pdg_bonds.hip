@@ -76,17 +76,17 @@ __global__ __launch_bounds__(256) void k_pass1(BondArgs a, NodeRec* __restrict__
       const NodeRec r = rec[j];
       wold = a.mask[w0 + (k >> 6)];
       brk = ((wold >> (k & 63)) & 1ull) != 0ull;
-      const BondGeom b = bond_geom_q<false>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
-                                            r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]);
+      const BondS b = bond_stretch(bond_q(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
+                                          r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]));
       if (DAMAGE) {
-        // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0); a time-dependent
-        // critical stretch replaces every damage-enabled node's
-        const float lim = ((a.sc_now > 0.0f && c.sc > 0.0f) ? a.sc_now : c.sc) * b.zeta;
-        if (NEAR && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
-        brk = brk || (b.ext > lim);
+        // a time-dependent critical stretch replaces every damage-enabled node's
+        const float scz = (a.sc_now > 0.0f && c.sc > 0.0f) ? a.sc_now : c.sc;
+        if (NEAR && !brk && fabsf(b.s - scz) <= a.near_rel * scz) nnear++;
+        brk = brk || (b.s > scz);
       }
-      const float w = omega_hat(a.influence, b.zeta, c.hhat);
-      if (!brk) acc = fmaf(w * b.zeta * b.ext, r.vol, acc);
+      // w zeta e V = w z2 s V
+      const float w = omega_z2(a.influence, b.z2, c.hhat);
+      if (!brk) acc = fmaf(w * b.z2 * b.s, r.vol, acc);
     }
     if (DAMAGE) {
       const unsigned long long bal = __ballot(brk);
@@ -148,17 +148,17 @@ __global__ __launch_bounds__(256) void k_pass2(BondArgs a, const NodeRec* __rest
     const NodeRec r = rec[j];
     const unsigned long long word = a.mask[w0 + (k >> 6)];
     const bool brk = ((word >> (k & 63)) & 1ull) != 0ull;
-    const BondGeom b = bond_geom_q<true>(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0],
-                                         r.xq[1], r.xq[2], r.u[0], r.u[1], r.u[2]);
+    const BondK b = bond_k(bond_q(xi[0], xi[1], xi[2], ui[0], ui[1], ui[2], r.xq[0], r.xq[1],
+                                  r.xq[2], r.u[0], r.u[1], r.u[2]));
     float t;
     if (a.influence == kPairModelPmb) {
-      // bond-based: (t_ij + t_ji) = c s with s = e / zeta
-      t = brk ? 0.0f : a.pmb_c * b.ext * frcp(b.zeta) * r.vol * b.ry;
+      t = pair_t<kPairModelPmb>(b, 0.0f, 0.0f, r.vol, hhat, a.pmb_c);
     } else {
-      const float w = omega_hat(a.influence, b.zeta, hhat);
-      t = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
-      t = brk ? 0.0f : w * t * r.vol * b.ry;
+      // t = w V_j (P + k (A - P)), src/materials/elastic.cxx:94-145
+      const float P = p_i + r.p, D = (alpha_i + r.alpha) - P;
+      t = omega_z2(a.influence, b.z2, hhat) * fmaf(b.k, D, P) * r.vol;
     }
+    t = brk ? 0.0f : t;
     f0 = fmaf(t, b.y0, f0);
     f1 = fmaf(t, b.y1, f1);
     f2 = fmaf(t, b.y2, f2);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "pdg_internal.hpp"
+#include "pdg_sched.hpp"
 
 namespace pdg {
 
@@ -32,7 +33,7 @@ struct Ctx {
 
   // Hilbert order and internal CSR
   std::vector<int> perm, iperm;  // perm[k]: original id of internal node k
-  std::vector<int> row, col;     // CSR over internal ids, rows sorted
+  std::vector<int> row, col;     // CSR over internal ids (tiled: rows in LDS-schedule order)
   std::vector<int> src;          // internal bond -> list-order bond index
   std::vector<int> rev;          // internal bond (i->j) -> internal bond (j->i)
   std::vector<int> mw;           // first mask word of each node
@@ -107,6 +108,8 @@ struct Ctx {
   int build_csr();
   int tile_bounds(std::vector<int>& bounds) const;
   int build_tiles(const std::vector<int>& bounds);
+  void order_bonds();
+  SchedStats sched;  // LDS service groups and their conflict cost, before and after order_bonds
 
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;

@@ -11,16 +11,18 @@ __device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v)
 __device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
 
 // Influence function (src/core/Peridigm_InfluenceFunction.hpp:69-88) of a
-// bond length and horizon given in the same unit.
-__device__ __forceinline__ float omega_hat(int kind, float zeta, float hhat) {
+// bond with squared length z2, horizon hhat in the same unit. The
+// Gaussian needs no square root (exp(-zeta^2 / (0.16 delta^2))).
+__device__ __forceinline__ float omega_z2(int kind, float z2, float hhat) {
   if (kind == PDG_INFLUENCE_ONE || kind == kPairModelPmb) return 1.0f;
   if (kind == PDG_INFLUENCE_PARABOLIC_DECAY) {
+    const float zeta = fsqrt(z2);
     if (zeta > hhat) return 0.0f;
     float s = zeta / hhat;
     return s < 0.5f ? 1.0f : -4.0f * s * s + 4.0f * s;
   }
   float h2 = hhat * hhat * 0.16f;
-  return expf(-(zeta * zeta) / h2);
+  return expf(-z2 / h2);
 }
 
 // Cross-lane moves inside 16-lane rows (DPP): quad_perm [1,0,3,2] and
@@ -78,55 +80,97 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-struct BondGeom {
+// Bond geometry in units of the position quantum q. xi is the integer
+// difference of the neighbours' 32-bit fixed-point positions (taken modulo
+// 2^32; every bond component is below 2^27 q), rounded once to FP32, so
+// xi_ij = -xi_ji exactly. eta = u_j - u_i is formed in FP64 from the records
+// and rounded once. d = |Y|^2 - |xi|^2 = eta . (2 xi + eta), with
+// Y = xi + eta, has no cancellation; with g = |xi| |Y| = sqrt(z2 yy) the
+// stretch and the extension over the current length follow with one square
+// root and one reciprocal:
+//   s     = (|Y| - |xi|) / |xi| = d / (z2 + g),
+//   e/|Y| = (|Y| - |xi|) / |Y|  = d / (yy + g).
+// Swapping i and j negates xi and eta, which leaves z2, d, s and e/|Y|
+// bit-identical, so both ends of a bond decide its break alike.
+struct BondQ {
   float x0, x1, x2;  // xi
-  float y0, y1, y2;  // Y = xi + eta
-  float zeta, ext;   // |xi|, |Y| - |xi|
-  float ry;          // 1/|Y| (when requested)
+  float e0, e1, e2;  // eta
 };
 
-// Compensated bond geometry, lengths in units of the position quantum q.
-// xi is the exact integer difference of the neighbours' 32-bit fixed-point
-// positions (taken modulo 2^32; every bond component is below 2^31 q), so
-// xi_ij = -xi_ji exactly; eta = u_j - u_i is formed in FP64 from the records
-// and rounded once. Both are rounded to FP32 once, Y = xi + eta, and the
-// extension uses e = eta . (xi + Y) / (|Y| + |xi|), which has no
-// cancellation.
-template <bool NEED_RY>
-__device__ __forceinline__ BondGeom bond_geom_q(unsigned xi0, unsigned xi1, unsigned xi2,
-                                                double ui0, double ui1, double ui2, unsigned xj0,
-                                                unsigned xj1, unsigned xj2, double uj0, double uj1,
-                                                double uj2) {
-  BondGeom b;
+__device__ __forceinline__ BondQ bond_q(unsigned xi0, unsigned xi1, unsigned xi2, double ui0,
+                                        double ui1, double ui2, unsigned xj0, unsigned xj1,
+                                        unsigned xj2, double uj0, double uj1, double uj2) {
+  BondQ b;
   b.x0 = (float)(int)(xj0 - xi0);
   b.x1 = (float)(int)(xj1 - xi1);
   b.x2 = (float)(int)(xj2 - xi2);
-  const float e0 = (float)(uj0 - ui0);
-  const float e1 = (float)(uj1 - ui1);
-  const float e2 = (float)(uj2 - ui2);
-  const float z2 = fmaf(b.x2, b.x2, fmaf(b.x1, b.x1, b.x0 * b.x0));
-  b.zeta = fsqrt(z2);
-  if (NEED_RY) {
-    // |Y|^2 - |xi|^2 = eta . (xi + Y)
-    b.y0 = b.x0 + e0;
-    b.y1 = b.x1 + e1;
-    b.y2 = b.x2 + e2;
-    const float dot = fmaf(e2, b.x2 + b.y2, fmaf(e1, b.x1 + b.y1, e0 * (b.x0 + b.y0)));
-    const float yy = z2 + dot;
-    b.ry = frsq(yy);
-    b.ext = dot * frcp(yy * b.ry + b.zeta);
-  } else {
-    // |Y|^2 = |xi|^2 + 2 xi.eta + eta.eta and eta.(xi + Y) = 2 xi.eta + eta.eta
-    // share the two small dot products
-    const float xe = fmaf(e2, b.x2, fmaf(e1, b.x1, e0 * b.x0));
-    const float ee = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
-    const float dot = fmaf(2.0f, xe, ee);
-    const float yy = z2 + dot;
-    b.y0 = b.y1 = b.y2 = 0.0f;
-    b.ry = 0.0f;
-    b.ext = dot * frcp(fsqrt(yy) + b.zeta);
-  }
+  b.e0 = (float)(uj0 - ui0);
+  b.e1 = (float)(uj1 - ui1);
+  b.e2 = (float)(uj2 - ui2);
   return b;
+}
+
+__device__ __forceinline__ float bond_z2(const BondQ& b) {
+  return fmaf(b.x2, b.x2, fmaf(b.x1, b.x1, b.x0 * b.x0));
+}
+
+// Pass-1 quantities: squared length and stretch.
+struct BondS {
+  float z2, s;
+};
+__device__ __forceinline__ BondS bond_stretch(const BondQ& b) {
+  BondS r;
+  const float w0 = fmaf(2.0f, b.x0, b.e0), w1 = fmaf(2.0f, b.x1, b.e1),
+              w2 = fmaf(2.0f, b.x2, b.e2);
+  const float d = fmaf(b.e2, w2, fmaf(b.e1, w1, b.e0 * w0));
+  r.z2 = bond_z2(b);
+  const float yy = r.z2 + d;
+  const float g = fsqrt(r.z2 * yy);
+  r.s = d * frcp(r.z2 + g);
+  return r;
+}
+
+// Pass-2 quantities: the current bond vector Y, squared length and
+// extension over current length k = e / |Y|.
+struct BondK {
+  float y0, y1, y2;
+  float z2, k;
+};
+__device__ __forceinline__ BondK bond_k(const BondQ& b) {
+  BondK r;
+  r.y0 = b.x0 + b.e0;
+  r.y1 = b.x1 + b.e1;
+  r.y2 = b.x2 + b.e2;
+  const float d = fmaf(b.e2, b.x2 + r.y2, fmaf(b.e1, b.x1 + r.y1, b.e0 * (b.x0 + r.y0)));
+  r.z2 = bond_z2(b);
+  const float yy = r.z2 + d;
+  const float g = fsqrt(r.z2 * yy);
+  r.k = d * frcp(yy + g);
+  return r;
+}
+
+// Pair force coefficient of a bond (both ends, f_i += t Y): linear
+// peridynamic solid, src/materials/elastic.cxx:94-145 with
+//   t = w V_j (zeta/|Y| P + e/|Y| A) = w V_j (P + k (A - P)),
+// P = p_i + p_j (dilatation terms), A = alpha_i + alpha_j, D = A - P passed as
+// D_i + D_j; bond-based, src/materials/elastic_bond_based.cxx:100-125 with
+// t = c s V_j / |Y| = c k V_j / zeta.
+template <int INFL>
+__device__ __forceinline__ float pair_t(const BondK& b, float P, float D, float vj, float hhat,
+                                        float pmb_c) {
+  if (INFL == kPairModelPmb) return pmb_c * b.k * frsq(b.z2) * vj;
+  const float t = fmaf(b.k, D, P) * vj;
+  return INFL == PDG_INFLUENCE_ONE ? t : omega_z2(INFL, b.z2, hhat) * t;
+}
+
+// The same with the neighbour's terms premultiplied by its volume:
+// V_j (P + k D) = (p_i V_j + V_j p_j) + k (d_i V_j + V_j d_j), d = alpha - p.
+template <int INFL>
+__device__ __forceinline__ float pair_tv(const BondK& b, float p_i, float d_i, float vj,
+                                         float vp_j, float vd_j, float hhat, float pmb_c) {
+  if (INFL == kPairModelPmb) return pmb_c * b.k * frsq(b.z2) * vj;
+  const float t = fmaf(b.k, fmaf(d_i, vj, vd_j), fmaf(p_i, vj, vp_j));
+  return INFL == PDG_INFLUENCE_ONE ? t : omega_z2(INFL, b.z2, hhat) * t;
 }
 
 // The global inputs of one Verlet component, loadable ahead of the force.
@@ -139,7 +183,8 @@ __device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, 
   VerletPre p;
   p.rho = va.rho[node];
   p.v = va.v[3 * node + l];
-  p.slot = va.start_step ? va.bc_slot[node] : -1;
+  const int slot = va.bc_slot[node];  // loaded unconditionally, selected after
+  p.slot = va.start_step ? slot : -1;
   return p;
 }
 

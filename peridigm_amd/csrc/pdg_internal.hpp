@@ -19,7 +19,7 @@ namespace pdg {
 //   xq: the low 32 bits of round(x / q), the reference position in fixed
 //      point: for any bond the difference xq_j - xq_i, taken modulo 2^32, is
 //      the exact integer (x_j - x_i) / q of the rounded positions, because
-//      every bond component is shorter than 2^31 q; the same difference with
+//      every bond component is shorter than 2^27 q; the same difference with
 //      i and j swapped is its exact negative;
 //   vol: V / S, alpha: 15 mu / m * L * S, p: theta * (3K/m - alpha/3) * L * S.
 // L and S are powers of two, so the scalings are exact.

@@ -2,14 +2,15 @@
 //
 // One workgroup (8 waves) owns a tile: a contiguous range of at most 128
 // Hilbert-ordered nodes (32 or 64 on small meshes), ordered by degree inside
-// the tile. It first stages the fixed-point reference positions, FP64
-// displacements and per-node scalars of the tile's halo (the union of the
-// tile's nodes and all their neighbours, at most 800 nodes) in LDS with one
-// 64-byte record load per halo node, then runs group-of-G-lanes bond loops
-// that read neighbours from LDS through 16-bit local indices, three 16-byte
-// LDS reads per neighbour. Per bond, HBM and L2 see a 2-byte index and one
-// bit of the damage mask instead of a 4-byte index plus a 64-byte record
-// gather.
+// the tile. It first stages its halo (the union of the tile's nodes and all
+// their neighbours, at most 800 nodes) in LDS with one 64-byte record load
+// per halo node, then runs group-of-G-lanes bond loops that read neighbours
+// from LDS through 16-bit local indices, three 16-byte LDS reads per
+// neighbour. Both passes stage the exact inputs (fixed-point positions, FP64
+// displacements), so both ends of a bond compute the same stretch in pass 1
+// and every bond vector keeps full FP32 accuracy in pass 2. Per bond, HBM and L2 see a
+// 2-byte index and one bit of the damage mask instead of a 4-byte index plus
+// a 64-byte record gather.
 //
 // A node's damage-mask words (at most 2, degree <= 128) are held in registers
 // for the whole bond loop, the lane's 16-bit neighbour indices are prefetched
@@ -69,8 +70,8 @@ __device__ __forceinline__ int xcd_tile_strided(int q, int nb, int ntiles) {
 inline unsigned tile_grid(int ntiles) { return 8u * (unsigned)((ntiles + 7) >> 3); }
 
 template <int INFL>
-__device__ __forceinline__ float omega_t(float zeta, float hhat) {
-  return omega_hat(INFL, zeta, hhat);
+__device__ __forceinline__ float omega_t(float z2, float hhat) {
+  return omega_z2(INFL, z2, hhat);
 }
 
 // Loads a node's mask words into registers, with the bits of the bonds past
@@ -108,76 +109,155 @@ __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lco
   for (int it = 0; it < MAXDEG / G; it++) lj[it] = (int)p[it * G];
 }
 
-// LDS staging arrays of one workgroup (shared by both passes), 16-byte slots
-// so one byte offset (16 x the halo position) addresses every array and a
-// neighbour takes three 16-byte reads:
-//   sa: fixed-point position and volume, sb: displacement x and y (units of
-//   q), sc: displacement z, alpha and p.
-struct __align__(16) SlotC {
-  double u2;
-  float alpha, p;
-};
+// LDS staging arrays of one workgroup: three arrays of 16-byte slots, so
+// one byte offset (16 x the halo position) addresses every array. Each pass
+// stages its own view of the halo:
+//   pass 1 (exact, so both ends of a bond compute the same stretch):
+//     c: displacement z (FP64, units of q), a: fixed-point position and
+//     volume, b: displacement x and y (FP64);
+//   pass 2: as pass 1, with the pair-force terms V (alpha - p) and V p
+//     after the displacement z in array c.
+// Each array has one slot past the capacity: staging threads without a slot
+// write there, so staging needs no branch (no bond addresses it).
 struct TileLds {
-  SlotC sc[HMAX];    // (u2, alpha, p); first, so its reads need no address add
-  uint4 sa[HMAX];    // (xq0, xq1, xq2, bits of V/S)
-  double2 sb[HMAX];  // (u0, u1)
+  uint4 raw_c[HMAX + 1];  // array c first, so its reads need no address add
+  uint4 raw_a[HMAX + 1];
+  uint4 raw_b[HMAX + 1];
   float sf[3 * kTileNodesMax];  // pass 2: the tile's forces, for the Verlet epilogue
 };
-static_assert(sizeof(SlotC) == kLdsSlot && sizeof(uint4) == kLdsSlot && sizeof(double2) == kLdsSlot,
-              "16-byte LDS slots");
+static_assert(sizeof(uint4) == kLdsSlot, "16-byte LDS slots");
 
-// One neighbour's staged data, loaded one bond ahead of its use by the
-// pipelined bond loops so LDS latency overlaps the previous bond's math.
-struct Nb {
-  unsigned x0, x1, x2;
-  float vol;
-  double u0, u1, u2;
-  float alpha, p;
-};
+__device__ __forceinline__ char* lds_c(TileLds& L) { return reinterpret_cast<char*>(L.raw_c); }
+__device__ __forceinline__ char* lds_a(TileLds& L) { return reinterpret_cast<char*>(L.raw_a); }
+__device__ __forceinline__ char* lds_b(TileLds& L) { return reinterpret_cast<char*>(L.raw_b); }
+__device__ __forceinline__ const char* lds_c(const TileLds& L) {
+  return reinterpret_cast<const char*>(L.raw_c);
+}
+__device__ __forceinline__ const char* lds_a(const TileLds& L) {
+  return reinterpret_cast<const char*>(L.raw_a);
+}
+__device__ __forceinline__ const char* lds_b(const TileLds& L) {
+  return reinterpret_cast<const char*>(L.raw_b);
+}
 
 // jb is the neighbour's byte offset in the slot arrays (local index times
 // kLdsSlot, as stored in the tile index array).
 template <typename T>
-__device__ __forceinline__ T lds_at(const void* arr, int jb) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(arr) + jb);
+__device__ __forceinline__ T lds_at(const char* arr, int jb) {
+  return *reinterpret_cast<const T*>(arr + jb);
+}
+template <typename T>
+__device__ __forceinline__ void lds_put(char* arr, int h, const T& v) {
+  *reinterpret_cast<T*>(arr + kLdsSlot * h) = v;
 }
 
-template <bool FORCE>
+// One neighbour's staged data, loaded one bond ahead of its use by the
+// pipelined bond loops so LDS latency overlaps the previous bond's math.
+struct Nb {  // pass 1
+  unsigned x0, x1, x2;
+  float vol;
+  double u0, u1, u2;
+};
+struct Nb2 {  // pass 2
+  unsigned x0, x1, x2;
+  float vol;
+  double u0, u1, u2;
+  float vd, vp;  // V (alpha - p), V p
+};
+
 __device__ __forceinline__ Nb nb_load(const TileLds& L, int jb) {
   Nb n;
-  const uint4 a = lds_at<uint4>(L.sa, jb);
-  const double2 b = lds_at<double2>(L.sb, jb);
+  const uint4 a = lds_at<uint4>(lds_a(L), jb);
+  const uint4 b = lds_at<uint4>(lds_b(L), jb);
   n.x0 = a.x; n.x1 = a.y; n.x2 = a.z; n.vol = __uint_as_float(a.w);
-  n.u0 = b.x; n.u1 = b.y;
-  if (FORCE) {
-    // one 16-byte read: (u2, the bits of alpha and p)
-    const double2 c = lds_at<double2>(L.sc, jb);
-    n.u2 = c.x;
-    n.alpha = __int_as_float(__double2loint(c.y));
-    n.p = __int_as_float(__double2hiint(c.y));
-  } else {
-    n.u2 = lds_at<double>(L.sc, jb);
-    n.alpha = 0.0f; n.p = 0.0f;
-  }
+  n.u0 = __hiloint2double((int)b.y, (int)b.x);
+  n.u1 = __hiloint2double((int)b.w, (int)b.z);
+  n.u2 = lds_at<double>(lds_c(L), jb);
   return n;
 }
 
-// Stages halo slot h from the node record r (zeros when r is null).
-__device__ __forceinline__ void stage_slot(TileLds& L, int h, const NodeRec* r) {
-  if (r) {
-    L.sa[h] = make_uint4(r->xq[0], r->xq[1], r->xq[2], __float_as_uint(r->vol));
-    L.sb[h] = make_double2(r->u[0], r->u[1]);
-    L.sc[h] = SlotC{r->u[2], r->alpha, r->p};
-  } else {
-    L.sa[h] = make_uint4(0u, 0u, 0u, 0u);
-    L.sb[h] = make_double2(0.0, 0.0);
-    L.sc[h] = SlotC{0.0, 0.0f, 0.0f};
+__device__ __forceinline__ Nb2 nb2_load(const TileLds& L, int jb) {
+  Nb2 n;
+  // three 16-byte reads (uint4, so none is split into two 8-byte halves,
+  // which would conflict at this stride)
+  const uint4 a = lds_at<uint4>(lds_a(L), jb);
+  const uint4 b = lds_at<uint4>(lds_b(L), jb);
+  const uint4 c = lds_at<uint4>(lds_c(L), jb);
+  n.x0 = a.x; n.x1 = a.y; n.x2 = a.z; n.vol = __uint_as_float(a.w);
+  n.u0 = __hiloint2double((int)b.y, (int)b.x);
+  n.u1 = __hiloint2double((int)b.w, (int)b.z);
+  n.u2 = __hiloint2double((int)c.y, (int)c.x);
+  n.vd = __uint_as_float(c.z);
+  n.vp = __uint_as_float(c.w);
+  return n;
+}
+
+// The record fields the LDS slots hold (48 of the record's 64 bytes).
+struct SlotVals {
+  double u0, u1, u2;
+  unsigned x0, x1, x2;
+  float vol, alpha, p;
+};
+
+__device__ __forceinline__ SlotVals slot_vals(const NodeRec* __restrict__ r) {
+  SlotVals v;
+  v.u0 = r->u[0]; v.u1 = r->u[1]; v.u2 = r->u[2];
+  v.x0 = r->xq[0]; v.x1 = r->xq[1]; v.x2 = r->xq[2];
+  v.vol = r->vol; v.alpha = r->alpha; v.p = r->p;
+  return v;
+}
+
+// Staging of the tile's halo into LDS: slot h < hn from the record of node
+// halo[h0 + h], slots hn..HMAX-1 zero-filled (masked-out bonds may address
+// them). Each thread handles slots threadIdx.x + r * WG. The three phases are
+// separate calls so a kernel can put every independent load of its prologue
+// in flight together: the halo indices (halo_idx), then the records
+// (halo_load, with the row data that depends on the node's CSR range), then
+// the LDS writes (halo_write). Pass 1 stages the fixed-point position and
+// volume (array a), the FP64 displacement (b: x, y; c: z); pass 2 adds
+// V (alpha - p) and V p after z in array c.
+constexpr int kSlotsPerThread = (HMAX + WG - 1) / WG;
+
+// Loads are unconditional (a slot past the halo reads its first entry) and
+// the values are selected in halo_write, so no branch makes the compiler wait
+// between the phases. A tile's halo holds at least its own nodes (hn >= 1).
+struct HaloStage {
+  int hn;
+  int idx[kSlotsPerThread];
+  SlotVals v[kSlotsPerThread];
+};
+
+__device__ __forceinline__ void halo_idx(HaloStage& hs, const TileArgs& t, int h0, int hn) {
+  hs.hn = hn;
+#pragma unroll
+  for (int r = 0; r < kSlotsPerThread; r++) {
+    const int h = threadIdx.x + r * WG;
+    hs.idx[r] = t.halo[(unsigned)(h0 + (h < hn ? h : 0))];
   }
 }
 
-// Component l of the displacement staged in slot h.
-__device__ __forceinline__ double slot_u(const TileLds& L, int h, int l) {
-  return l == 0 ? L.sb[h].x : (l == 1 ? L.sb[h].y : L.sc[h].u2);
+__device__ __forceinline__ void halo_load(HaloStage& hs, const NodeRec* __restrict__ rec) {
+#pragma unroll
+  for (int r = 0; r < kSlotsPerThread; r++) hs.v[r] = slot_vals(&rec[(unsigned)hs.idx[r]]);
+}
+
+template <bool PASS2>
+__device__ __forceinline__ void halo_write(TileLds& L, const HaloStage& hs) {
+  const SlotVals zero{0.0, 0.0, 0.0, 0u, 0u, 0u, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int r = 0; r < kSlotsPerThread; r++) {
+    const int hh = threadIdx.x + r * WG;
+    const int h = hh < HMAX ? hh : HMAX;  // the spare slot
+    const SlotVals& v = hh < hs.hn ? hs.v[r] : zero;
+    lds_put(lds_a(L), h, make_uint4(v.x0, v.x1, v.x2, __float_as_uint(v.vol)));
+    lds_put(lds_b(L), h, make_double2(v.u0, v.u1));
+    if (PASS2) {
+      const float dp[2] = {v.vol * (v.alpha - v.p), v.vol * v.p};
+      lds_put(lds_c(L), h, make_double2(v.u2, *reinterpret_cast<const double*>(dp)));
+    } else {
+      lds_put(lds_c(L), h, make_double2(v.u2, 0.0));
+    }
+  }
 }
 
 // Break bit of bond k = it * G + l of a node (it a compile-time constant):
@@ -195,27 +275,43 @@ __device__ __forceinline__ bool mask_bit(const unsigned long long (&m)[NWORDS], 
 // local indices, constants.
 template <int G>
 struct RowData {
-  int rs, deg, w0;
+  int rs, re, deg, w0;
   unsigned long long m[NWORDS];
   int lj[MAXDEG / G];
   NodeConst c;
 };
 
+// Row data in two phases: the node's CSR range, mask-word offset and
+// constants (row_scalars), then the mask words and local indices they
+// address (row_arrays).
+// The loads are unconditional (an invalid group reads the valid node
+// `fallback`) and nothing computes on their values until row_arrays, so a
+// prologue's loads stay in flight together (no branch or early use makes the
+// compiler wait for them).
 template <int G>
-__device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, int node,
-                                         bool valid, int l, RowData<G>& r) {
-  r.rs = 0;
-  r.deg = 0;
-  r.w0 = 0;
-  r.c = NodeConst{0.0f, 0.0f, 0.0f, 1.0f};
-  if (valid) {
-    r.rs = a.row[node];
-    r.deg = a.row[node + 1] - r.rs;
-    r.w0 = a.mw[node];
-    r.c = a.nc[node];
-  }
+__device__ __forceinline__ void row_scalars(const BondArgs& a, int node, bool valid, int fallback,
+                                            RowData<G>& r) {
+  const int nd = valid ? node : fallback;
+  r.rs = a.row[nd];
+  r.re = a.row[nd + 1];
+  r.w0 = a.mw[nd];
+  r.c = a.nc[nd];
+}
+
+template <int G>
+__device__ __forceinline__ void row_arrays(const BondArgs& a, const TileArgs& t, int l, bool valid,
+                                           RowData<G>& r) {
+  r.deg = valid ? r.re - r.rs : 0;
+  if (!valid) r.c = NodeConst{0.0f, 0.0f, 0.0f, 1.0f};
   load_words(a.mask, r.w0, r.deg, r.m);
   load_lcol<G>(t.lcol, r.rs, l, r.lj);
+}
+
+template <int G>
+__device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, int node,
+                                         bool valid, int fallback, int l, RowData<G>& r) {
+  row_scalars<G>(a, node, valid, fallback, r);
+  row_arrays<G>(a, t, l, valid, r);
 }
 
 // Lane masks of the wave (compare results kept in scalar registers):
@@ -253,17 +349,21 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   const int l = threadIdx.x & (G - 1);
   // the first round's row data is loaded before the halo staging, so the two
   // sets of global loads are in flight together
+  // independent loads first (halo indices, CSR ranges), then what they
+  // address (records, mask words, local indices), then the LDS writes
+  HaloStage hs;
+  halo_idx(hs, t, h0, hn);
   RowData<G> rd;
-  row_load<G>(a, t, n0 + grp, n0 + grp < n1, l, rd);
-  // slots past the halo are zero-filled: masked-out bonds may address them
-  for (int h = threadIdx.x; h < HMAX; h += WG)
-    stage_slot(L, h, h < hn ? &rec[(unsigned)t.halo[(unsigned)(h0 + h)]] : nullptr);
+  row_scalars<G>(a, n0 + grp, n0 + grp < n1, n0, rd);
+  halo_load(hs, rec);
+  row_arrays<G>(a, t, l, n0 + grp < n1, rd);
+  halo_write<false>(L, hs);
   __syncthreads();
   const int own = t.own_off[tile];
   for (int base = n0; base < n1; base += NG) {
     const int node = base + grp;
     const bool valid = node < n1;
-    if (base != n0) row_load<G>(a, t, node, valid, l, rd);
+    if (base != n0) row_load<G>(a, t, node, valid, n0, l, rd);
     const int deg = rd.deg, w0 = rd.w0;
     const unsigned long long (&mo)[NWORDS] = rd.m;
     const int (&lj)[MAXDEG / G] = rd.lj;
@@ -272,15 +372,16 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     double ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
     if (valid) {
       const int li = own + (node - n0);
-      const uint4 sa = L.sa[li];
-      xi0 = sa.x; xi1 = sa.y; xi2 = sa.z;
-      ui0 = L.sb[li].x; ui1 = L.sb[li].y; ui2 = L.sc[li].u2;
+      const Nb me = nb_load(L, kLdsSlot * li);
+      xi0 = me.x0; xi1 = me.x1; xi2 = me.x2;
+      ui0 = me.u0; ui1 = me.u1; ui2 = me.u2;
     }
     unsigned long long mn[NWORDS];
 #pragma unroll
     for (int w = 0; w < NWORDS; w++) mn[w] = 0ull;
     // a time-dependent critical stretch replaces every damage-enabled node's
     const float scz = (a.sc_now > 0.0f && c.sc > 0.0f) ? a.sc_now : c.sc;
+    const float nlim = a.near_rel * scz;
     const int nmax = groups_max<G>((deg + G - 1) / G);
     float acc = 0.0f;
     int nnear = 0;               // plain loops: per lane
@@ -292,32 +393,33 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       // registers) and are merged into the group's mask words after the loop.
       unsigned long long balv[MAXDEG / G];
       Nb nb[2];
-      nb[0] = nb_load<false>(L, lj[0]);
+      nb[0] = nb_load(L, lj[0]);
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
           if (it + 1 < MAXDEG / G && it + 1 < nmax)
-            nb[(it + 1) & 1] = nb_load<false>(L, lj[(it + 1) % (MAXDEG / G)]);
+            nb[(it + 1) & 1] = nb_load(L, lj[(it + 1) % (MAXDEG / G)]);
           const Nb& cur = nb[it & 1];
           // lanes whose bond exists and is unbroken; the masks stay in scalar
           // registers, so combining them costs no vector instructions
           const unsigned long long okm = intact_lanes<G>(mo, it, l);
-          const BondGeom b = bond_geom_q<false>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
-                                                cur.x2, cur.u0, cur.u1, cur.u2);
+          const BondS b = bond_stretch(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
+                                              cur.x2, cur.u0, cur.u1, cur.u2));
           unsigned long long brkm = 0ull;  // lanes whose bond breaks now
           if (DAMAGE) {
-            const float lim = scz * b.zeta;
             if (NEAR) {
               // intact bonds with |s - s_c| <= near_rel * s_c, counted as a
               // wave-wide lane mask (scalar population count)
               const unsigned long long nm =
-                  okm & __builtin_amdgcn_fcmpf(fabsf(b.ext - lim), a.near_rel * lim, CMP_FOLE);
+                  okm & __builtin_amdgcn_fcmpf(fabsf(b.s - scz), nlim, CMP_FOLE);
               nnear_w += (unsigned)__builtin_popcountll(nm);
             }
-            brkm = okm & __builtin_amdgcn_fcmpf(b.ext, lim, CMP_FOGT);
+            brkm = okm & __builtin_amdgcn_fcmpf(b.s, scz, CMP_FOGT);
           }
-          const float w = omega_t<INFL>(b.zeta, c.hhat);
-          const float nacc = fmaf(w * b.zeta * b.ext, cur.vol, acc);
+          // w zeta e V = w z2 s V
+          float term = b.z2 * b.s;
+          if (INFL != PDG_INFLUENCE_ONE) term *= omega_t<INFL>(b.z2, c.hhat);
+          const float nacc = fmaf(term, cur.vol, acc);
           acc = __builtin_amdgcn_inverse_ballot_w64(okm & ~brkm) ? nacc : acc;
           if (DAMAGE) balv[it] = brkm;
         } else if (DAMAGE) {
@@ -340,17 +442,15 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
         const bool act = k < deg;
         bool brk = act && mask_bit<G>(mo, it, l);
         if (act) {
-          const Nb r = nb_load<false>(L, lj[it]);
-          const BondGeom b = bond_geom_q<false>(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2,
-                                                r.u0, r.u1, r.u2);
+          const Nb r = nb_load(L, lj[it]);
+          const BondS b = bond_stretch(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2,
+                                              r.u0, r.u1, r.u2));
           if (DAMAGE) {
-            // s = e / zeta > sc  <=>  e > sc * zeta  (zeta > 0)
-            const float lim = scz * b.zeta;
-            if (NEAR && !brk && fabsf(b.ext - lim) <= a.near_rel * lim) nnear++;
-            brk = brk || (b.ext > lim);
+            if (NEAR && !brk && fabsf(b.s - scz) <= nlim) nnear++;
+            brk = brk || (b.s > scz);
           }
-          const float w = omega_t<INFL>(b.zeta, c.hhat);
-          if (!brk) acc = fmaf(w * b.zeta * b.ext, r.vol, acc);
+          const float w = omega_t<INFL>(b.z2, c.hhat);
+          if (!brk) acc = fmaf(w * b.z2 * b.s, r.vol, acc);
         }
         if (DAMAGE) {
           const unsigned long long bal = __ballot(brk);
@@ -410,62 +510,63 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   constexpr int NG = WG / G;
   const int grp = threadIdx.x / G;
   const int l = threadIdx.x & (G - 1);
-  // the first round's row data and this thread's Verlet inputs (component
-  // q = threadIdx.x; a tile has at most 3 * 128 <= WG components) are loaded
-  // before the halo staging, so all these global loads are in flight together
+  // independent loads first (halo indices, CSR ranges, this thread's Verlet
+  // inputs: component q = threadIdx.x, a tile has at most 3 * 128 <= WG
+  // components), then what they address (records, mask words, local
+  // indices), then the LDS writes
+  HaloStage hs;
+  halo_idx(hs, t, h0, hn);
   RowData<G> rd;
-  row_load<G>(a, t, n0 + grp, n0 + grp < n1, l, rd);
+  row_scalars<G>(a, n0 + grp, n0 + grp < n1, n0, rd);
   const int q = threadIdx.x;
   const bool has_q = q < 3 * (n1 - n0);
   VerletPre vp{0.0, 0.0, -1};
-  if (VERLET && has_q) vp = verlet_pre(va, n0 + q / 3, q - 3 * (q / 3));
-  // slots past the halo are zero-filled: masked-out bonds may address them
-  for (int h = threadIdx.x; h < HMAX; h += WG)
-    stage_slot(L, h, h < hn ? &rec[(unsigned)t.halo[(unsigned)(h0 + h)]] : nullptr);
+  double u_own = 0.0;  // this thread's component of the FP64 displacement (units of q)
+  if (VERLET) {
+    // unconditional (a thread past the tile's components reads component 0)
+    const int qq = has_q ? q : 0;
+    vp = verlet_pre(va, n0 + qq / 3, qq - 3 * (qq / 3));
+    u_own = rec[n0 + qq / 3].u[qq - 3 * (qq / 3)];
+  }
+  halo_load(hs, rec);
+  row_arrays<G>(a, t, l, n0 + grp < n1, rd);
+  halo_write<true>(L, hs);
   __syncthreads();
   const int own = t.own_off[tile];
   for (int base = n0; base < n1; base += NG) {
     const int node = base + grp;
     const bool valid = node < n1;
-    if (base != n0) row_load<G>(a, t, node, valid, l, rd);
+    if (base != n0) row_load<G>(a, t, node, valid, n0, l, rd);
     const int deg = rd.deg;
     const unsigned long long (&mo)[NWORDS] = rd.m;
     const int (&lj)[MAXDEG / G] = rd.lj;
     const float hhat = rd.c.hhat;
-    unsigned xi0 = 0u, xi1 = 0u, xi2 = 0u;
-    double ui0 = 0.0, ui1 = 0.0, ui2 = 0.0;
-    float alpha_i = 0.0f, p_i = 0.0f;
+    Nb2 me{};
+    float d_i = 0.0f, p_i = 0.0f;
     if (valid) {
-      const int li = own + (node - n0);
-      const uint4 sa = L.sa[li];
-      xi0 = sa.x; xi1 = sa.y; xi2 = sa.z;
-      ui0 = L.sb[li].x; ui1 = L.sb[li].y;
-      const SlotC sc = L.sc[li];
-      ui2 = sc.u2; alpha_i = sc.alpha; p_i = sc.p;
+      // the node's own alpha - p and p from its staged V (alpha - p), V p
+      me = nb2_load(L, kLdsSlot * (own + (node - n0)));
+      const float rv = me.vol > 0.0f ? 1.0f / me.vol : 0.0f;
+      d_i = me.vd * rv;
+      p_i = me.vp * rv;
     }
+    const unsigned xi0 = me.x0, xi1 = me.x1, xi2 = me.x2;
+    const double ui0 = me.u0, ui1 = me.u1, ui2 = me.u2;
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
       const int nmax = groups_max<G>((deg + G - 1) / G);
-      Nb nb[2];
-      nb[0] = nb_load<true>(L, lj[0]);
+      Nb2 nb[2];
+      nb[0] = nb2_load(L, lj[0]);
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
           if (it + 1 < MAXDEG / G && it + 1 < nmax)
-            nb[(it + 1) & 1] = nb_load<true>(L, lj[(it + 1) % (MAXDEG / G)]);
-          const Nb& cur = nb[it & 1];
+            nb[(it + 1) & 1] = nb2_load(L, lj[(it + 1) % (MAXDEG / G)]);
+          const Nb2& cur = nb[it & 1];
           const unsigned long long okm = intact_lanes<G>(mo, it, l);
-          const BondGeom b = bond_geom_q<true>(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
-                                               cur.x2, cur.u0, cur.u1, cur.u2);
-          float tt;
-          if (INFL == kPairModelPmb) {
-            // bond-based: (t_ij + t_ji) = c s with s = e / zeta
-            tt = a.pmb_c * b.ext * frcp(b.zeta) * cur.vol * b.ry;
-          } else {
-            const float w = omega_t<INFL>(b.zeta, hhat);
-            tt = fmaf(b.zeta, p_i + cur.p, b.ext * (alpha_i + cur.alpha));
-            tt = w * tt * cur.vol * b.ry;
-          }
+          const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
+                                        cur.u0, cur.u1, cur.u2));
+          float tt = pair_tv<INFL>(b, p_i, d_i, cur.vol, cur.vp, cur.vd, hhat, a.pmb_c);
           tt = __builtin_amdgcn_inverse_ballot_w64(okm) ? tt : 0.0f;
           f0 = fmaf(tt, b.y0, f0);
           f1 = fmaf(tt, b.y1, f1);
@@ -478,17 +579,11 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
       const int k = it * G + l;
       if (k < deg) {
         const bool brk = mask_bit<G>(mo, it, l);
-        const Nb r = nb_load<true>(L, lj[it]);
-        const BondGeom b = bond_geom_q<true>(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2,
-                                             r.u0, r.u1, r.u2);
-        float tt;
-        if (INFL == kPairModelPmb) {
-          tt = brk ? 0.0f : a.pmb_c * b.ext * frcp(b.zeta) * r.vol * b.ry;
-        } else {
-          const float w = omega_t<INFL>(b.zeta, hhat);
-          tt = fmaf(b.zeta, p_i + r.p, b.ext * (alpha_i + r.alpha));
-          tt = brk ? 0.0f : w * tt * r.vol * b.ry;
-        }
+        const Nb2 r = nb2_load(L, lj[it]);
+        const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, r.x0, r.x1, r.x2, r.u0,
+                                      r.u1, r.u2));
+        float tt = pair_tv<INFL>(b, p_i, d_i, r.vol, r.vp, r.vd, hhat, a.pmb_c);
+        tt = brk ? 0.0f : tt;
         f0 = fmaf(tt, b.y0, f0);
         f1 = fmaf(tt, b.y1, f1);
         f2 = fmaf(tt, b.y2, f2);
@@ -511,7 +606,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     __syncthreads();
     if (has_q) {
       const int k = q / 3, l3 = q - 3 * k;
-      verlet_epilogue(va, n0 + k, l3, L.sf[q], slot_u(L, own + k, l3), rec_next, vp);
+      verlet_epilogue(va, n0 + k, l3, L.sf[q], u_own, rec_next, vp);
     }
   }
   __syncthreads();  // LDS is reused by the next tile
