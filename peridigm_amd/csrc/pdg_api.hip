@@ -284,6 +284,9 @@ int Ctx::build_tiles(const std::vector<int>& bounds) {
       std::sort(H.begin(), H.end());
       H.erase(std::unique(H.begin(), H.end()), H.end());
       if ((int)H.size() > kTileHaloMax) { over = 1; continue; }
+      // the kernels take a wave's trip count from its first node
+      for (int i = n0 + 1; i < n1; i++)
+        if (row[i + 1] - row[i] > row[i] - row[i - 1]) { over = 1; break; }
       own[t] = (int)(std::lower_bound(H.begin(), H.end(), n0) - H.begin());
       for (int k = n0; k < n1; k++)
         for (int b = row[k]; b < row[k + 1]; b++)

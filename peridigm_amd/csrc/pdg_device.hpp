@@ -181,9 +181,9 @@ struct VerletPre {
 
 __device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, int l) {
   VerletPre p;
-  p.rho = va.rho[node];
-  p.v = va.v[3 * node + l];
-  const int slot = va.bc_slot[node];  // loaded unconditionally, selected after
+  p.rho = va.rho[(unsigned)node];
+  p.v = va.v[3u * (unsigned)node + (unsigned)l];
+  const int slot = va.bc_slot[(unsigned)node];  // loaded unconditionally, selected after
   p.slot = va.start_step ? slot : -1;
   return p;
 }

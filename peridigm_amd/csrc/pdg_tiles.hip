@@ -87,7 +87,7 @@ __device__ __forceinline__ unsigned long long valid_bits(int deg, int w) {
 __device__ __forceinline__ void load_words(const unsigned long long* __restrict__ mask, int w0,
                                            int deg, unsigned long long (&m)[NWORDS]) {
   const int nw = (deg + 63) >> 6;
-  const unsigned long long* __restrict__ p = mask + w0;
+  const unsigned long long* __restrict__ p = mask + (unsigned)w0;
 #pragma unroll
   for (int w = 0; w < NWORDS; w++) {
     const unsigned long long v = p[w];
@@ -99,12 +99,13 @@ __device__ __forceinline__ void load_words(const unsigned long long* __restrict_
 // address, constant offsets. Entries past the row's end belong to later rows
 // (or the zero padding), so they are byte offsets below kLdsSlot * HMAX; the
 // bonds they stand for are masked out, and the LDS slots they address always
-// hold finite values (staged or zero-filled), so those bonds' arithmetic
-// stays finite.
+// hold staged node data; such a bond's arithmetic may still give NaN (a slot
+// holding the node itself has xi = 0), so every use of a masked-out bond's
+// results goes through a select, never a multiplication by zero.
 template <int G>
 __device__ __forceinline__ void load_lcol(const unsigned short* __restrict__ lcol, int rs, int l,
                                           int (&lj)[MAXDEG / G]) {
-  const unsigned short* __restrict__ p = lcol + rs + l;
+  const unsigned short* __restrict__ p = lcol + (unsigned)(rs + l);
 #pragma unroll
   for (int it = 0; it < MAXDEG / G; it++) lj[it] = (int)p[it * G];
 }
@@ -208,8 +209,9 @@ __device__ __forceinline__ SlotVals slot_vals(const NodeRec* __restrict__ r) {
 }
 
 // Staging of the tile's halo into LDS: slot h < hn from the record of node
-// halo[h0 + h], slots hn..HMAX-1 zero-filled (masked-out bonds may address
-// them). Each thread handles slots threadIdx.x + r * WG. The three phases are
+// halo[h0 + h]; slots hn..HMAX-1 repeat the first halo node's record (finite
+// values: masked-out bonds may address them, and every use of a masked-out
+// bond's results goes through a select). Each thread handles slots threadIdx.x + r * WG. The three phases are
 // separate calls so a kernel can put every independent load of its prologue
 // in flight together: the halo indices (halo_idx), then the records
 // (halo_load, with the row data that depends on the node's CSR range), then
@@ -218,9 +220,9 @@ __device__ __forceinline__ SlotVals slot_vals(const NodeRec* __restrict__ r) {
 // V (alpha - p) and V p after z in array c.
 constexpr int kSlotsPerThread = (HMAX + WG - 1) / WG;
 
-// Loads are unconditional (a slot past the halo reads its first entry) and
-// the values are selected in halo_write, so no branch makes the compiler wait
-// between the phases. A tile's halo holds at least its own nodes (hn >= 1).
+// Loads are unconditional (a slot past the halo reads its first entry), so
+// no branch makes the compiler wait between the phases. A tile's halo holds
+// at least its own nodes (hn >= 1).
 struct HaloStage {
   int hn;
   int idx[kSlotsPerThread];
@@ -243,12 +245,11 @@ __device__ __forceinline__ void halo_load(HaloStage& hs, const NodeRec* __restri
 
 template <bool PASS2>
 __device__ __forceinline__ void halo_write(TileLds& L, const HaloStage& hs) {
-  const SlotVals zero{0.0, 0.0, 0.0, 0u, 0u, 0u, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int r = 0; r < kSlotsPerThread; r++) {
     const int hh = threadIdx.x + r * WG;
     const int h = hh < HMAX ? hh : HMAX;  // the spare slot
-    const SlotVals& v = hh < hs.hn ? hs.v[r] : zero;
+    const SlotVals& v = hs.v[r];
     lds_put(lds_a(L), h, make_uint4(v.x0, v.x1, v.x2, __float_as_uint(v.vol)));
     lds_put(lds_b(L), h, make_double2(v.u0, v.u1));
     if (PASS2) {
@@ -292,10 +293,11 @@ template <int G>
 __device__ __forceinline__ void row_scalars(const BondArgs& a, int node, bool valid, int fallback,
                                             RowData<G>& r) {
   const int nd = valid ? node : fallback;
-  r.rs = a.row[nd];
-  r.re = a.row[nd + 1];
-  r.w0 = a.mw[nd];
-  r.c = a.nc[nd];
+  // unsigned indices: no sign extension in the 64-bit address arithmetic
+  r.rs = a.row[(unsigned)nd];
+  r.re = a.row[(unsigned)nd + 1u];
+  r.w0 = a.mw[(unsigned)nd];
+  r.c = a.nc[(unsigned)nd];
 }
 
 template <int G>
@@ -312,6 +314,14 @@ __device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, i
                                          bool valid, int fallback, int l, RowData<G>& r) {
   row_scalars<G>(a, node, valid, fallback, r);
   row_arrays<G>(a, t, l, valid, r);
+}
+
+// Bond-loop trip count of a wave: inside a tile nodes are ordered by
+// descending degree (checked when the tiles are built), so the wave's first
+// node group has the largest degree.
+template <int G>
+__device__ __forceinline__ int wave_trips(int deg) {
+  return (__builtin_amdgcn_readfirstlane(deg) + G - 1) / G;
 }
 
 // Lane masks of the wave (compare results kept in scalar registers):
@@ -382,7 +392,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     // a time-dependent critical stretch replaces every damage-enabled node's
     const float scz = (a.sc_now > 0.0f && c.sc > 0.0f) ? a.sc_now : c.sc;
     const float nlim = a.near_rel * scz;
-    const int nmax = groups_max<G>((deg + G - 1) / G);
+    const int nmax = wave_trips<G>(deg);
     float acc = 0.0f;
     int nnear = 0;               // plain loops: per lane
     unsigned nnear_w = 0;        // pipelined loops: per wave (scalar)
@@ -427,8 +437,11 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
         }
       }
       if (DAMAGE) {
+        // new breaks are rare: a wave-uniform test skips the iterations
+        // without any
 #pragma unroll
         for (int it = 0; it < MAXDEG / G; ++it) {
+          if (balv[it] == 0ull) continue;
           const unsigned long long gb =
               G == 64 ? balv[it] : ((balv[it] >> (g * G)) & ((1ull << (G & 63)) - 1ull));
           mn[(it * G) >> 6] |= gb << ((it * G) & 63);
@@ -465,7 +478,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
     if (NEAR) nnear = group_sum_i<G>(nnear);
     if (valid && l == 0) {
       theta[node] = c.ctheta * acc;
-      rec[node].p = c.kp * acc;
+      rec[(unsigned)node].p = c.kp * acc;
       if (DAMAGE) {
         const int nw = (deg + 63) >> 6;
         int nb = 0;
@@ -526,7 +539,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     // unconditional (a thread past the tile's components reads component 0)
     const int qq = has_q ? q : 0;
     vp = verlet_pre(va, n0 + qq / 3, qq - 3 * (qq / 3));
-    u_own = rec[n0 + qq / 3].u[qq - 3 * (qq / 3)];
+    u_own = rec[(unsigned)(n0 + qq / 3)].u[qq - 3 * (qq / 3)];
   }
   halo_load(hs, rec);
   row_arrays<G>(a, t, l, n0 + grp < n1, rd);
@@ -554,7 +567,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     const double ui0 = me.u0, ui1 = me.u1, ui2 = me.u2;
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
-      const int nmax = groups_max<G>((deg + G - 1) / G);
+      const int nmax = wave_trips<G>(deg);
       Nb2 nb[2];
       nb[0] = nb2_load(L, lj[0]);
 #pragma unroll
