@@ -46,8 +46,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # Source: profiles/bond_passes_tiled_plate_400x400x4.md. Reported only for
 # that configuration.
 MEASURED_TRAFFIC = {
-    "pass1": (94833.0 * 2.0 + 25002.7 * 1.0) * 1024,
-    "pass2": (101719.7 * 2.0 + 36232.9 * 1.0) * 1024,
+    "pass1": (94792.0 * 2.0 + 25002.7 * 1.0) * 1024,
+    "pass2": (101676.7 * 2.0 + 36232.9 * 1.0) * 1024,
 }
 
 
