@@ -182,6 +182,38 @@ int pdg_verlet_run(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEv
 int pdg_verlet_run_persistent(pdg_ctx* ctx, int nsteps, double dt, double t0, int initialEval,
                               int* gridSize);
 
+/* ---- short-range contact ------------------------------------------------
+ * src/contact/Peridigm_ShortRangeForceContactModel.cpp:98-258 and
+ * src/core/Peridigm_ContactManager.cpp:700-750. Every node takes part, with
+ * the context's horizon. */
+/* ContactManager (Search Radius, Search Frequency) with one
+ * ShortRangeForceContactModel (Contact Radius, Spring Constant, Friction
+ * Coefficient); contactRadius <= 0 removes the model. Set after the
+ * constants; pdg_set_topology and pdg_set_constants remove it. With a model
+ * set, pdg_verlet_run searches at its initial evaluation and at every step
+ * whose number (counted from the last initial evaluation) is a multiple of
+ * searchFrequency, in the previous step's configuration (src/core/
+ * Peridigm.cpp:1468-1472), and adds the contact force density to the
+ * internal one before the acceleration (:1559-1565); Force_Density then
+ * holds the sum, as in the reference. */
+int pdg_set_contact(pdg_ctx* ctx, double searchRadius, int searchFrequency,
+                    double contactRadius, double springConstant,
+                    double frictionCoefficient);
+/* ContactManager::contactSearch: for each node of the configuration y (3N),
+ * the nodes within the search radius (squared distance <= R^2), itself and
+ * its bonded neighbours (broken or not) excluded; flat list
+ * [n, j1..jn]..., neighbours ascending. *listSize: capacity in, size out
+ * (contactList may be null to query the size). */
+int pdg_contact_search(pdg_ctx* ctx, const double* y, int* contactList, long long* listSize);
+/* ShortRangeForceContactModel::computeForce: Contact_Force_Density (3N) of
+ * configuration y and velocity v (3N) for a flat contact list (any list: each
+ * listed pair acts on both of its nodes, as in the reference). */
+int pdg_contact_compute_force(pdg_ctx* ctx, double dt, const double* y, const double* v,
+                              const int* contactList, long long listSize,
+                              double* contactForceDensity);
+/* Contact_Force_Density (3N) of the last pdg_verlet_run evaluation. */
+int pdg_contact_force_density(pdg_ctx* ctx, double* contactForceDensity);
+
 /* Stable time step (ComputeCriticalTimeStep, constant horizon,
  * src/core/Peridigm_CriticalTimeStep.cpp:54-126) for density rho, computed on
  * the device: min over nodes of sqrt(2 rho / sum_j V_j 18K/(pi delta^4) / |xi|). */

@@ -286,11 +286,38 @@ def compute_force_pmb(x, y, vol, bond_damage, nlist, K, horizon, planar=0, heigh
     return f
 
 
+def contact_search(y, R, nlist):
+    """Contact neighbour list (flat [n, j...]) of the configuration y: points
+    within R, bonded neighbours and the point itself excluded."""
+    y = _c(y, np.float64); nlist = _c(nlist, np.int32)
+    N = y.shape[0]
+    sz = C.c_longlong(0)
+    lib().orc_contact_search(C.c_int(N), _p(y), C.c_double(R), _p(nlist), None, C.c_longlong(0),
+                             C.byref(sz))
+    out = np.zeros(sz.value, np.int32)
+    lib().orc_contact_search(C.c_int(N), _p(y), C.c_double(R), _p(nlist), _p(out),
+                             C.c_longlong(sz.value), C.byref(sz))
+    return out
+
+
+def contact_force(y, v, vol, clist, rc, ks, mu_f, horizon):
+    """Short-range contact force density for the contact list clist."""
+    y = _c(y, np.float64); v = _c(v, np.float64); vol = _c(vol, np.float64)
+    clist = _c(clist, np.int32)
+    f = np.zeros((y.shape[0], 3))
+    lib().orc_contact_force(C.c_int(y.shape[0]), _p(y), _p(v), _p(vol), _p(clist),
+                            C.c_double(rc), C.c_double(ks), C.c_double(mu_f), C.c_double(horizon),
+                            _p(f))
+    return f
+
+
 def explicit_run_ext(x, vol, rho, nlist, K, mu, horizon, sc, dt, nsteps, material=0, planar=0,
                      height=0.0, sc_eval=None, u0=None, v0=None, bond_damage=None, bcs=(),
-                     t0=0.0, infl=0):
+                     t0=0.0, infl=0, contact=None):
     """explicit_run with a material model (0 elastic LPS, 1 elastic bond
-    based) and optional per-evaluation critical stretches sc_eval (nsteps+1)."""
+    based), optional per-evaluation critical stretches sc_eval (nsteps+1) and
+    optional short-range contact: contact = dict(search_radius,
+    search_frequency, contact_radius, spring_constant, friction)."""
     x = _c(x, np.float64); vol = _c(vol, np.float64)
     N = x.shape[0]
     rho = _c(np.broadcast_to(rho, (N,)), np.float64)
@@ -300,19 +327,25 @@ def explicit_run_ext(x, vol, rho, nlist, K, mu, horizon, sc, dt, nsteps, materia
     v = np.zeros((N, 3)) if v0 is None else _c(v0, np.float64).copy()
     bd = np.zeros(B) if bond_damage is None else _c(bond_damage, np.float64).copy()
     a = np.zeros((N, 3)); th = np.zeros(N); f = np.zeros((N, 3)); dmg = np.zeros(N); m = np.zeros(N)
+    fc = np.zeros((N, 3))
     ptr, nodes, coord, c0, c1 = _bc_arrays(bcs)
     sce = None if sc_eval is None else _c(sc_eval, np.float64)
     if sce is not None:
         assert sce.size == nsteps + 1
+    ct = contact or {}
     rc = lib().orc_explicit_run_ext(
         C.c_int(N), _p(x), _p(vol), _p(rho), _p(nlist), C.c_int(material), C.c_int(planar),
         C.c_double(height), C.c_double(K), C.c_double(mu), C.c_double(horizon), C.c_int(infl),
         C.c_double(sc), None if sce is None else _p(sce), C.c_double(dt), C.c_double(t0),
         C.c_int(nsteps), C.c_int(len(bcs)), _p(ptr), _p(nodes), _p(coord), _p(c0), _p(c1),
-        _p(u), _p(v), _p(a), _p(bd), _p(dmg), _p(th), _p(f), _p(m))
+        C.c_double(ct.get("search_radius", 0.0)), C.c_int(ct.get("search_frequency", 0)),
+        C.c_double(ct.get("contact_radius", 0.0)), C.c_double(ct.get("spring_constant", 0.0)),
+        C.c_double(ct.get("friction", 0.0)),
+        _p(u), _p(v), _p(a), _p(bd), _p(dmg), _p(th), _p(f), _p(fc), _p(m))
     if rc != 0:
         raise FloatingPointError("oracle explicit run: non-finite force")
-    return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f, m=m)
+    return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f, m=m,
+                contact_force=fc)
 
 
 def force_at(x, u, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):

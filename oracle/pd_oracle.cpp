@@ -665,19 +665,39 @@ void orc_compute_force_pmb(int N, const double* x, const double* y, const double
 // evaluation (sc_eval[k] for the evaluation that ends step k, k = 0..nsteps:
 // "Time Dependent Critical Stretch" evaluated at every step,
 // src/core/Peridigm.cpp:1447-1463); sc > 0 enables damage.
+int orc_contact_search(int N, const double* y, double R, const int* nlist, int* out,
+                       long long cap, long long* size);
+void orc_contact_force(int N, const double* y, const double* v, const double* vol,
+                       const int* clist, double rc, double ks, double muf, double horizon,
+                       double* f);
+
+// With contact (crc > 0; src/core/Peridigm.cpp:1468-1472, 1515-1524,
+// 1559-1565): the contact list is rebuilt from the configuration at the start
+// of every step whose number is a multiple of cfreq (and at step 0), and
+// every evaluation adds the contact force density (velocity after the first
+// half kick) to the internal one; fcontact receives the last evaluation's.
 int orc_explicit_run_ext(int N, const double* x, const double* vol, const double* rho,
                          const int* nlist, int material, int planar, double height, double K,
                          double mu, double horizon, int infl, double sc, const double* sc_eval,
                          double dt, double t0, int nsteps, int nbc, const int* bc_ptr,
                          const int* bc_nodes, const int* bc_coord, const double* bc_c0,
-                         const double* bc_c1, double* u, double* v, double* a,
-                         double* bond_damage, double* damage, double* theta, double* force,
+                         const double* bc_c1, double csearch, int cfreq, double crc, double cks,
+                         double cmu, double* u, double* v, double* a, double* bond_damage,
+                         double* damage, double* theta, double* force, double* fcontact,
                          double* m) {
   BCs bc{nbc, bc_ptr, bc_nodes, bc_coord, bc_c0, bc_c1};
   std::vector<double> y(3 * N);
   orc_weighted_volume(N, x, vol, nlist, horizon, infl, m);
   for (int i = 0; i < 3 * N; i++) y[i] = x[i] + u[i];
   for (int i = 0; i < N; i++) damage[i] = 0.0;
+  const bool contact = crc > 0.0;
+  std::vector<int> clist;
+  auto search = [&]() {
+    long long sz = 0;
+    orc_contact_search(N, y.data(), csearch, nlist, nullptr, 0, &sz);
+    clist.assign((size_t)sz, 0);
+    orc_contact_search(N, y.data(), csearch, nlist, clist.data(), sz, &sz);
+  };
   auto eval = [&](int k) {
     if (sc > 0.0)
       damage_impl(N, x, y.data(), bond_damage, damage, nlist, sc_eval ? sc_eval[k] : sc, nullptr,
@@ -690,12 +710,18 @@ int orc_explicit_run_ext(int N, const double* x, const double* vol, const double
       orc_compute_force(N, x, y.data(), m, vol, bond_damage, nlist, K, mu, horizon, infl, theta,
                         force);
     }
+    if (contact) {
+      orc_contact_force(N, y.data(), v, vol, clist.data(), crc, cks, cmu, horizon, fcontact);
+      for (int i = 0; i < 3 * N; i++) force[i] += fcontact[i];
+    }
   };
+  if (contact) search();
   eval(0);
   for (int i = 0; i < 3 * N; i++) { a[i] = force[i]; a[i] += 0.0; a[i] /= rho[i / 3]; }
   double dt2 = dt / 2.0;
   double tc = t0, tp = t0;
   for (int step = 1; step <= nsteps; step++) {
+    if (contact && cfreq > 0 && step % cfreq == 0) search();
     tp = tc;
     tc = t0 + (step * dt);
     for (int i = 0; i < 3 * N; i++) v[i] = v[i] + dt2 * a[i];
@@ -710,6 +736,116 @@ int orc_explicit_run_ext(int N, const double* x, const double* vol, const double
     for (int i = 0; i < 3 * N; i++) v[i] = v[i] + dt2 * a[i];
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Short-range contact
+// ---------------------------------------------------------------------------
+
+// Contact search (src/core/Peridigm_ContactManager.cpp:700-750, the search
+// itself the neighbourhood search of src/io/pdneigh/NeighborhoodList.cxx:652
+// with the contact search radius): for every point, the points within
+// searchRadius of it in the configuration y (dx^2 + dy^2 + dz^2 <= R^2), the
+// point itself and its bonded neighbours (nlist, broken or not) excluded.
+// Flat [n, j...] layout, neighbours in ascending index (brute force: test
+// sizes). *size receives the list length; out may be null (size query).
+int orc_contact_search(int N, const double* y, double R, const int* nlist, int* out,
+                       long long cap, long long* size) {
+  std::vector<std::vector<int>> bonded(N);
+  long long p = 0;
+  for (int i = 0; i < N; i++) {
+    const int nn = nlist[p++];
+    for (int k = 0; k < nn; k++) bonded[i].push_back(nlist[p++]);
+    std::sort(bonded[i].begin(), bonded[i].end());
+  }
+  const double R2 = R * R;
+  long long n = 0;
+  std::vector<int> row;
+  for (int i = 0; i < N; i++) {
+    row.clear();
+    for (int j = 0; j < N; j++) {
+      if (j == i) continue;
+      const double dx = y[3 * j] - y[3 * i], dy = y[3 * j + 1] - y[3 * i + 1],
+                   dz = y[3 * j + 2] - y[3 * i + 2];
+      if (dx * dx + dy * dy + dz * dz <= R2 &&
+          !std::binary_search(bonded[i].begin(), bonded[i].end(), j))
+        row.push_back(j);
+    }
+    if (out && n + 1 + (long long)row.size() <= cap) {
+      out[n] = (int)row.size();
+      for (size_t k = 0; k < row.size(); k++) out[n + 1 + k] = row[k];
+    }
+    n += 1 + (long long)row.size();
+  }
+  *size = n;
+  return (out && n > cap) ? -1 : 0;
+}
+
+// ShortRangeForceContactModel::computeForce
+// (src/contact/Peridigm_ShortRangeForceContactModel.cpp:98-258): the
+// contact force density is zeroed, then every listed pair (i, j) closer than
+// the contact radius adds its normal force (and, with a nonzero friction
+// coefficient, its friction force) to both points, in list order, with the
+// reference's expressions.
+void orc_contact_force(int N, const double* y, const double* v, const double* vol,
+                       const int* clist, double rc, double ks, double muf, double horizon,
+                       double* f) {
+  for (int i = 0; i < 3 * N; i++) f[i] = 0.0;
+  const double pi = 3.14159265358979323846;
+  const double rc2 = rc * rc;
+  long long p = 0;
+  for (int i = 0; i < N; i++) {
+    const int nn = clist[p++];
+    if (nn <= 0) continue;
+    const double X0 = y[3 * i], X1 = y[3 * i + 1], X2 = y[3 * i + 2];
+    const double V0 = v[3 * i], V1 = v[3 * i + 1], V2 = v[3 * i + 2];
+    const double vi = vol[i];
+    for (int k = 0; k < nn; k++) {
+      const int j = clist[p++];
+      const double d0 = y[3 * j] - X0, d1 = y[3 * j + 1] - X1, d2 = y[3 * j + 2] - X2;
+      const double dsq = d0 * d0 + d1 * d1 + d2 * d2;
+      if (!(dsq < rc2)) continue;
+      const double d = std::sqrt(dsq);
+      const double c = 9.0 * ks / (pi * horizon * horizon * horizon * horizon);
+      const double temp = c * (rc - d) / horizon;
+      const double vj = vol[j];
+      if (muf != 0.0) {
+        const double n0 = d0 / d, n1 = d1 / d, n2 = d2 / d;
+        const double cn = V0 * n0 + V1 * n1 + V2 * n2;
+        const double nbn = v[3 * j] * n0 + v[3 * j + 1] * n1 + v[3 * j + 2] * n2;
+        const double cp0 = V0 - cn * n0, cp1 = V1 - cn * n1, cp2 = V2 - cn * n2;
+        const double np0 = v[3 * j] - nbn * n0, np1 = v[3 * j + 1] - nbn * n1,
+                     np2 = v[3 * j + 2] - nbn * n2;
+        const double m0 = 0.5 * (cp0 + np0), m1 = 0.5 * (cp1 + np1), m2 = 0.5 * (cp2 + np2);
+        const double cr0 = cp0 - m0, cr1 = cp1 - m1, cr2 = cp2 - m2;
+        const double nr0 = np0 - m0, nr1 = np1 - m1, nr2 = np2 - m2;
+        const double ncr = std::sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+        const double nnr = std::sqrt(nr0 * nr0 + nr1 * nr1 + nr2 * nr2);
+        const double cf0 = -(temp * vj * d0 / d), cf1 = -(temp * vj * d1 / d),
+                     cf2 = -(temp * vj * d2 / d);
+        const double nf0 = (temp * vi * d0 / d), nf1 = (temp * vi * d1 / d),
+                     nf2 = (temp * vi * d2 / d);
+        const double ncf = std::sqrt(cf0 * cf0 + cf1 * cf1 + cf2 * cf2);
+        const double nnf = std::sqrt(nf0 * nf0 + nf1 * nf1 + nf2 * nf2);
+        double cfr0 = 0.0, cfr1 = 0.0, cfr2 = 0.0, nfr0 = 0.0, nfr1 = 0.0, nfr2 = 0.0;
+        if (ncr != 0.0) {
+          cfr0 = -muf * ncf * cr0 / ncr; cfr1 = -muf * ncf * cr1 / ncr; cfr2 = -muf * ncf * cr2 / ncr;
+        }
+        if (nnr != 0.0) {
+          nfr0 = -muf * nnf * nr0 / nnr; nfr1 = -muf * nnf * nr1 / nnr; nfr2 = -muf * nnf * nr2 / nnr;
+        }
+        f[3 * i] += cf0 + cfr0; f[3 * i + 1] += cf1 + cfr1; f[3 * i + 2] += cf2 + cfr2;
+        f[3 * j] += nf0 + nfr0; f[3 * j + 1] += nf1 + nfr1; f[3 * j + 2] += nf2 + nfr2;
+      } else {
+        f[3 * i] -= temp * vj * d0 / d;
+        f[3 * i + 1] -= temp * vj * d1 / d;
+        f[3 * i + 2] -= temp * vj * d2 / d;
+        f[3 * j] += temp * vi * d0 / d;
+        f[3 * j + 1] += temp * vi * d1 / d;
+        f[3 * j + 2] += temp * vi * d2 / d;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -88,6 +88,10 @@ SIGNATURES = {
     "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),
     "pdg_verlet_run_persistent": (_i, [_vp, _i, _d, _d, _i, C.POINTER(_i)]),
     "pdg_critical_time_step": (_i, [_vp, _d, C.POINTER(_d)]),
+    "pdg_set_contact": (_i, [_vp, _d, _i, _d, _d, _d]),
+    "pdg_contact_search": (_i, [_vp, _vp, _vp, C.POINTER(_ll)]),
+    "pdg_contact_compute_force": (_i, [_vp, _d, _vp, _vp, _vp, _ll, _vp]),
+    "pdg_contact_force_density": (_i, [_vp, _vp]),
     "pdg_critical_time_step_noalloc": (_i, [_vp, _d, _vp]),
     "pdg_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
     "pdg_set_per_pass_timing": (_i, [_vp, _i]),

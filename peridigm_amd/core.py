@@ -155,6 +155,41 @@ class Context:
                                                _p(th), _p(f)))
         return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f)
 
+    # -------------------------------------------------------------- contact
+    def set_contact(self, search_radius, search_frequency, contact_radius, spring_constant,
+                    friction=0.0):
+        """ContactManager + ShortRangeForceContactModel parameters
+        (src/core/Peridigm_ContactManager.cpp, src/contact/
+        Peridigm_ShortRangeForceContactModel.cpp); contact_radius <= 0 removes
+        the model. verlet_run then searches and adds the contact force."""
+        self.check(self.lib.pdg_set_contact(self.h, float(search_radius), int(search_frequency),
+                                            float(contact_radius), float(spring_constant),
+                                            float(friction)))
+
+    def contact_search(self, y):
+        """ContactManager::contactSearch of configuration y: flat [n, j...] list."""
+        y = _f64(y, (self.N, 3))
+        size = C.c_longlong(0)
+        self.check(self.lib.pdg_contact_search(self.h, _p(y), None, C.byref(size)))
+        out = np.zeros(max(size.value, 1), np.int32)
+        self.check(self.lib.pdg_contact_search(self.h, _p(y), _p(out), C.byref(size)))
+        return out[:size.value]
+
+    def contact_compute_force(self, y, v, contact_list, dt=0.0):
+        """ShortRangeForceContactModel::computeForce: contact force density (N, 3)."""
+        y = _f64(y, (self.N, 3)); v = _f64(v, (self.N, 3))
+        cl = _i32(contact_list)
+        f = np.zeros((self.N, 3))
+        self.check(self.lib.pdg_contact_compute_force(self.h, float(dt), _p(y), _p(v), _p(cl),
+                                                      int(cl.size), _p(f)))
+        return f
+
+    def contact_force_density(self):
+        """Contact force density (N, 3) of the last verlet_run evaluation."""
+        f = np.zeros((self.N, 3))
+        self.check(self.lib.pdg_contact_force_density(self.h, _p(f)))
+        return f
+
     def bc_array(self, bcs):
         """ctypes pdg_bc array of (node_ids, coord, c0, c1): u_coord(t) = c1*t + c0;
         the node arrays are kept alive on the context."""

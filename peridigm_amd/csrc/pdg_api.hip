@@ -1115,10 +1115,14 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
     c->tev.push_back(e);
   }
   int nevals = 0;
+  const bool contact = contact_on(c);
+  va.fadd = contact ? contact_force_ptr(c) : nullptr;
   // pass1 + pass2 of one force evaluation, finishing step `k` (k=0: initial)
   auto eval = [&](int k, bool end_kick) -> int {
     int r = 0;
-    if (nev) r = c->hip(hipEventRecord(c->tev[3 * nevals], s), "event");
+    // contact force at the evaluation's positions and half-step velocities
+    if (contact) r = contact_step(c, k, dt, c->d_rec[c->cur ^ 1], c->d_rec[c->cur]);
+    if (!r && nev) r = c->hip(hipEventRecord(c->tev[3 * nevals], s), "event");
     // a time-dependent critical stretch at the time this evaluation ends
     c->sc_cur = c->sc_at(step_time(k));
     if (!r) r = c->hip(c->pass1(c->d_rec[c->cur], c->d_nbroken, damage, damage), "pass1");
@@ -1172,6 +1176,7 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   if (!c->tiled()) return c->fail(PDG_E_UNSUPPORTED, "persistent run needs the tiled kernels");
   if (!c->sc_t.empty())
     return c->fail(PDG_E_UNSUPPORTED, "persistent run with a time-dependent critical stretch");
+  if (contact_on(c)) return c->fail(PDG_E_UNSUPPORTED, "persistent run with contact");
   const int tl = c->tile_lanes();
   if (tl != 4 && tl != 8 && tl != 16)
     return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 4, 8 or 16 lanes per node");

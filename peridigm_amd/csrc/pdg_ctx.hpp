@@ -12,6 +12,7 @@ namespace pdg {
 
 struct QsState;   // quasi-static solver workspace (pdg_qs.hip)
 struct GnnState;  // learned preconditioner (pdg_gnn.hip)
+struct ContactState;  // short-range contact (pdg_contact.hip)
 
 struct Ctx {
   int device = 0;
@@ -113,11 +114,13 @@ struct Ctx {
 
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
+  ContactState* contact = nullptr;  // null: no contact model
   unsigned gnn_epoch = 0;  // bumped whenever the network's buffers are (re)allocated
   bool qs_abs_tol = false;         // Newton tolerance is absolute (no reaction scaling)
   double qs_tol_multiplier = 1.0;  // reaction norm used by the last pdg_qs_newton
 
   void free_gnn();
+  void free_contact();
   double* qs_dinv();
   const signed char* qs_fixed();
   int fail(int code, const std::string& msg);
@@ -131,6 +134,16 @@ struct Ctx {
   int download_vec3(const double* dsrc, double* out);
   int upload_vec3(const double* in, double* ddst);
 };
+
+// short-range contact (pdg_contact.hip): whether a contact model is set, its
+// force density of the last explicit evaluation (device, internal order), and
+// evaluation k (0: initial) of an explicit run with step dt: a search in the
+// configuration of rec_prev (the step's start) when one is due, then the
+// force at y = x + u_prev + dt v with the context's half-step velocities v
+// (at k = 0: search and force at rec).
+bool contact_on(const Ctx* c);
+const double* contact_force_ptr(const Ctx* c);
+int contact_step(Ctx* c, int k, double dt, const NodeRec* rec_prev, const NodeRec* rec);
 
 // quasi-static solver (pdg_qs.hip) and learned preconditioner (pdg_gnn.hip)
 int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int maxit, int* iters,

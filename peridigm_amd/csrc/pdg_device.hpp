@@ -176,7 +176,8 @@ __device__ __forceinline__ float pair_tv(const BondK& b, float p_i, float d_i, f
 // The global inputs of one Verlet component, loadable ahead of the force.
 struct VerletPre {
   double rho, v;
-  int slot;  // boundary-condition slot (-1: none, or no new step starts)
+  double fadd;  // added force density (contact), 0 without
+  int slot;     // boundary-condition slot (-1: none, or no new step starts)
 };
 
 __device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, int l) {
@@ -185,6 +186,7 @@ __device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, 
   p.v = va.v[3u * (unsigned)node + (unsigned)l];
   const int slot = va.bc_slot[(unsigned)node];  // loaded unconditionally, selected after
   p.slot = va.start_step ? slot : -1;
+  p.fadd = va.fadd ? va.fadd[3u * (unsigned)node + (unsigned)l] : 0.0;
   return p;
 }
 
@@ -199,8 +201,10 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
                                                 const VerletPre& pre) {
   const int q = 3 * node + l;
   const double f = (double)fc;
-  if (!isfinite(f)) *va.err = 1;
-  double acc = f;
+  double tot = f;
+  if (va.fadd) tot += pre.fadd;  // contact force density
+  if (!isfinite(tot)) *va.err = 1;
+  double acc = tot;
   acc += 0.0;  // external force (none)
   acc /= pre.rho;
   double v = pre.v;
@@ -220,7 +224,7 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
     u = u + va.dt * v;
   } else {
     va.a[q] = acc;
-    va.f[q] = f;
+    va.f[q] = tot;
   }
   va.v[q] = v;
   rec_next[node].u[l] = u * va.inv_uq;

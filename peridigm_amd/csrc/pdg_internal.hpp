@@ -82,6 +82,7 @@ struct VerletArgs {
   double* v;          // 3n, FP64
   double* a;          // 3n, FP64 (written on the last step only)
   double* f;          // 3n, FP64 force density (written on the last step only)
+  const double* fadd; // 3n, FP64 force density added to the internal one (contact), or null
   const int* bc_slot; // n, -1 if none
   const BcSlot* bcs;
   double dt, t_cur, t_prev;

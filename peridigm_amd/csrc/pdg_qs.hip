@@ -735,6 +735,7 @@ void Ctx::free_extra() {
     qs = nullptr;
   }
   free_gnn();
+  free_contact();  // sized by the topology and horizon
 }
 
 // K x = b on device vectors in internal order; x holds the initial guess.

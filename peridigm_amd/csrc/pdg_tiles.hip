@@ -533,7 +533,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   row_scalars<G>(a, n0 + grp, n0 + grp < n1, n0, rd);
   const int q = threadIdx.x;
   const bool has_q = q < 3 * (n1 - n0);
-  VerletPre vp{0.0, 0.0, -1};
+  VerletPre vp{0.0, 0.0, 0.0, -1};
   double u_own = 0.0;  // this thread's component of the FP64 displacement (units of q)
   if (VERLET) {
     // unconditional (a thread past the tile's components reads component 0)

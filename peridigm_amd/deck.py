@@ -20,8 +20,12 @@ Supported subset (anything else raises DeckError):
     device integrator applies them;
   - elastic constants: any two of bulk, shear, Young's modulus and Poisson's
     ratio (src/materials/Peridigm_Material.cpp:335-433);
-  - one block group (all the mesh's blocks, one material and horizon);
-    influence function "One", "Parabolic Decay" or "Gaussian".
+  - block groups that together hold all the mesh's blocks and share one
+    material, horizon, damage model and influence function ("One",
+    "Parabolic Decay" or "Gaussian"); "Omit Bonds Between Blocks" "All" or
+    "None" (src/io/discretization/Peridigm_ExodusDiscretization.cpp:709-760);
+  - short-range contact ("Contact": Search Radius, Search Frequency and one
+    "Short Range Force" model, every node taking part).
 """
 from __future__ import annotations
 
@@ -315,6 +319,11 @@ def discretize(deck, base_dir, horizon):
         mesh = EX.read_mesh(os.path.join(base_dir, d["Input Mesh File"]))
         x, vol, blk, sets = EX.discretize(mesh)
         nl = radius_search(x, horizon, planes)
+        omit = d.get("Omit Bonds Between Blocks", "None")
+        if omit == "All":
+            nl = omit_bonds_between_blocks(nl, blk)
+        elif omit != "None":
+            raise DeckError("Omit Bonds Between Blocks: All or None, not %r" % omit)
         return x, vol, nl, sets, [b[0] for b in mesh["blocks"]]
     if typ == "PdQuickGrid":
         g = d["TensorProduct3DMeshGenerator"]
@@ -329,6 +338,42 @@ def discretize(deck, base_dir, horizon):
     raise DeckError("discretization type %r not supported" % typ)
 
 
+def omit_bonds_between_blocks(nl, blk):
+    """The neighbour list without the bonds whose ends lie in different
+    blocks (ExodusDiscretization::filterBonds with "All")."""
+    nl = np.asarray(nl)
+    out = []
+    p = 0
+    for i in range(len(blk)):
+        n = int(nl[p])
+        row = nl[p + 1:p + 1 + n]
+        keep = row[blk[row] == blk[i]]
+        out.append(len(keep))
+        out.extend(int(j) for j in keep)
+        p += n + 1
+    return np.array(out, np.int32)
+
+
+def contact_params(deck):
+    """Contact parameters of the deck (None without a Contact list): search
+    radius and frequency and the first model's constants, as ContactManager
+    uses them (src/core/Peridigm_ContactManager.cpp:228-253)."""
+    c = deck.get("Contact")
+    if not c:
+        return None
+    models = list(c.get("Models", {}).values())
+    if not models:
+        raise DeckError("Contact needs a model")
+    m = models[0]
+    if m.get("Contact Model") != "Short Range Force":
+        raise DeckError("contact model %r not supported" % m.get("Contact Model"))
+    return dict(search_radius=float(c["Search Radius"]),
+                search_frequency=int(c["Search Frequency"]),
+                contact_radius=float(m["Contact Radius"]),
+                spring_constant=float(m["Spring Constant"]),
+                friction=float(m.get("Friction Coefficient", 0.0)))
+
+
 # ---------------------------------------------------------------- the run
 _COORD = {"x": 0, "y": 1, "z": 2}
 
@@ -341,9 +386,11 @@ def _bcs(deck, x, sets):
     presc = []
     sets = dict(sets)
     bcl = deck.get("Boundary Conditions", {})
-    # node sets listed in the deck itself: 1-based ids, space separated
+    # node sets listed in the deck itself (any plain parameter whose name
+    # holds "Node Set", src/core/Peridigm_BoundaryAndInitialConditionManager.cpp:80-86):
+    # 1-based ids, space separated
     for name, val in bcl.items():
-        if not isinstance(val, dict) and name.startswith("Node Set"):
+        if not isinstance(val, dict) and "NODE SET" in name.upper():
             sets[name] = np.array([int(tk) - 1 for tk in str(val).split()], np.int64)
     for name, bc in bcl.items():
         if not isinstance(bc, dict):
@@ -377,9 +424,11 @@ def run_deck(path, output=None, lanes=4):
     deck = parse_deck(path)
     base = os.path.dirname(os.path.abspath(path))
     groups = list(deck["Blocks"].values())
-    if len(groups) != 1:
-        raise DeckError("one block group (one material and horizon) is supported")
     blk = groups[0]
+    for g in groups[1:]:
+        for key in ("Material", "Horizon", "Damage Model", "Influence Function"):
+            if g.get(key) != blk.get(key):
+                raise DeckError("block groups must share one %s" % key.lower())
     horizon = float(blk["Horizon"])
     mat = deck["Materials"][blk["Material"]]
     model = mat.get("Material Model")
@@ -412,7 +461,7 @@ def run_deck(path, output=None, lanes=4):
         else:
             raise DeckError("damage model %r not supported" % dm.get("Damage Model"))
     x, vol, nl, sets, mesh_blocks = discretize(deck, base, horizon)
-    named = str(blk.get("Block Names", "")).split()
+    named = [nm for g in groups for nm in str(g.get("Block Names", "")).split()]
     if named and set(named) != set(mesh_blocks):
         raise DeckError("the block group must hold every block of the mesh (%s)" % mesh_blocks)
     N = x.shape[0]
@@ -429,6 +478,9 @@ def run_deck(path, output=None, lanes=4):
         if model == "Elastic Bond Based":
             ctx.set_material_model(model, planar, height)
         ctx.set_constants(x, vol, K, mu, horizon, infl, sc)
+        contact = contact_params(deck)
+        if contact is not None:
+            ctx.set_contact(**contact)
         m = ctx.initialize()
         deg = _degrees(nl, N)
         frames = {}
@@ -441,6 +493,8 @@ def run_deck(path, output=None, lanes=4):
                 "Displacement": st["u"], "Velocity": st["v"], "Force_Density": f,
                 "Force": f * vol[:, None],
             }
+            if "contact_force" in st:
+                fields["Contact_Force_Density"] = st["contact_force"]
             for name, arr in fields.items():
                 if name in want:
                     for c, ax in enumerate("XYZ"):
@@ -470,13 +524,13 @@ def run_deck(path, output=None, lanes=4):
             ctx.upload_state(u=u0, v=v0)
             done = 0
             ctx.verlet_run(0, dt, t0=t0, initial_eval=True)
-            record(t0, _explicit_state(ctx))
+            record(t0, _explicit_state(ctx, contact is not None))
             while done < nsteps:
                 k = min(freq, nsteps - done)
                 ctx.verlet_run(k, dt, t0=t0 + done * dt, initial_eval=False)
                 done += k
                 if done % freq == 0 or done == nsteps:
-                    record(t0 + done * dt, _explicit_state(ctx))
+                    record(t0 + done * dt, _explicit_state(ctx, contact is not None))
             info = dict(solver="Verlet", dt=dt, steps=nsteps)
         elif "QuasiStatic" in solver:
             from . import quasistatic as Q
@@ -503,7 +557,8 @@ def run_deck(path, output=None, lanes=4):
                x=x, volume=vol, nlist=nl, info=info)
     if output:
         nodal = {k: v for k, v in res["frames"].items()
-                 if k[:-1] in ("Displacement", "Velocity", "Force_Density", "Force")}
+                 if k[:-1] in ("Displacement", "Velocity", "Force_Density", "Force",
+                               "Contact_Force_Density")}
         elem = {k: v for k, v in res["frames"].items() if k not in nodal}
         EX.write_results(output, x, res["times"], nodal, elem)
     return res
@@ -518,10 +573,13 @@ def _degrees(nl, N):
     return out
 
 
-def _explicit_state(ctx):
+def _explicit_state(ctx, contact=False):
     st = ctx.download_state()
-    return dict(u=st["u"], v=st["v"], force=st["force"], dilatation=st["theta"],
-                damage=st["damage"])
+    out = dict(u=st["u"], v=st["v"], force=st["force"], dilatation=st["theta"],
+               damage=st["damage"])
+    if contact:
+        out["contact_force"] = ctx.contact_force_density()
+    return out
 
 
 def _qs_state(ctx, u, Q):

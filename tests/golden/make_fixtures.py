@@ -296,6 +296,39 @@ def bond_breaking_time_dependent(ref):
     return out
 
 
+def contact_2x1x1(ref):
+    """Contact_2x1x1 (verification): two unbonded points 2 mm apart at the
+    contact radius closing at 1 mm/ms each; short-range contact force only;
+    1000 Verlet steps, 11 frames."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/Contact_2x1x1")
+    f = netcdf_file(os.path.join(d, "Contact_2x1x1_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-13), tol_velocity=np.array(1e-13))
+    out.update(_exodus_all_blocks(f, list(range(len(time)))))
+    f.close()
+    return out
+
+
+def contact_friction(ref):
+    """Contact_Friction (verification): two blocks of an elastic material
+    without bonds between them, driven together at 10 m/s; short-range
+    contact with friction coefficient 0.3."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/Contact_Friction")
+    f = netcdf_file(os.path.join(d, "Contact_Friction_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-13), tol_velocity=np.array(1e-13),
+               tol_contact_force=np.array(5e-6))
+    out.update(_exodus_all_blocks(f, list(range(len(time)))))
+    f.close()
+    return out
+
+
 def copy_decks(ref, here):
     """The input decks and meshes of the WaveInBar and PrecrackedPlate
     regression tests (data files, copied unchanged)."""
@@ -309,7 +342,10 @@ def copy_decks(ref, here):
                       ("test/verification/ElasticBondBased/ElasticBondBased3D/ElasticBondBased3D.yaml", None),
                       ("test/verification/ElasticBondBased/ElasticBondBased3D/ElasticBondBased3D.g", None),
                       ("test/verification/BondBreakingInitialVelocity_TimeDependentCS/"
-                       "BondBreakingInitialVelocity.xml", "BondBreakingInitialVelocity_TimeDependentCS.xml")):
+                       "BondBreakingInitialVelocity.xml", "BondBreakingInitialVelocity_TimeDependentCS.xml"),
+                      ("test/verification/Contact_2x1x1/Contact_2x1x1.xml", None),
+                      ("test/verification/Contact_Friction/Contact_Friction.xml", None),
+                      ("test/verification/Contact_Friction/Contact_Friction.g", None)):
         shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, name or os.path.basename(rel)))
 
 
@@ -324,6 +360,9 @@ def main():
     np.savez_compressed(os.path.join(here, "bond_breaking_time_dependent.npz"),
                         **bond_breaking_time_dependent(ref))
     print("wrote elastic_bond_based_3d.npz, bond_breaking_time_dependent.npz")
+    np.savez_compressed(os.path.join(here, "contact_2x1x1.npz"), **contact_2x1x1(ref))
+    np.savez_compressed(os.path.join(here, "contact_friction.npz"), **contact_friction(ref))
+    print("wrote contact_2x1x1.npz, contact_friction.npz")
     copy_decks(ref, here)
     print("copied decks")
     fx = {

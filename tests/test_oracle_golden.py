@@ -544,3 +544,85 @@ def test_time_dependent_critical_stretch_oracle_against_gold(golden_dir):
         for c, ax in enumerate("XYZ"):
             assert np.abs(r["u"][:, c] - g["Displacement" + ax][f]).max() <= float(g["tol_displacement"])
             assert np.abs(r["v"][:, c] - g["Velocity" + ax][f]).max() <= float(g["tol_velocity"])
+
+
+def _contact_deck(name):
+    import os
+    from peridigm_amd import deck as Dk
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks", name)
+    return Dk.parse_deck(path), path
+
+
+def test_contact_2x1x1_oracle_against_gold(golden_dir):
+    """Contact_2x1x1 (verification deck): two unbonded points at the contact
+    radius closing at 1 mm/ms; the oracle's contact search and short-range
+    force (src/contact/Peridigm_ShortRangeForceContactModel.cpp:98-258) and
+    its Verlet loop with contact reproduce the 11 gold frames at the .comp
+    tolerances (1e-13 absolute)."""
+    import os
+    g = np.load(os.path.join(golden_dir, "contact_2x1x1.npz"))
+    d, _ = _contact_deck("Contact_2x1x1.xml")
+    hz = float(list(d["Blocks"].values())[0]["Horizon"])
+    gen = d["Discretization"]["TensorProduct3DMeshGenerator"]
+    x, vol, nl = O.quickgrid(gen["Number Points X"], gen["Number Points Y"], gen["Number Points Z"],
+                             [gen["X Origin"], gen["Y Origin"], gen["Z Origin"]],
+                             [gen["X Length"], gen["Y Length"], gen["Z Length"]], hz, spherical=True)
+    assert np.abs(x - g["coords"]).max() <= 1e-12
+    assert O.num_bonds(nl, 2) == 0
+    mat = list(d["Materials"].values())[0]
+    cm = list(d["Contact"]["Models"].values())[0]
+    contact = dict(search_radius=d["Contact"]["Search Radius"],
+                   search_frequency=d["Contact"]["Search Frequency"],
+                   contact_radius=cm["Contact Radius"], spring_constant=cm["Spring Constant"],
+                   friction=cm.get("Friction Coefficient", 0.0))
+    v0 = np.zeros((2, 3)); v0[0, 0] = 1.0; v0[1, 0] = -1.0
+    dt = d["Solver"]["Verlet"]["Fixed dt"]
+    every = d["Output"]["Output Frequency"]
+    for f in range(len(g["time"])):
+        r = O.explicit_run_ext(x, vol, mat["Density"], nl, mat["Bulk Modulus"], mat["Shear Modulus"],
+                               hz, 0.0, dt, f * every, v0=v0, contact=contact)
+        for c, ax in enumerate("XYZ"):
+            assert np.abs(r["u"][:, c] - g["Displacement" + ax][f]).max() <= float(g["tol_displacement"]), (f, ax)
+            assert np.abs(r["v"][:, c] - g["Velocity" + ax][f]).max() <= float(g["tol_velocity"]), (f, ax)
+    # the points do come into contact: the final velocities differ from the initial
+    assert np.abs(r["v"] - v0).max() > 1e-9
+
+
+def test_contact_friction_oracle_against_gold(golden_dir):
+    """Contact_Friction (verification deck): two single-element blocks (no
+    bonds between blocks) meeting obliquely at 10 m/s with friction 0.3; the
+    oracle reproduces the gold's displacements and velocities at the .comp
+    tolerances and its contact force density within 5e-6 (absolute, on
+    values of about 3e9)."""
+    import os
+    from peridigm_amd import exodus as EX
+    g = np.load(os.path.join(golden_dir, "contact_friction.npz"))
+    d, path = _contact_deck("Contact_Friction.xml")
+    x, vol, blk, sets = EX.discretize(EX.read_mesh(os.path.join(os.path.dirname(path),
+                                                                 d["Discretization"]["Input Mesh File"])))
+    assert np.abs(x - g["coords"]).max() <= 1e-12
+    hz = float(list(d["Blocks"].values())[0]["Horizon"])
+    nl = np.zeros(2, np.int32)  # one point per block, no bonds between blocks
+    mat = list(d["Materials"].values())[0]
+    cm = list(d["Contact"]["Models"].values())[0]
+    contact = dict(search_radius=d["Contact"]["Search Radius"],
+                   search_frequency=d["Contact"]["Search Frequency"],
+                   contact_radius=cm["Contact Radius"], spring_constant=cm["Spring Constant"],
+                   friction=cm["Friction Coefficient"])
+    v0 = np.zeros((2, 3))
+    peak = 0.0
+    for bc in d["Boundary Conditions"].values():
+        if isinstance(bc, dict) and bc.get("Type") == "Initial Velocity":
+            v0[sets[bc["Node Set"]], "xyz".index(bc["Coordinate"])] = float(bc["Value"])
+    dt = d["Solver"]["Verlet"]["Fixed dt"]
+    every = d["Output"]["Output Frequency"]
+    for f in range(len(g["time"])):
+        r = O.explicit_run_ext(x, vol, mat["Density"], nl, mat["Bulk Modulus"], mat["Shear Modulus"],
+                               hz, 0.0, dt, f * every, v0=v0, contact=contact)
+        for c, ax in enumerate("XYZ"):
+            assert np.abs(r["u"][:, c] - g["Displacement" + ax][f]).max() <= float(g["tol_displacement"]), (f, ax)
+            assert np.abs(r["v"][:, c] - g["Velocity" + ax][f]).max() <= float(g["tol_velocity"]), (f, ax)
+            assert np.abs(r["contact_force"][:, c] - g["Contact_Force_Density" + ax][f]).max() <= float(g["tol_contact_force"]), (f, ax)
+        peak = max(peak, np.abs(r["contact_force"]).max())
+    # the blocks touch during the first two output frames, then separate
+    assert peak > 1e9
