@@ -19,7 +19,8 @@ given) on a bounded sample and checks the GPU against FP64: `accuracy` holds
 the force/dilatation error at the timed run's end state and the displacement
 and velocity error of the sample's trajectory.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plate|bar|cylinder|batched]
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+                    [--workload plate|plate_contact|bar|cylinder|batched]
                     [--lanes 4|8|16|32|64] [--persistent] [--no-qs] [--no-cpu-baseline]
                     [--no-accuracy]
 """
@@ -72,6 +73,8 @@ def build(workload):
     from peridigm_amd import problems
     if workload == "plate":
         p = problems.plate()
+    elif workload == "plate_contact":
+        p = problems.plate_contact()
     elif workload == "bar":
         p = problems.bar()
     elif workload == "cylinder":
@@ -290,6 +293,8 @@ def main():
                       float(np.max(sc)) if per_node_sc else sc)
     if per_node_sc:
         ctx.set_critical_stretch(sc)
+    if "contact" in p:
+        ctx.set_contact(**p["contact"])
     ctx.set_density(np.full(N, p["density"]))
     ctx.set_bcs(p["bcs"])
     ctx.upload_state(v=p["v0"])
@@ -352,6 +357,9 @@ def main():
             "workload": {
                 "plate": "notched plate 400x400x4 LPS + critical stretch 5e-4, h=0.5mm, "
                          "horizon 3.015h, +-2 m/s pull, fixed dt 2.5e-8 s",
+                "plate_contact": "notched plate 400x400x4 as 'plate', with short-range contact "
+                                 "on every node: search radius = horizon every 100 steps, "
+                                 "contact radius 0.9h, spring constant 15K",
                 "bar": "LPS elastic bar 100x10x10, h=1mm, horizon 3.015h, dt 1e-8 s",
                 "cylinder": "expanding hollow cylinder 40h-50h x 200h, h=1mm, perturbed, sc 1.5e-3",
                 "batched": "64 notched plates 5k-60k nodes (seed 2024), sc U[3e-4,1e-3], "

@@ -12,12 +12,12 @@
 //
 // Search: positions hashed into a spatial grid of cells no smaller than the
 // search radius (2^k buckets, k the smallest with 2^k >= 2N), points sorted by
-// bucket. A point visits the buckets of its 27 surrounding cells and accepts
-// a candidate only in the visit of the candidate's own cell, so a bucket
-// shared by several cells (a hash collision) yields each point once; the
-// distance predicate alone decides membership, as in the reference. Bonded
-// neighbours are found by binary search in the point's bond row, kept sorted
-// in a copy of the bond list.
+// bucket. A point visits each distinct bucket of its 27 surrounding cells
+// once; the distance predicate alone decides membership, as in the
+// reference. Bonded neighbours are found by binary search in the point's bond
+// row, kept sorted in a copy of the bond list. One pass counts every point's
+// contacts and keeps the first 32; rows that long or shorter are copied into
+// the list, and only longer ones are walked a second time.
 //
 // Force: the search list is symmetric (j lists i exactly when i lists j: the
 // squared distance and the bond lists are symmetric), and the reference adds
@@ -56,6 +56,13 @@ struct ContactState {
   long long *cnt = nullptr, *off = nullptr;  // n+1 each (cnt[n] = 0)
   int* list = nullptr;
   long long cap = 0, size = 0;
+  int* act = nullptr;   // n: points with contacts
+  int* nact_d = nullptr;
+  int nact = 0;
+  int* keep = nullptr;  // n x kKeep: first contacts of every point (first pass)
+  int* over = nullptr;  // n: points with more than kKeep contacts
+  int* nover_d = nullptr;
+  int nover = 0;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   // drop-in call buffers (reference order)
@@ -140,41 +147,66 @@ __device__ __forceinline__ bool bonded(const int* __restrict__ bcol, int lo, int
   return false;
 }
 
-// Contact neighbours of every point: counts (FILL = false) or the list.
+constexpr int kKeep = 32;  // contacts per point kept by the first search pass
+
+// Contact neighbours of a point: the buckets of the 27 cells around it, each
+// visited once (a bucket that two of the cells share, a hash collision, is
+// skipped the second time, so no point is met twice; points of other cells
+// in a visited bucket fail the distance predicate, being farther than one
+// cell). FILL = false: the first pass over every point, threads taking the
+// points in bucket order so the lanes of a wave mostly share a cell (same
+// candidates, uniform trip counts); counts every contact and keeps the
+// first kKeep in `keep`. FILL = true: the points of `pts` (those with more
+// than kKeep), their whole rows written at off[i], in the same visit order.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_contact_rows(
-    int n, const double* __restrict__ y, double R2, double inv_cs, unsigned mask,
-    const int* __restrict__ bstart, const int* __restrict__ bend, const int* __restrict__ sidx,
-    const int* __restrict__ brow, const int* __restrict__ bcol, long long* __restrict__ cnt,
-    const long long* __restrict__ off, int* __restrict__ list) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+    int n, const int* __restrict__ pts, const double* __restrict__ y, double R2, double inv_cs,
+    unsigned mask, const int* __restrict__ bstart, const int* __restrict__ bend,
+    const int* __restrict__ sidx, const int* __restrict__ brow, const int* __restrict__ bcol,
+    long long* __restrict__ cnt, int* __restrict__ keep, const long long* __restrict__ off,
+    int* __restrict__ list) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = pts[t];
   const double Y0 = y[3 * i], Y1 = y[3 * i + 1], Y2 = y[3 * i + 2];
   const int a = cell_of(Y0, inv_cs), b = cell_of(Y1, inv_cs), c = cell_of(Y2, inv_cs);
   const int r0 = brow[i], r1 = brow[i + 1];
-  long long o = FILL ? off[i] : 0;
+  int* const out = FILL ? list + off[i] : keep + (size_t)i * kKeep;
   long long count = 0;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        const int ca = a + dx, cb = b + dy, cc = c + dz;
-        const unsigned bk = bucket_of(ca, cb, cc, mask);
-        const int s = bstart[bk], e = bend[bk];
-        for (int q = s; q < e; q++) {
-          const int j = sidx[q];
-          if (j == i) continue;
-          const double y0 = y[3 * j], y1 = y[3 * j + 1], y2 = y[3 * j + 2];
-          // the candidate's own cell only (a bucket may hold several cells)
-          if (cell_of(y0, inv_cs) != ca || cell_of(y1, inv_cs) != cb || cell_of(y2, inv_cs) != cc)
-            continue;
-          const double d0 = y0 - Y0, d1 = y1 - Y1, d2 = y2 - Y2;
-          if (!(d0 * d0 + d1 * d1 + d2 * d2 <= R2)) continue;
-          if (bonded(bcol, r0, r1, j)) continue;
-          if (FILL) list[o++] = j;
-          count++;
-        }
-      }
+  for (int k = 0; k < 27; k++) {
+    const unsigned bk = bucket_of(a + k % 3 - 1, b + (k / 3) % 3 - 1, c + k / 9 - 1, mask);
+    bool dup = false;
+    for (int p = 0; p < k; p++)
+      dup |= bucket_of(a + p % 3 - 1, b + (p / 3) % 3 - 1, c + p / 9 - 1, mask) == bk;
+    if (dup) continue;
+    const int s = bstart[bk], e = bend[bk];
+    for (int q = s; q < e; q++) {
+      const int j = sidx[q];
+      const double d0 = y[3 * j] - Y0, d1 = y[3 * j + 1] - Y1, d2 = y[3 * j + 2] - Y2;
+      if (!(d0 * d0 + d1 * d1 + d2 * d2 <= R2) || j == i) continue;
+      if (bonded(bcol, r0, r1, j)) continue;
+      if (FILL || count < kKeep) out[count] = j;
+      count++;
+    }
+  }
   if (!FILL) cnt[i] = count;
+}
+
+// Rows of at most kKeep contacts copied from `keep` into the list; the
+// points with more appended to `over` for the second pass.
+__global__ void k_contact_copy(int n, const long long* __restrict__ cnt,
+                               const long long* __restrict__ off, const int* __restrict__ keep,
+                               int* __restrict__ list, int* __restrict__ nover,
+                               int* __restrict__ over) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long m = cnt[i];
+  if (m > kKeep) {
+    over[atomicAdd(nover, 1)] = i;
+    return;
+  }
+  const long long o = off[i];
+  for (int k = 0; k < (int)m; k++) list[o + k] = keep[(size_t)i * kKeep + k];
 }
 
 // One pair of the short-range force model seen from `cur` (position X,
@@ -224,26 +256,43 @@ __device__ __forceinline__ bool contact_pair(const PairParams& P, const double X
   return true;
 }
 
-// Explicit-run contact force (internal order): each point gathers twice its
-// own side of every pair of its (symmetric) list. Positions as the reference
-// forms them, y = (x + u) + dt v, from the record of the step's start (u) and
-// the half-step velocity v (dt = 0 at the initial evaluation).
-__global__ __launch_bounds__(256) void k_contact_gather(
-    int n, const NodeRec* __restrict__ rec, const double* __restrict__ x, double q, double dt,
-    const double* __restrict__ v, const double* __restrict__ vol, const long long* __restrict__ off,
-    const int* __restrict__ list, PairParams P, double* __restrict__ fout) {
+// Points with a non-empty contact list, appended in any order (each point's
+// force is computed on its own, so the order does not reach the results).
+__global__ void k_contact_active(int n, const long long* __restrict__ cnt, int* __restrict__ nact,
+                                 int* __restrict__ act) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long e0 = off[i], e1 = off[i + 1];
+  if (i >= n || cnt[i] == 0) return;
+  act[atomicAdd(nact, 1)] = i;
+}
+
+constexpr int kGatherLanes = 16;  // lanes per point of the contact gather
+
+// Explicit-run contact force (internal order) of the points with contacts
+// (the others keep the zero written at the search): each point gathers twice
+// its own side of every pair of its (symmetric) list, kGatherLanes lanes per
+// point, lane l taking entries l, l + kGatherLanes, ..., then a lane-tree sum.
+// Positions as the reference forms them, y = (x + u) + dt v, from the record
+// of the step's start (u) and the half-step velocity v (dt = 0 at the
+// initial evaluation).
+__global__ __launch_bounds__(256) void k_contact_gather(
+    int nact, const int* __restrict__ act, const NodeRec* __restrict__ rec,
+    const double* __restrict__ x, double q, double dt, const double* __restrict__ v,
+    const double* __restrict__ vol, const long long* __restrict__ off,
+    const int* __restrict__ list, PairParams P, double* __restrict__ fout) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = t / kGatherLanes, lane = t % kGatherLanes;
+  const bool valid = a < nact;
+  const int i = valid ? act[a] : 0;
   double f[3] = {0.0, 0.0, 0.0};
-  if (e1 > e0) {
+  if (valid) {
+    const long long e0 = off[i], e1 = off[i + 1];
     double X[3], V[3];
     for (int l = 0; l < 3; l++) {
       V[l] = v[3 * i + l];
       X[l] = (x[3 * i + l] + rec[i].u[l] * q) + dt * V[l];
     }
     const double vc = vol[i];
-    for (long long e = e0; e < e1; e++) {
+    for (long long e = e0 + lane; e < e1; e += kGatherLanes) {
       const int j = list[e];
       double Y[3], W[3], fc[3], fn[3];
       for (int l = 0; l < 3; l++) {
@@ -254,7 +303,11 @@ __global__ __launch_bounds__(256) void k_contact_gather(
         for (int l = 0; l < 3; l++) f[l] += fc[l];
     }
   }
-  for (int l = 0; l < 3; l++) fout[3 * i + l] = 2.0 * f[l];
+  // every lane takes part in the shuffles (the group of a point is aligned)
+  for (int m = kGatherLanes / 2; m >= 1; m >>= 1)
+    for (int l = 0; l < 3; l++) f[l] += __shfl_xor(f[l], m, kGatherLanes);
+  if (valid && lane == 0)
+    for (int l = 0; l < 3; l++) fout[3 * i + l] = 2.0 * f[l];
 }
 
 // Drop-in contact force (reference order, any list): both sides of every
@@ -317,8 +370,9 @@ int tmp_reserve(Ctx* c, size_t bytes) {
 }
 
 // The contact list of the positions in S->y (internal order) into S->off /
-// S->list; synchronises the stream once (the list size).
-int search(Ctx* c) {
+// S->list; synchronises the stream once (the list size). For an explicit run
+// (`run`) also the list of points with contacts and a zero contact force.
+int search(Ctx* c, bool run) {
   ContactState* S = c->contact;
   const int N = c->N;
   hipStream_t s = c->stream;
@@ -340,8 +394,9 @@ int search(Ctx* c) {
   hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(T)), dim3(256), 0, s, T, S->bend, 0);
   hipLaunchKernelGGL(k_bucket_bounds, dim3(nb), dim3(256), 0, s, N, S->skey, S->bstart, S->bend);
   const double R2 = S->R * S->R;
-  hipLaunchKernelGGL((k_contact_rows<false>), dim3(nb), dim3(256), 0, s, N, S->y, R2, S->inv_cs,
-                     mask, S->bstart, S->bend, S->sidx, c->d_row, S->bcol, S->cnt, nullptr, nullptr);
+  hipLaunchKernelGGL((k_contact_rows<false>), dim3(nb), dim3(256), 0, s, N, S->sidx, S->y, R2,
+                     S->inv_cs, mask, S->bstart, S->bend, S->sidx, c->d_row, S->bcol, S->cnt,
+                     S->keep, nullptr, nullptr);
   rc = c->hip(hipGetLastError(), "contact count");
   bytes = 0;
   if (!rc) rc = c->hip(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->cnt, S->off, N + 1, s),
@@ -349,16 +404,36 @@ int search(Ctx* c) {
   if (!rc) rc = tmp_reserve(c, bytes);
   if (!rc) rc = c->hip(hipcub::DeviceScan::ExclusiveSum(S->tmp, bytes, S->cnt, S->off, N + 1, s),
                        "contact scan");
+  // points with contacts; the force of every other point is zero until the
+  // next search
+  if (!rc && run) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, 1ll, S->nact_d, 0);
+    hipLaunchKernelGGL(k_contact_active, dim3(nb), dim3(256), 0, s, N, S->cnt, S->nact_d, S->act);
+    hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(3ll * N)), dim3(256), 0, s, 3ll * N, S->fc);
+  }
   long long total = 0;
+  int nact = 0;
   if (!rc) rc = c->hip(hipMemcpyAsync(&total, S->off + N, 8, hipMemcpyDeviceToHost, s), "d2h size");
+  if (!rc && run) rc = c->hip(hipMemcpyAsync(&nact, S->nact_d, 4, hipMemcpyDeviceToHost, s), "d2h active");
   if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (run) S->nact = nact;
   if (!rc) rc = grow(c, S->list, S->cap, total, "alloc contact list");
   if (rc) return rc;
-  hipLaunchKernelGGL((k_contact_rows<true>), dim3(nb), dim3(256), 0, s, N, S->y, R2, S->inv_cs,
-                     mask, S->bstart, S->bend, S->sidx, c->d_row, S->bcol, nullptr, S->off, S->list);
-  rc = c->hip(hipGetLastError(), "contact fill");
+  // rows of at most kKeep from the first pass; the longer ones walked again
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, 1ll, S->nover_d, 0);
+  hipLaunchKernelGGL(k_contact_copy, dim3(nb), dim3(256), 0, s, N, S->cnt, S->off, S->keep,
+                     S->list, S->nover_d, S->over);
+  int nover = 0;
+  rc = c->hip(hipMemcpyAsync(&nover, S->nover_d, 4, hipMemcpyDeviceToHost, s), "d2h overflow");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (!rc && nover > 0)
+    hipLaunchKernelGGL((k_contact_rows<true>), dim3(blocks_for(nover)), dim3(256), 0, s, nover,
+                       S->over, S->y, R2, S->inv_cs, mask, S->bstart, S->bend, S->sidx, c->d_row,
+                       S->bcol, nullptr, nullptr, S->off, S->list);
+  if (!rc) rc = c->hip(hipGetLastError(), "contact fill");
   S->size = total;
-  S->built = true;
+  S->nover = nover;
+  S->built = run;
   return rc;
 }
 
@@ -369,7 +444,7 @@ void Ctx::free_contact() {
   ContactState* S = contact;
   void* ptrs[] = {S->y, S->fc, S->bcol, S->key, S->idx, S->skey, S->sidx, S->bstart,
                   S->bend, S->cnt, S->off, S->list, S->tmp, S->vref, S->dv, S->dfc,
-                  S->dhead, S->dlist};
+                  S->dhead, S->dlist, S->act, S->nact_d, S->keep, S->over, S->nover_d};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   delete S;
   contact = nullptr;
@@ -390,11 +465,13 @@ int contact_eval(Ctx* c, const NodeRec* rec_search, const NodeRec* rec, double d
   if (rec_search) {
     hipLaunchKernelGGL(k_contact_y, dim3(blocks_for(3ll * N)), dim3(256), 0, s, N, rec_search,
                        c->d_x, c->q, S->y);
-    rc = search(c);
+    rc = search(c, true);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_contact_gather, dim3(blocks_for(N)), dim3(256), 0, s, N, rec, c->d_x, c->q,
-                     dt, c->d_v, c->d_vol, S->off, S->list, pair_params(c), S->fc);
+  if (S->nact == 0) return PDG_OK;  // all zero since the search
+  hipLaunchKernelGGL(k_contact_gather, dim3(blocks_for((long long)S->nact * kGatherLanes)),
+                     dim3(256), 0, s, S->nact, S->act, rec, c->d_x, c->q, dt, c->d_v, c->d_vol,
+                     S->off, S->list, pair_params(c), S->fc);
   return c->hip(hipGetLastError(), "contact force");
 }
 
@@ -464,6 +541,11 @@ int pdg_set_contact(pdg_ctx* h, double searchRadius, int searchFrequency, double
   if (!rc) rc = A(S->cnt, N + 1ll, "alloc contact counts");
   if (!rc) rc = A(S->off, N + 1ll, "alloc contact offsets");
   if (!rc) rc = A(S->vref, N, "alloc contact volumes");
+  if (!rc) rc = A(S->act, N, "alloc contact active list");
+  if (!rc) rc = A(S->nact_d, 1, "alloc contact active count");
+  if (!rc) rc = A(S->keep, (long long)N * kKeep, "alloc contact rows");
+  if (!rc) rc = A(S->over, N, "alloc contact overflow list");
+  if (!rc) rc = A(S->nover_d, 1, "alloc contact overflow count");
   if (rc) { c->free_contact(); return rc; }
   // bond rows (internal ids) sorted ascending
   std::vector<int> bc(c->col.begin(), c->col.end());
@@ -491,10 +573,9 @@ int pdg_contact_search(pdg_ctx* h, const double* y, int* contactList, long long*
   if (rc) return rc;
   hipLaunchKernelGGL(k_contact_y_perm, dim3(blocks_for(3ll * N)), dim3(256), 0, s, N, c->d_y,
                      c->d_perm, S->y);
-  rc = search(c);
+  // the explicit run rebuilds its list from its own state afterwards
+  rc = search(c, false);
   if (rc) return rc;
-  // the explicit run rebuilds its list from its own state
-  S->built = false;
   const long long size = S->size + N;
   const long long cap = *listSize;
   *listSize = size;

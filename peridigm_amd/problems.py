@@ -17,6 +17,10 @@ check.
            0 <= z < 200h, positions perturbed by U[-0.05h, 0.05h] (seeded),
            radial velocity 200 (1 + U[-0.01, 0.01]) m/s, critical stretch
            1.5e-3, 500 steps.
+  plate_contact
+           the plate with short-range contact on every node: search radius
+           = horizon every 100 steps, contact radius 0.9h, spring constant
+           15 K (the ratio of the reference's Contact_2x1x1 deck).
   qs_bar   60x20x20 lattice, h = 1 mm, quasi-static tension to 1e-3 strain in
            10 load steps with prescribed x displacement on 3-node end layers.
 """
@@ -64,6 +68,14 @@ def plate(nx=400, ny=400, nz=4, h=5.0e-4, sc=5.0e-4, notch_cells=40, dt=2.5e-8, 
                 horizon=3.015 * h, critical_stretch=sc, dt=dt, nsteps=nsteps,
                 v0=np.zeros_like(x), bcs=bcs, search="radius",
                 planes=plate_notch_plane(h, nx, ny, nz, notch_cells)[None, :], **STEEL)
+
+
+def plate_contact(**kw):
+    p = plate(**kw)
+    p["name"] = "plate_contact"
+    p["contact"] = dict(search_radius=p["horizon"], search_frequency=100,
+                        contact_radius=0.9 * p["h"], spring_constant=15.0 * p["K"], friction=0.0)
+    return p
 
 
 def cylinder(h=1.0e-3, r_in=40, r_out=50, length=200, seed=1234, sc=1.5e-3, dt=5.0e-8, nsteps=500):
