@@ -205,36 +205,43 @@ def copy_bandwidth(mib=1024, reps=10):
     return 2.0 * mib * 1024 * 1024 / sec / 1e9
 
 
-def qs_figure(nsamples=100, epochs=20):
-    """Quasi-static tensile bar 60x20x20 (24k nodes), 10 load steps to 1e-3
-    strain, every linear solve to 1e-8 relative residual: wall time with
-    block-Jacobi CG and with the learned preconditioner in flexible CG (trained
-    on the device on residual/correction pairs from `nsamples` block-Jacobi
-    solves under random end loads, two intermediate iterates each)."""
+def qs_figure(nsamples=100, epochs=20, large=False):
+    """Quasi-static tensile bar 60x20x20 (24k nodes; 125x40x40, 200k nodes,
+    with `large`), 10 load steps to 1e-3 strain, every linear solve to 1e-8
+    relative residual: wall time with block-Jacobi CG and with the learned
+    preconditioner in flexible CG (trained on the device on
+    residual/correction pairs from `nsamples` block-Jacobi solves under random
+    end loads, two intermediate iterates each)."""
     from peridigm_amd import Context
     from peridigm_amd import learned as Lr
     from peridigm_amd import quasistatic as Q
-    from peridigm_amd.problems import qs_bar, neighbours
-    p = qs_bar()
+    from peridigm_amd.problems import qs_bar, qs_bar_200k, neighbours
+    p = qs_bar_200k() if large else qs_bar()
     nl = neighbours(p)
     N = len(p["volume"])
     out = {}
     with Context(0) as ctx:
         ctx.set_topology(nl, N)
         ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, 0.0)
+        def note(msg):  # progress on stderr (the JSON line stays alone on stdout)
+            print("bench.py: QS N=%d: %s" % (N, msg), file=sys.stderr, flush=True)
         u_j, st_j, wall_j = Q.run_tension(ctx, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8)
+        note("block-Jacobi run %.2f s" % wall_j)
         t0 = time.perf_counter()
         R, E, it_gen = Lr.generate_pairs(ctx, p, nsamples=nsamples, seed=11)
         t_gen = time.perf_counter() - t0
+        note("training pairs %.2f s" % t_gen)
         Lr.init(ctx, 5)
         t0 = time.perf_counter()
         loss = Lr.train(ctx, R, E, epochs=epochs, lr=2e-3, seed=3)
         t_train = time.perf_counter() - t0
+        note("training %.2f s" % t_train)
         u_l, st_l, wall_l = Q.run_tension(ctx, p, precond="learned", tol=1e-6, lin_tol=1e-8)
+        note("learned run %.2f s" % wall_l)
         fallbacks = Q.fallback_count(ctx)
     out = {
-        "workload": "QS tensile bar 60x20x20 (N=%d), 10 load steps to 1e-3 strain, "
-                    "linear solves to 1e-8 relative residual" % N,
+        "workload": "QS tensile bar %dx%dx%d (N=%d), 10 load steps to 1e-3 strain, "
+                    "linear solves to 1e-8 relative residual" % (*p["n"], N),
         "jacobi_pcg_s": wall_j, "learned_fcg_s": wall_l, "speedup": wall_j / wall_l,
         "jacobi_linear_iters": int(sum(s["linear"] for s in st_j)),
         "learned_linear_iters": int(sum(s["linear"] for s in st_l)),
@@ -260,6 +267,8 @@ def main():
                     help="skip the quasi-static solve figure (block-Jacobi CG vs learned FCG)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the force and trajectory errors against the FP64 CPU path")
+    ap.add_argument("--qs-large", action="store_true",
+                    help="also the 200k-node quasi-static bar (125x40x40) with both solvers")
     ap.add_argument("--persistent", action="store_true",
                     help="time the steps as one persistent cooperative launch")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -389,6 +398,8 @@ def main():
     out["config"]["near_critical_bonds_timed"] = near_timed
     if not args.no_qs:
         out["quasi_static"] = qs_figure()
+    if args.qs_large:
+        out["quasi_static_200k"] = qs_figure(nsamples=40, epochs=10, large=True)
     if not args.no_cpu_baseline:
         # the cpu_baseline leg: the FP64 CPU path timed on the host, and the
         # GPU results checked against it

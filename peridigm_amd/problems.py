@@ -23,6 +23,9 @@ check.
            15 K (the ratio of the reference's Contact_2x1x1 deck).
   qs_bar   60x20x20 lattice, h = 1 mm, quasi-static tension to 1e-3 strain in
            10 load steps with prescribed x displacement on 3-node end layers.
+  qs_bar_200k
+           the same tension on a 125x40x40 lattice (200,000 nodes), the
+           quasi-static variant of the batched-sweep configuration.
 """
 from __future__ import annotations
 
@@ -112,6 +115,12 @@ def qs_bar(nx=60, ny=20, nz=20, h=1.0e-3, strain=1.0e-3, load_steps=10):
     return dict(name="qs_bar", n=n, h=h, origin=[0.0, 0.0, 0.0], length=L, x=x, volume=vol,
                 horizon=3.015 * h, critical_stretch=0.0, left=left, right=right,
                 strain=strain, load_steps=load_steps, search="quickgrid", planes=None, **STEEL)
+
+
+def qs_bar_200k(**kw):
+    p = qs_bar(nx=125, ny=40, nz=40, **kw)
+    p["name"] = "qs_bar_200k"
+    return p
 
 
 def batched_plates(nspec=64, seed=2024, h=5.0e-4, nz=4, nodes=(5000, 60000), sc=(3e-4, 1e-3),

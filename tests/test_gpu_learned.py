@@ -83,3 +83,49 @@ def test_training_and_flexible_cg():
     kx[fixed] = xl.ravel()[fixed]
     assert np.linalg.norm(kx - r.ravel()) / np.linalg.norm(r) <= 1e-7
     assert np.abs(xl - xj).max() <= 1e-6 * np.abs(xj).max()
+
+
+def test_learned_fcg_on_200k_node_bar():
+    """The 200,000-node quasi-static bar (125x40x40): the network trained on
+    the device on this mesh's own block-Jacobi solves, then a tangent solve
+    with flexible CG to 1e-8. Checks: the reported residual, the true
+    residual |b - K x| / |b| from the device tangent (the block-Jacobi path's
+    operator), and the block-Jacobi CG solution."""
+    import time
+    from peridigm_amd import Context
+    from peridigm_amd import learned as Lr
+    from peridigm_amd import quasistatic as Q
+    from peridigm_amd.problems import qs_bar_200k, neighbours
+    p = qs_bar_200k()
+    nl = neighbours(p)
+    N = len(p["volume"])
+    assert N == 200000
+    with Context(0) as ctx:
+        ctx.set_topology(nl, N)
+        ctx.set_constants(p["x"], p["volume"], p["K"], p["mu"], p["horizon"], 0, 0.0)
+        t0 = time.perf_counter()
+        R, E, it_j = Lr.generate_pairs(ctx, p, nsamples=8, seed=4)
+        Lr.init(ctx, 3)
+        loss0 = Lr.train(ctx, R[:4], E[:4], epochs=1, lr=0.0)
+        loss = Lr.train(ctx, R, E, epochs=6, lr=2e-3, seed=5)
+        t_train = time.perf_counter() - t0
+        assert loss < loss0
+        Q.set_fixed(ctx, Q.end_dofs(p["left"], p["right"]))
+        u = np.zeros((N, 3))
+        rng = np.random.default_rng(8)
+        u[p["left"]] = rng.normal(0, 2e-5, 3)
+        u[p["right"]] = rng.normal(0, 2e-5, 3)
+        r, _ = Q.residual(ctx, u)
+        u0 = np.zeros((N, 3))
+        t0 = time.perf_counter()
+        xj, itj, relj = Q.solve(ctx, u0, r, precond="block_jacobi", rel_tol=1e-8)
+        tj = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xl, itl, rell = Q.solve(ctx, u0, r, precond="learned", rel_tol=1e-8)
+        tl = time.perf_counter() - t0
+        print(f"200k bar: loss {loss0:.3f}->{loss:.3f} (data+train {t_train:.1f} s); "
+              f"block-Jacobi CG {itj} it {tj:.2f} s, learned FCG {itl} it {tl:.2f} s")
+        assert relj <= 1e-8 and rell <= 1e-8
+        kx = Q.tangent_apply(ctx, u0, xl)
+        assert np.linalg.norm(kx - r) / np.linalg.norm(r) <= 2e-8
+        assert np.abs(xl - xj).max() <= 1e-6 * np.abs(xj).max()
