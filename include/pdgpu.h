@@ -46,6 +46,7 @@ extern "C" {
  * (src/materials/Peridigm_ElasticBondBasedMaterial.cpp) */
 #define PDG_MATERIAL_ELASTIC 0
 #define PDG_MATERIAL_ELASTIC_BOND_BASED 1
+#define PDG_MATERIAL_LINEAR_LPS 2
 
 /* influence functions (src/core/Peridigm_InfluenceFunction.hpp:69-88) */
 #define PDG_INFLUENCE_ONE 0
@@ -112,7 +113,11 @@ int pdg_set_critical_stretch(pdg_ctx* ctx, const double* criticalStretch);
  * src/materials/elastic_bond_based.cxx:55-130) takes the bulk modulus only;
  * planar = 1 (plane strain) or 2 (plane stress) uses c = 12K/(pi h delta^3)
  * with thickness `height` (:80-84). Its dilatation output is zero and the
- * quasi-static calls reject it. */
+ * quasi-static calls reject it. PDG_MATERIAL_LINEAR_LPS is the linearized
+ * LPS of "Linear LPS Partial Volume" without partial volumes
+ * (src/materials/Peridigm_LinearLPSPVMaterial.cpp:229-394: theta and force
+ * linear in u, the force along the reference bond); pdg_compute_force and
+ * the quasi-static calls evaluate it in FP64, the explicit runs reject it. */
 int pdg_set_material_model(pdg_ctx* ctx, int model, int planar, double height);
 
 /* Critical stretch used by every damage-enabled node from the next damage

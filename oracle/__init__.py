@@ -131,6 +131,20 @@ def compute_force(x, y, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):
     return th, f
 
 
+def compute_force_linear(x, y, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):
+    """Linearized LPS ("Linear LPS Partial Volume" without partial volumes):
+    returns (theta[N], force_density[N,3])."""
+    x = _c(x, np.float64); y = _c(y, np.float64); m = _c(m, np.float64)
+    vol = _c(vol, np.float64); bd = _c(bond_damage, np.float64); nlist = _c(nlist, np.int32)
+    N = x.shape[0]
+    th = np.zeros(N)
+    f = np.zeros((N, 3))
+    lib().orc_compute_force_linear(C.c_int(N), _p(x), _p(y), _p(m), _p(vol), _p(bd), _p(nlist),
+                                   C.c_double(K), C.c_double(mu), C.c_double(horizon),
+                                   C.c_int(infl), _p(th), _p(f))
+    return th, f
+
+
 def force_given_theta(x, y, m, vol, theta, bond_damage, nlist, K, mu, horizon, infl=0):
     x = _c(x, np.float64); y = _c(y, np.float64); m = _c(m, np.float64)
     vol = _c(vol, np.float64); th = _c(theta, np.float64); bd = _c(bond_damage, np.float64)

@@ -660,6 +660,76 @@ void orc_compute_force_pmb(int N, const double* x, const double* y, const double
   force_pmb(N, x, y, vol, bondDamage, f, nlist, K, horizon, planar, height);
 }
 
+// Linearized LPS without partial volumes ("Linear LPS Partial Volume",
+// src/materials/Peridigm_LinearLPSPVMaterial.cpp:229-289 precompute and
+// :296-394 computeForce, the influence function value of each bond taken
+// from the influence function at its reference length as initialize stores
+// it): theta_i = 3/m_i sum_j w (1-d) xi.(u_j - u_i) V_j, and every bond i->j
+// adds (1-d)(T1 xi + T2 (xi xi^T)(u_j - u_i)) with
+// T1 = (9K - 15mu) w theta_i / (3 m_i), T2 = 15 mu w / (m_i |xi|^2) to f_i
+// times V_j and subtracts it from f_j times V_i, in list order.
+void orc_compute_force_linear(int N, const double* x, const double* y, const double* m,
+                              const double* vol, const double* bondDamage, const int* nlist,
+                              double K, double mu, double horizon, int infl, double* theta,
+                              double* f) {
+  long long li = 0, bi = 0;
+  for (int p = 0; p < N; p++) {
+    theta[p] = 0.0;
+    const int nn = nlist[li++];
+    const double* xs = x + 3 * p;
+    const double* ys = y + 3 * p;
+    for (int k = 0; k < nn; k++, bi++) {
+      const int j = nlist[li++];
+      const double* xn = x + 3 * j;
+      const double* yn = y + 3 * j;
+      double zeta[3], u[3], un[3];
+      for (int c = 0; c < 3; c++) {
+        zeta[c] = xn[c] - xs[c];
+        u[c] = ys[c] - xs[c];
+        un[c] = yn[c] - xn[c];
+      }
+      const double nz = std::sqrt(zeta[0] * zeta[0] + zeta[1] * zeta[1] + zeta[2] * zeta[2]);
+      const double w = omega_fn(infl, nz, horizon);
+      const double dot = zeta[0] * (un[0] - u[0]) + zeta[1] * (un[1] - u[1]) + zeta[2] * (un[2] - u[2]);
+      theta[p] += w * (1.0 - bondDamage[bi]) * dot * vol[j];
+    }
+    if (nn > 0) theta[p] *= 3.0 / m[p];
+  }
+  for (int i = 0; i < 3 * N; i++) f[i] = 0.0;
+  li = 0;
+  bi = 0;
+  for (int p = 0; p < N; p++) {
+    const int nn = nlist[li++];
+    const double* xs = x + 3 * p;
+    const double* ys = y + 3 * p;
+    for (int k = 0; k < nn; k++, bi++) {
+      const int j = nlist[li++];
+      const double* xn = x + 3 * j;
+      const double* yn = y + 3 * j;
+      double zeta[3], u[3], un[3];
+      for (int c = 0; c < 3; c++) {
+        zeta[c] = xn[c] - xs[c];
+        u[c] = ys[c] - xs[c];
+        un[c] = yn[c] - xn[c];
+      }
+      const double z2 = zeta[0] * zeta[0] + zeta[1] * zeta[1] + zeta[2] * zeta[2];
+      const double w = omega_fn(infl, std::sqrt(z2), horizon);
+      const double t1 = (9.0 * K - 15.0 * mu) * w * theta[p] / (3.0 * m[p]);
+      const double t2 = 15.0 * mu * w / (m[p] * z2);
+      double mv[3];
+      for (int r = 0; r < 3; r++)
+        mv[r] = zeta[r] * zeta[0] * (un[0] - u[0]) + zeta[r] * zeta[1] * (un[1] - u[1]) +
+                zeta[r] * zeta[2] * (un[2] - u[2]);
+      const double dmg = 1.0 - bondDamage[bi];
+      for (int c = 0; c < 3; c++) {
+        const double fc = dmg * (t1 * zeta[c] + t2 * mv[c]);
+        f[3 * p + c] += fc * vol[j];
+        f[3 * j + c] -= fc * vol[p];
+      }
+    }
+  }
+}
+
 // Explicit run as orc_explicit_run_sc with a material model (0: elastic
 // LPS, 1: elastic bond based) and optionally one critical stretch per force
 // evaluation (sc_eval[k] for the evaluation that ends step k, k = 0..nsteps:

@@ -95,10 +95,12 @@ class Context:
         self.check(self.lib.pdg_set_critical_stretch(self.h, _p(v)))
 
     def set_material_model(self, model, planar=0, height=0.0):
-        """model: "Elastic" (0) or "Elastic Bond Based" (1); planar: 0 3D,
+        """model: "Elastic" (0), "Elastic Bond Based" (1) or "Linear LPS" (2,
+        the linearized LPS of "Linear LPS Partial Volume"); planar: 0 3D,
         1 plane strain, 2 plane stress (bond-based only). Call before
         set_constants."""
-        codes = {"Elastic": 0, "Elastic Bond Based": 1}
+        codes = {"Elastic": 0, "Elastic Bond Based": 1, "Linear LPS": 2,
+                 "Linear LPS Partial Volume": 2, "Elastic Partial Volume": 0}
         m = codes.get(model, model)
         self.check(self.lib.pdg_set_material_model(self.h, int(m), int(planar), float(height)))
 

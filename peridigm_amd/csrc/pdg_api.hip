@@ -894,7 +894,8 @@ int pdg_set_critical_stretch(pdg_ctx* h, const double* sc) {
 int pdg_set_material_model(pdg_ctx* h, int model, int planar, double height) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
-  if (model != PDG_MATERIAL_ELASTIC && model != PDG_MATERIAL_ELASTIC_BOND_BASED)
+  if (model != PDG_MATERIAL_ELASTIC && model != PDG_MATERIAL_ELASTIC_BOND_BASED &&
+      model != PDG_MATERIAL_LINEAR_LPS)
     return c->fail(PDG_E_ARG, "unknown material model");
   if (planar < 0 || planar > 2 || (planar && model != PDG_MATERIAL_ELASTIC_BOND_BASED))
     return c->fail(PDG_E_ARG, "plane strain/stress is a variant of the bond-based model");
@@ -982,6 +983,14 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || (c->N > 0 && (!y || !force))) return PDG_E_ARG;
   (void)hipSetDevice(c->device);
+  if (c->material == PDG_MATERIAL_LINEAR_LPS) {
+    // the linearized model through the FP64 quasi-static kernels at u = y - x
+    int r = c->upload_mask(bdNP1, true);
+    if (r) return r;
+    std::vector<double> u(3 * (size_t)c->N);
+    for (size_t q = 0; q < u.size(); q++) u[q] = y[q] - c->x[q];
+    return pdg_qs_force_dilatation(h, u.data(), force, dilatation);
+  }
   hipStream_t s = c->stream;
   const int N = c->N;
   // y in and f, theta out go through one pinned buffer (PCIe at full rate);
@@ -1088,6 +1097,8 @@ int pdg_set_bcs(pdg_ctx* h, const pdg_bc* bcs, int nb) {
 int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
+  if (c->material == PDG_MATERIAL_LINEAR_LPS)
+    return c->fail(PDG_E_UNSUPPORTED, "explicit runs implement the elastic and bond-based models");
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
   const bool damage = c->sc > 0.0;
@@ -1174,6 +1185,8 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
   if (!c->tiled()) return c->fail(PDG_E_UNSUPPORTED, "persistent run needs the tiled kernels");
+  if (c->material == PDG_MATERIAL_LINEAR_LPS)
+    return c->fail(PDG_E_UNSUPPORTED, "explicit runs implement the elastic and bond-based models");
   if (!c->sc_t.empty())
     return c->fail(PDG_E_UNSUPPORTED, "persistent run with a time-dependent critical stretch");
   if (contact_on(c)) return c->fail(PDG_E_UNSUPPORTED, "persistent run with contact");

@@ -113,6 +113,7 @@ struct QsArgs {
   double* diag;
   double horizon;
   int influence;
+  int linear;  // linearized LPS (force along the reference bond, linear in u)
 };
 
 // dilatation (+ its directional derivative, + d theta_i / d u_i)
@@ -131,10 +132,24 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
       if ((a.mask[w0 + (k >> 6)] >> (k & 63)) & 1ull) continue;
       const int j = a.col[rs + k];
       const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
+      const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+      if (a.linear) {
+        // theta_i = 3/m_i sum w V_j xi.(u_j - u_i)
+        // (src/materials/Peridigm_LinearLPSPVMaterial.cpp:229-289)
+        const double c = infl64(a.influence, zeta, a.horizon) * a.vol[j];
+        th += c * (X0 * (a.u[3 * j] - ui[0]) + X1 * (a.u[3 * j + 1] - ui[1]) +
+                   X2 * (a.u[3 * j + 2] - ui[2]));
+        if (DUAL) {
+          const double* wi = a.w + 3 * node;
+          dth += c * (X0 * (a.w[3 * j] - wi[0]) + X1 * (a.w[3 * j + 1] - wi[1]) +
+                      X2 * (a.w[3 * j + 2] - wi[2]));
+        }
+        if (DIAG) { g0 -= c * X0; g1 -= c * X1; g2 -= c * X2; }
+        continue;
+      }
       const double Y0 = X0 + (a.u[3 * j] - ui[0]);
       const double Y1 = X1 + (a.u[3 * j + 1] - ui[1]);
       const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
-      const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
       const double iy = 1.0 / yl;
       const double e = yl - zeta;
@@ -180,6 +195,28 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
       if ((a.mask[w0 + (k >> 6)] >> (k & 63)) & 1ull) continue;
       const int j = a.col[rs + k];
       const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
+      const double vj = a.vol[j];
+      if (a.linear) {
+        // both ends of bond i-j: w V_j [(theta_i b_i + theta_j b_j)
+        //   + (a_i + a_j) xi.(u_j - u_i) / |xi|^2] xi
+        // (src/materials/Peridigm_LinearLPSPVMaterial.cpp:296-394)
+        const double z2 = X0 * X0 + X1 * X1 + X2 * X2;
+        const double om = infl64(a.influence, sqrt(z2), a.horizon);
+        const double T2 = om * (ai + a.alpha[j]) / z2;
+        double s;
+        if (MODE != 1) {
+          const double pr = X0 * (a.u[3 * j] - ui[0]) + X1 * (a.u[3 * j + 1] - ui[1]) +
+                            X2 * (a.u[3 * j + 2] - ui[2]);
+          s = om * (thi * bi + a.theta[j] * a.beta[j]) + T2 * pr;
+        } else {
+          const double* wi = a.w + 3 * node;
+          const double pr = X0 * (a.w[3 * j] - wi[0]) + X1 * (a.w[3 * j + 1] - wi[1]) +
+                            X2 * (a.w[3 * j + 2] - wi[2]);
+          s = om * (dthi * bi + a.dtheta[j] * a.beta[j]) + T2 * pr;
+        }
+        f0 += s * X0 * vj; f1 += s * X1 * vj; f2 += s * X2 * vj;
+        continue;
+      }
       const double Y0 = X0 + (a.u[3 * j] - ui[0]);
       const double Y1 = X1 + (a.u[3 * j + 1] - ui[1]);
       const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
@@ -190,7 +227,6 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
       const double om = infl64(a.influence, zeta, a.horizon);
       const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
       const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
-      const double vj = a.vol[j];
       if (MODE != 1) {
         f0 += T * n0 * vj; f1 += T * n1 * vj; f2 += T * n2 * vj;
       } else {
@@ -233,6 +269,22 @@ __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
       if ((a.mask[w0 + (k >> 6)] >> (k & 63)) & 1ull) continue;
       const int j = a.col[rs + k];
       const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
+      const double vj = a.vol[j];
+      const double cj = a.m[j] > 0.0 ? 3.0 / a.m[j] : 0.0;
+      if (a.linear) {
+        // d/du_i of s xi with s = w (theta_i b_i + theta_j b_j) + T2 xi.(u_j - u_i):
+        // d theta_i/du_i = g_i, d theta_j/du_i = -(3/m_j) w V_i xi
+        const double X[3] = {X0, X1, X2};
+        const double z2 = X0 * X0 + X1 * X1 + X2 * X2;
+        const double om = infl64(a.influence, sqrt(z2), a.horizon);
+        const double T2 = om * (ai + a.alpha[j]) / z2;
+        const double hj = a.beta[j] * cj * om * vi;
+        double gs[3];
+        for (int c = 0; c < 3; c++) gs[c] = om * (bi * gg[c] - hj * X[c]) - T2 * X[c];
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++) D[3 * r + c] += vj * X[r] * gs[c];
+        continue;
+      }
       const double Y0 = X0 + (a.u[3 * j] - ui[0]);
       const double Y1 = X1 + (a.u[3 * j + 1] - ui[1]);
       const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
@@ -242,9 +294,7 @@ __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
       const double om = infl64(a.influence, zeta, a.horizon);
       const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
       const double n[3] = {Y0 / yl, Y1 / yl, Y2 / yl};
-      const double vj = a.vol[j];
       // grad_{u_i}(t_ij + t_ji) = om [b_i zeta g_i - a_i n - b_j zeta^2 om V_i (3/m_j) n - a_j n]
-      const double cj = a.m[j] > 0.0 ? 3.0 / a.m[j] : 0.0;
       const double sn = -ai - a.alpha[j] - a.beta[j] * zeta * zeta * om * vi * cj;
       double gt[3];
       for (int c = 0; c < 3; c++) gt[c] = om * (bi * zeta * gg[c] + sn * n[c]);
@@ -535,12 +585,14 @@ static QsArgs args(Ctx* c) {
   a.beta = S->beta; a.theta = S->theta; a.dtheta = S->dtheta; a.g = S->g; a.fixed = S->fixed;
   a.out = S->out; a.dinv = S->dinv; a.diag = S->diag; a.horizon = c->horizon;
   a.influence = c->infl;
+  a.linear = c->material == PDG_MATERIAL_LINEAR_LPS ? 1 : 0;
   return a;
 }
 
 static int ensure(Ctx* c) {
-  if (c->material != PDG_MATERIAL_ELASTIC)
-    return c->fail(PDG_E_UNSUPPORTED, "the quasi-static path implements the elastic (LPS) model");
+  if (c->material != PDG_MATERIAL_ELASTIC && c->material != PDG_MATERIAL_LINEAR_LPS)
+    return c->fail(PDG_E_UNSUPPORTED,
+                   "the quasi-static path implements the elastic and linearized LPS models");
   if (c->qs && c->qs->n == c->N) return PDG_OK;
   if (c->qs) c->free_extra();
   QsState* S = new QsState();

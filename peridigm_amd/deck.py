@@ -306,7 +306,8 @@ _PLANE_KEYS = ["Normal_X", "Normal_Y", "Normal_Z", "Lower_Left_Corner_X", "Lower
 
 
 def discretize(deck, base_dir, horizon):
-    """(x, volume, neighbour list, point sets) of the deck's discretization."""
+    """(x, volume, neighbour list, point sets, block names, block index of
+    every point) of the deck's discretization."""
     d = deck["Discretization"]
     typ = d.get("Type")
     planes = []
@@ -324,7 +325,7 @@ def discretize(deck, base_dir, horizon):
             nl = omit_bonds_between_blocks(nl, blk)
         elif omit != "None":
             raise DeckError("Omit Bonds Between Blocks: All or None, not %r" % omit)
-        return x, vol, nl, sets, [b[0] for b in mesh["blocks"]]
+        return x, vol, nl, sets, [b[0] for b in mesh["blocks"]], blk
     if typ == "PdQuickGrid":
         g = d["TensorProduct3DMeshGenerator"]
         n = [g["Number Points X"], g["Number Points Y"], g["Number Points Z"]]
@@ -334,7 +335,7 @@ def discretize(deck, base_dir, horizon):
             raise DeckError("PdQuickGrid does not take bond filters (as in the reference)")
         sph = d.get("NeighborhoodType", "Parallelepiped") == "Spherical"
         x, vol, nl = quickgrid(n, o, L, horizon, spherical=sph)
-        return x, vol, nl, {}, ["block_1"]
+        return x, vol, nl, {}, ["block_1"], np.zeros(len(vol), np.int64)
     raise DeckError("discretization type %r not supported" % typ)
 
 
@@ -417,23 +418,17 @@ def _bcs(deck, x, sets):
     return v0, u0, presc
 
 
-def run_deck(path, output=None, lanes=4):
-    """Runs a deck. Returns dict(times, frames {variable: (frames, N)}, x,
-    volume, neighbour list, info); with `output` a file name, also writes the
-    frames there in ExodusII."""
-    deck = parse_deck(path)
-    base = os.path.dirname(os.path.abspath(path))
-    groups = list(deck["Blocks"].values())
-    blk = groups[0]
-    for g in groups[1:]:
-        for key in ("Material", "Horizon", "Damage Model", "Influence Function"):
-            if g.get(key) != blk.get(key):
-                raise DeckError("block groups must share one %s" % key.lower())
-    horizon = float(blk["Horizon"])
-    mat = deck["Materials"][blk["Material"]]
+def _material(deck, grp):
+    """Material, damage and influence constants of a block group."""
+    mat = deck["Materials"][grp["Material"]]
     model = mat.get("Material Model")
     planar, height = 0, 0.0
-    if model == "Elastic":
+    if model in ("Elastic", "Elastic Partial Volume", "Linear LPS Partial Volume"):
+        # the partial-volume models without partial volumes (the element-horizon
+        # intersections are not computed): node volumes, as the reference uses
+        # them then
+        if mat.get("Use Partial Volume", False) or mat.get("Use Analytic Weighted Volume", False):
+            raise DeckError("partial volumes and analytic weighted volumes are not supported")
         K, mu = elastic_constants(mat)
     elif model == "Elastic Bond Based":
         # one constant; the second is Poisson's ratio 1/4 (1/3 in plane
@@ -448,11 +443,9 @@ def run_deck(path, output=None, lanes=4):
         K, mu = elastic_constants(one)
     else:
         raise DeckError("material model %r not supported" % model)
-    rho = float(mat["Density"])
-    infl = INFLUENCE[blk.get("Influence Function", "One")]
     sc, sc_fn = 0.0, None
-    if "Damage Model" in blk:
-        dm = deck["Damage Models"][blk["Damage Model"]]
+    if "Damage Model" in grp:
+        dm = deck["Damage Models"][grp["Damage Model"]]
         if dm.get("Damage Model") == "Critical Stretch":
             sc = float(dm["Critical Stretch"])
         elif dm.get("Damage Model") == "Time Dependent Critical Stretch":
@@ -460,25 +453,130 @@ def run_deck(path, output=None, lanes=4):
             sc = 1.0  # enables the damage model; the function sets the value
         else:
             raise DeckError("damage model %r not supported" % dm.get("Damage Model"))
-    x, vol, nl, sets, mesh_blocks = discretize(deck, base, horizon)
-    named = [nm for g in groups for nm in str(g.get("Block Names", "")).split()]
-    if named and set(named) != set(mesh_blocks):
-        raise DeckError("the block group must hold every block of the mesh (%s)" % mesh_blocks)
+    return dict(model=model, K=K, mu=mu, planar=planar, height=height,
+                rho=float(mat["Density"]), infl=INFLUENCE[grp.get("Influence Function", "One")],
+                sc=sc, sc_fn=sc_fn)
+
+
+def _subset(idx, N, nl, v0, u0, presc):
+    """The points idx (ascending) as a problem of their own: renumbered
+    neighbour list (no bond may leave the subset) and conditions."""
+    new = np.full(N, -1, np.int64)
+    new[idx] = np.arange(len(idx))
+    nl = np.asarray(nl)
+    heads = np.zeros(N, np.int64)
+    p = 0
+    for i in range(N):
+        heads[i] = p
+        p += int(nl[p]) + 1
+    out = []
+    for i in idx:
+        h = heads[i]
+        n = int(nl[h])
+        row = new[nl[h + 1:h + 1 + n]]
+        if n and row.min() < 0:
+            raise DeckError("bonds join block groups of different materials")
+        out.append(n)
+        out.extend(int(j) for j in row)
+    sub_presc = []
+    for ids, c, c0, c1 in presc:
+        keep = new[np.asarray(ids, np.int64)]
+        keep = keep[keep >= 0]
+        if keep.size:
+            sub_presc.append((keep, c, c0, c1))
+    return np.array(out, np.int32), v0[idx], u0[idx], sub_presc
+
+
+def run_deck(path, output=None, lanes=4):
+    """Runs a deck. Returns dict(times, frames {variable: (frames, N)}, x,
+    volume, neighbour list, info); with `output` a file name, also writes the
+    frames there in ExodusII. Block groups of different materials must share
+    one horizon and no bonds; each runs in a context of its own."""
+    deck = parse_deck(path)
+    base = os.path.dirname(os.path.abspath(path))
+    groups = list(deck["Blocks"].values())
+    horizons = {float(g["Horizon"]) for g in groups}
+    if len(horizons) != 1:
+        raise DeckError("block groups must share one horizon")
+    horizon = horizons.pop()
+    x, vol, nl, sets, mesh_blocks, blk = discretize(deck, base, horizon)
     N = x.shape[0]
+    owner = {}
+    for gi, g in enumerate(groups):
+        for nm in str(g.get("Block Names", "")).split():
+            owner[nm] = gi
+    if owner and set(owner) != set(mesh_blocks):
+        raise DeckError("the block groups must hold every block of the mesh (%s)" % mesh_blocks)
+    point_group = (np.array([owner[mesh_blocks[b]] for b in blk], np.int64) if owner
+                   else np.zeros(N, np.int64))
+    # groups of one material, damage model and influence function run together
+    sig = [(g.get("Material"), g.get("Damage Model"), g.get("Influence Function")) for g in groups]
+    run_of = {}
+    for s_ in sig:
+        run_of.setdefault(s_, len(run_of))
+    point_run = np.array([run_of[sig[g]] for g in point_group], np.int64)
     v0, u0, presc = _bcs(deck, x, sets)
+    contact = contact_params(deck)
+    if contact is not None and len(run_of) > 1:
+        raise DeckError("contact between block groups of different materials is not supported")
+    solver = deck["Solver"]
+    if len(run_of) > 1 and "Verlet" in solver and "Fixed dt" not in solver["Verlet"]:
+        raise DeckError("several materials need a fixed time step")
+    out = deck.get("Output", deck.get("Output1", {}))
+    want = {k for k, v in out.get("Output Variables", {}).items() if v}
+    times, frames, info = None, {}, {}
+    for s_, r in run_of.items():
+        idx = np.nonzero(point_run == r)[0]
+        if idx.size == 0:
+            continue
+        grp = groups[sig.index(s_)]
+        if len(run_of) == 1:
+            sub = (nl, v0, u0, presc)
+        else:
+            sub = _subset(idx, N, nl, v0, u0, presc)
+        t_r, f_r, info = _run_points(deck, _material(deck, grp), horizon, x[idx], vol[idx], sub,
+                                     contact, lanes)
+        if times is None:
+            times = t_r
+        elif len(t_r) != len(times) or np.abs(np.asarray(t_r) - times).max() > 0:
+            raise DeckError("block groups ran to different output times")
+        for name, arr in f_r.items():
+            if name not in frames:
+                frames[name] = np.zeros((arr.shape[0], N))
+            frames[name][:, idx] = arr
+    for name in ("Element_Id",):
+        if name in want:
+            frames[name] = np.tile(np.arange(1, N + 1, dtype=float), (len(times), 1))
+    res = dict(times=np.array(times), frames=frames, x=x, volume=vol, nlist=nl, info=info)
+    if output:
+        nodal = {k: v for k, v in res["frames"].items()
+                 if k[:-1] in ("Displacement", "Velocity", "Force_Density", "Force",
+                               "Contact_Force_Density")}
+        elem = {k: v for k, v in res["frames"].items() if k not in nodal}
+        EX.write_results(output, x, res["times"], nodal, elem)
+    return res
+
+
+def _run_points(deck, mat, horizon, x, vol, sub, contact, lanes):
+    """Runs the deck's solver on one set of points of one material: returns
+    (times, frames {variable: (frames, n)}, info)."""
+    nl, v0, u0, presc = sub
+    N = x.shape[0]
     solver = deck["Solver"]
     t0 = float(solver.get("Initial Time", 0.0))
     t1 = float(solver.get("Final Time", 1.0))
     out = deck.get("Output", deck.get("Output1", {}))
     freq = int(out.get("Output Frequency", 1))
     want = {k for k, v in out.get("Output Variables", {}).items() if v}
+    model, rho, sc, sc_fn = mat["model"], mat["rho"], mat["sc"], mat["sc_fn"]
     ctx = Context(0, lanes=lanes)
     try:
         ctx.set_topology(nl, N)
         if model == "Elastic Bond Based":
-            ctx.set_material_model(model, planar, height)
-        ctx.set_constants(x, vol, K, mu, horizon, infl, sc)
-        contact = contact_params(deck)
+            ctx.set_material_model(model, mat["planar"], mat["height"])
+        elif model == "Linear LPS Partial Volume":
+            ctx.set_material_model(model)
+        ctx.set_constants(x, vol, mat["K"], mat["mu"], horizon, mat["infl"], sc)
         if contact is not None:
             ctx.set_contact(**contact)
         m = ctx.initialize()
@@ -536,8 +634,7 @@ def run_deck(path, output=None, lanes=4):
             from . import quasistatic as Q
             qp = solver["QuasiStatic"]
             nls = int(qp["Number of Load Steps"])
-            ts = [t0 + (i + 1) * (t1 - t0) / nls for i in range(nls)]
-            ts = [t0] + ts
+            ts = [t0] + [t0 + (i + 1) * (t1 - t0) / nls for i in range(nls)]
             prm = dict(maxSolverIterations=int(qp.get("Maximum Solver Iterations", 10)),
                        tolerance=float(qp.get("Relative Tolerance", 1e-6)),
                        disableHeuristics=int(bool(solver.get("Disable Heuristics", False))),
@@ -545,23 +642,20 @@ def run_deck(path, output=None, lanes=4):
             if "Absolute Tolerance" in qp:
                 prm.update(absoluteTolerance=1, tolerance=float(qp["Absolute Tolerance"]))
             record(t0, _qs_state(ctx, u0, Q))
-            u, steps = Q.run(ctx, presc, ts, u0=u0, **prm)
-            st = _qs_state(ctx, u, Q)
-            record(ts[-1], st)
+            # the load steps in runs of `freq`, a frame after each (the
+            # predictor starts afresh at every frame)
+            u, steps = u0, []
+            for a in range(0, nls, freq):
+                b = min(a + freq, nls)
+                u, st = Q.run(ctx, presc, ts[a:b + 1], u0=u, **prm)
+                steps += st
+                record(ts[b], _qs_state(ctx, u, Q))
             info = dict(solver="QuasiStatic", steps=steps)
         else:
             raise DeckError("solver not supported (Verlet or QuasiStatic)")
     finally:
         ctx.close()
-    res = dict(times=np.array(times), frames={k: np.array(v) for k, v in frames.items()},
-               x=x, volume=vol, nlist=nl, info=info)
-    if output:
-        nodal = {k: v for k, v in res["frames"].items()
-                 if k[:-1] in ("Displacement", "Velocity", "Force_Density", "Force",
-                               "Contact_Force_Density")}
-        elem = {k: v for k, v in res["frames"].items() if k not in nodal}
-        EX.write_results(output, x, res["times"], nodal, elem)
-    return res
+    return times, {k: np.array(v) for k, v in frames.items()}, info
 
 
 def _degrees(nl, N):

@@ -329,6 +329,26 @@ def contact_friction(ref):
     return out
 
 
+def linear_lps_bar(ref):
+    """LinearLPSBar (verification): two disjoint hex bars (blocks 1-3 and 4-6,
+    500 points each) of "Linear LPS Partial Volume" and "Elastic Partial
+    Volume" without partial volumes, every point moved by the prescribed
+    displacement u = (0.1 x t, 0, 0); quasi-static, five load steps, six
+    frames, the .comp tolerances."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/LinearLPSBar")
+    f = netcdf_file(os.path.join(d, "LinearLPSBar_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-9), tol_force=np.array(1e-3),
+               tol_weighted_volume=np.array(1e-14), tol_dilatation=np.array(1e-10),
+               tol_volume=np.array(1e-12))
+    out.update(_exodus_all_blocks(f, list(range(len(time)))))
+    f.close()
+    return out
+
+
 def copy_decks(ref, here):
     """The input decks and meshes of the WaveInBar and PrecrackedPlate
     regression tests (data files, copied unchanged)."""
@@ -345,7 +365,9 @@ def copy_decks(ref, here):
                        "BondBreakingInitialVelocity.xml", "BondBreakingInitialVelocity_TimeDependentCS.xml"),
                       ("test/verification/Contact_2x1x1/Contact_2x1x1.xml", None),
                       ("test/verification/Contact_Friction/Contact_Friction.xml", None),
-                      ("test/verification/Contact_Friction/Contact_Friction.g", None)):
+                      ("test/verification/Contact_Friction/Contact_Friction.g", None),
+                      ("test/verification/LinearLPSBar/LinearLPSBar.yaml", None),
+                      ("test/verification/LinearLPSBar/LinearLPSBar.g", None)):
         shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, name or os.path.basename(rel)))
 
 
@@ -363,6 +385,8 @@ def main():
     np.savez_compressed(os.path.join(here, "contact_2x1x1.npz"), **contact_2x1x1(ref))
     np.savez_compressed(os.path.join(here, "contact_friction.npz"), **contact_friction(ref))
     print("wrote contact_2x1x1.npz, contact_friction.npz")
+    np.savez_compressed(os.path.join(here, "linear_lps_bar.npz"), **linear_lps_bar(ref))
+    print("wrote linear_lps_bar.npz")
     copy_decks(ref, here)
     print("copied decks")
     fx = {

@@ -626,3 +626,45 @@ def test_contact_friction_oracle_against_gold(golden_dir):
         peak = max(peak, np.abs(r["contact_force"]).max())
     # the blocks touch during the first two output frames, then separate
     assert peak > 1e9
+
+
+def test_linear_lps_bar_oracle_against_gold(golden_dir):
+    """LinearLPSBar (verification deck): two disjoint bars, blocks 1-3 of
+    "Linear LPS Partial Volume" and blocks 4-6 of "Elastic Partial Volume",
+    neither with partial volumes, every point under u = (0.1 x t, 0, 0). The
+    oracle's linearized LPS (src/materials/Peridigm_LinearLPSPVMaterial.cpp:
+    229-394) on the first bar and its elastic LPS (which the partial-volume
+    kernels of src/materials/elastic_pv.cxx reduce to without partial volumes)
+    on the second reproduce the six gold frames of force, dilatation and
+    weighted volume at the .comp tolerances."""
+    import os
+    from peridigm_amd import exodus as EX
+    g = np.load(os.path.join(golden_dir, "linear_lps_bar.npz"))
+    x, vol, blk, sets = EX.discretize(EX.read_mesh(os.path.join(golden_dir, "decks", "LinearLPSBar.g")))
+    assert np.abs(x - g["coords"]).max() <= 1e-12
+    assert np.abs(vol - g["Volume"][0]).max() <= float(g["tol_volume"])
+    hz = 0.1502
+    K, mu = 160.0e10, 79.3e10
+    nl = O.radius_search(x, hz)
+    N = len(vol)
+    bd = np.zeros(O.num_bonds(nl, N))
+    m = O.weighted_volume(x, vol, nl, hz)
+    assert np.abs(m - g["Weighted_Volume"][0]).max() <= float(g["tol_weighted_volume"])
+    lin = blk <= 2
+    # no bond joins the two bars
+    rp, col = O.list_to_csr(nl, N)
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    assert np.array_equal(lin[rows], lin[col])
+    for k, t in enumerate(g["time"]):
+        u = np.zeros((N, 3))
+        u[:, 0] = 0.1 * x[:, 0] * t
+        assert np.abs(u[:, 0] - g["DisplacementX"][k]).max() <= float(g["tol_displacement"])
+        th_l, f_l = O.compute_force_linear(x, x + u, m, vol, bd, nl, K, mu, hz)
+        th_e, f_e = O.compute_force(x, x + u, m, vol, bd, nl, K, mu, hz)
+        th = np.where(lin, th_l, th_e)
+        F = np.where(lin[:, None], f_l, f_e) * vol[:, None]
+        assert np.abs(th - g["Dilatation"][k]).max() <= float(g["tol_dilatation"]), k
+        for c, ax in enumerate("XYZ"):
+            assert np.abs(F[:, c] - g["Force" + ax][k]).max() <= float(g["tol_force"]), (k, ax)
+    # the two models differ at this strain (the linearization is visible)
+    assert np.abs(th_l[lin] - th_e[lin]).max() > 1e3 * float(g["tol_dilatation"])
