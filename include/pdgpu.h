@@ -335,6 +335,36 @@ int pdg_qs_params_default(pdg_qs_params* params);
 int pdg_qs_run(pdg_ctx* ctx, const pdg_bc* bcs, int numBcs, const double* times, int numTimes,
                const pdg_qs_params* params, double* u, pdg_qs_step* steps, int maxSteps,
                int* numSteps);
+/* The same run, also writing the displacement of every accepted load step
+ * (maxSteps x 3N, reference order) to uSteps, where the reference writes
+ * its output frames (:2984-2990). */
+int pdg_qs_run_record(pdg_ctx* ctx, const pdg_bc* bcs, int numBcs, const double* times,
+                      int numTimes, const pdg_qs_params* params, double* u, pdg_qs_step* steps,
+                      int maxSteps, int* numSteps, double* uSteps);
+
+/* ---- thermal strains and partial stress (elastic LPS) --------------------
+ * ElasticMaterial with "Thermal Expansion Coefficient"
+ * (src/materials/Peridigm_ElasticMaterial.cpp:78-81): every bond's extension
+ * loses alpha dT_i zeta, dT_i the Temperature_Change of the bond's owner
+ * (src/materials/elastic.cxx:112-113, material_utilities.cxx:281-282).
+ * Evaluated in FP64 by pdg_compute_force and the quasi-static calls; the
+ * explicit runs reject it. coefficient 0 turns it off. */
+int pdg_set_thermal_expansion(pdg_ctx* ctx, double coefficient);
+/* Temperature_Change per node as dT(t) = dT1 t + dT0 (N each, reference
+ * order; dT1 may be null; both null clears the field). pdg_qs_run evaluates
+ * it at every load step's time, as a prescribed temperature's change from
+ * its initial value (src/core/Peridigm_BoundaryAndInitialConditionManager.cpp:
+ * 148-159); the other calls at the time set by pdg_set_temperature_time or
+ * the last load step (0 at first). */
+int pdg_set_temperature_change(pdg_ctx* ctx, const double* dT0, const double* dT1);
+int pdg_set_temperature_time(pdg_ctx* ctx, double t);
+/* Partial_Stress (9N, per node row-major: force component x reference bond
+ * component) of ElasticMaterial::computeForce with "Compute Partial Stress"
+ * (src/materials/elastic.cxx:132-142): sum over the node's own bonds of
+ * f_ij (x_j - x_i)^T V_j at configuration y with bond damage (B, may be
+ * null), thermal strains included; FP64. */
+int pdg_compute_partial_stress(pdg_ctx* ctx, const double* y, const double* bondDamage,
+                               double* partialStress);
 /* 1: linear solves replay each CG iteration's two fixed launch sequences as
  * HIP graphs, captured once per context; 0 (default): every kernel is
  * launched directly. Same arithmetic, bitwise. */

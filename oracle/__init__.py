@@ -46,7 +46,7 @@ def lib():
 
 
 def _p(a):
-    return a.ctypes.data_as(C.c_void_p)
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _c(a, dt):
@@ -129,6 +129,38 @@ def compute_force(x, y, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):
                             C.c_double(K), C.c_double(mu), C.c_double(horizon), C.c_int(infl),
                             _p(th), _p(f))
     return th, f
+
+
+def compute_force_thermal(x, y, m, vol, bond_damage, nlist, K, mu, horizon, alpha_t, delta_t,
+                          infl=0, partial_stress=False):
+    """ElasticMaterial::computeForce with thermal strains (delta_t per point)
+    and optionally the partial stress: (theta, force_density, ps (N,3,3) or None)."""
+    x = _c(x, np.float64); y = _c(y, np.float64); m = _c(m, np.float64)
+    vol = _c(vol, np.float64); bd = _c(bond_damage, np.float64); nlist = _c(nlist, np.int32)
+    N = x.shape[0]
+    dt = None if delta_t is None else _c(np.broadcast_to(delta_t, (N,)), np.float64)
+    th = np.zeros(N); f = np.zeros((N, 3))
+    ps = np.zeros((N, 3, 3)) if partial_stress else None
+    lib().orc_compute_force_thermal(C.c_int(N), _p(x), _p(y), _p(m), _p(vol), _p(bd), _p(nlist),
+                                    C.c_double(K), C.c_double(mu), C.c_double(horizon),
+                                    C.c_int(infl), C.c_double(alpha_t), _p(dt), _p(th), _p(f),
+                                    _p(ps))
+    return th, f, ps
+
+
+def force_directional_thermal(x, u, w, m, vol, bond_damage, nlist, K, mu, horizon, alpha_t,
+                              delta_t, infl=0):
+    """(f(u), J(u) w) with thermal strains, by forward-mode dual numbers."""
+    x = _c(x, np.float64); u = _c(u, np.float64); w = _c(w, np.float64); m = _c(m, np.float64)
+    vol = _c(vol, np.float64); bd = _c(bond_damage, np.float64); nlist = _c(nlist, np.int32)
+    N = x.shape[0]
+    dt = None if delta_t is None else _c(np.broadcast_to(delta_t, (N,)), np.float64)
+    f = np.zeros((N, 3)); jw = np.zeros((N, 3))
+    lib().orc_force_directional_thermal(C.c_int(N), _p(x), _p(u), _p(w), _p(m), _p(vol), _p(bd),
+                                        _p(nlist), C.c_double(K), C.c_double(mu),
+                                        C.c_double(horizon), C.c_int(infl), C.c_double(alpha_t),
+                                        _p(dt), _p(f), _p(jw))
+    return f, jw
 
 
 def compute_force_linear(x, y, m, vol, bond_damage, nlist, K, mu, horizon, infl=0):

@@ -65,10 +65,12 @@ inline double val(double a) { return a; }
 inline double val(Dual a) { return a.v; }
 
 // src/materials/material_utilities.cxx:235-288 computeDilatation<ScalarT>
+// With deltaT: the thermal strain of :281-282, e -= alphaT deltaT_p zeta.
 template <typename T>
 void dilatation_t(int numOwned, const double* x, const T* y, const double* m,
                   const double* vol, const double* bondDamage, T* theta,
-                  const int* nlist, double horizon, int infl) {
+                  const int* nlist, double horizon, int infl, const double* deltaT = nullptr,
+                  double alphaT = 0.0) {
   const int* neighPtr = nlist;
   const double* bd = bondDamage;
   for (int p = 0; p < numOwned; p++) {
@@ -92,6 +94,7 @@ void dilatation_t(int numOwned, const double* x, const T* y, const double* m,
       double d = std::sqrt(zetaSquared);
       T e = dsqrt(dY);
       e = e - T(d);
+      if (deltaT) e = e - T(alphaT * deltaT[p] * d);
       double omega = omega_fn(infl, d, horizon);
       th += 3.0 * omega * (1.0 - *bd) * d * e * cellVolume / m[p];
     }
@@ -101,11 +104,14 @@ void dilatation_t(int numOwned, const double* x, const T* y, const double* m,
 
 // src/materials/elastic.cxx:55-146 computeInternalForceLinearElastic<ScalarT>
 // (owner-side gather plus neighbour-side scatter; f must be zeroed first as in
-// src/materials/Peridigm_ElasticMaterial.cpp:149).
+// src/materials/Peridigm_ElasticMaterial.cpp:149). With deltaT the thermal
+// strain (:112-113); with ps the owner-side partial stress, 9 per point,
+// row = force component, column = reference bond component (:132-142).
 template <typename T>
 void force_t(int numOwned, const double* x, const T* y, const double* m,
              const double* vol, const T* theta, const double* bondDamage, T* f,
-             const int* nlist, double K, double MU, double horizon, int infl) {
+             const int* nlist, double K, double MU, double horizon, int infl,
+             const double* deltaT = nullptr, double alphaT = 0.0, T* ps = nullptr) {
   const int* neighPtr = nlist;
   const double* bd = bondDamage;
   for (int p = 0; p < numOwned; p++) {
@@ -128,6 +134,7 @@ void force_t(int numOwned, const double* x, const T* y, const double* m,
       T Y_dz = YP[2] - Y[2];
       T dY = dsqrt(Y_dx * Y_dx + Y_dy * Y_dy + Y_dz * Y_dz);
       T e = dY - T(zeta);
+      if (deltaT) e = e - T(alphaT * deltaT[p] * zeta);
       double omega = omega_fn(infl, zeta, horizon);
       T c1 = omega * theta[p] * (3.0 * K / m[p] - alpha / 3.0);
       T t = (1.0 - *bd) * (c1 * zeta + (1.0 - *bd) * omega * alpha * e);
@@ -140,6 +147,12 @@ void force_t(int numOwned, const double* x, const T* y, const double* m,
       f[3 * j + 0] -= fx * selfCellVolume;
       f[3 * j + 1] -= fy * selfCellVolume;
       f[3 * j + 2] -= fz * selfCellVolume;
+      if (ps) {
+        T* q = ps + 9 * p;
+        q[0] += fx * X_dx * cellVolume; q[1] += fx * X_dy * cellVolume; q[2] += fx * X_dz * cellVolume;
+        q[3] += fy * X_dx * cellVolume; q[4] += fy * X_dy * cellVolume; q[5] += fy * X_dz * cellVolume;
+        q[6] += fz * X_dx * cellVolume; q[7] += fz * X_dy * cellVolume; q[8] += fz * X_dz * cellVolume;
+      }
     }
   }
 }
@@ -660,6 +673,20 @@ void orc_compute_force_pmb(int N, const double* x, const double* y, const double
   force_pmb(N, x, y, vol, bondDamage, f, nlist, K, horizon, planar, height);
 }
 
+// ElasticMaterial::computeForce with thermal strains (deltaT per point, may be
+// null) and the partial stress (ps, 9 per point, may be null).
+void orc_compute_force_thermal(int N, const double* x, const double* y, const double* m,
+                               const double* vol, const double* bondDamage, const int* nlist,
+                               double K, double mu, double horizon, int infl, double alphaT,
+                               const double* deltaT, double* theta, double* f, double* ps) {
+  for (int i = 0; i < 3 * N; i++) f[i] = 0.0;
+  if (ps)
+    for (int i = 0; i < 9 * N; i++) ps[i] = 0.0;
+  dilatation_t<double>(N, x, y, m, vol, bondDamage, theta, nlist, horizon, infl, deltaT, alphaT);
+  force_t<double>(N, x, y, m, vol, theta, bondDamage, f, nlist, K, mu, horizon, infl, deltaT,
+                  alphaT, ps);
+}
+
 // Linearized LPS without partial volumes ("Linear LPS Partial Volume",
 // src/materials/Peridigm_LinearLPSPVMaterial.cpp:229-289 precompute and
 // :296-394 computeForce, the influence function value of each bond taken
@@ -934,15 +961,26 @@ void orc_force_at(int N, const double* x, const double* u, const double* m, cons
 // Directional derivative Jw = d f(u + eps w)/d eps at eps = 0, by forward-mode
 // dual numbers through the same dilatation and scatter force loops (the
 // counterpart of the Sacado tangent, applied to a vector).
+void orc_force_directional_thermal(int N, const double* x, const double* u, const double* w,
+                                   const double* m, const double* vol, const double* bondDamage,
+                                   const int* nlist, double K, double mu, double horizon,
+                                   int infl, double alphaT, const double* deltaT, double* f,
+                                   double* Jw) {
+  std::vector<Dual> y(3 * N), th(N), ff(3 * N);
+  for (int i = 0; i < 3 * N; i++) y[i] = Dual(x[i] + u[i], w[i]);
+  dilatation_t<Dual>(N, x, y.data(), m, vol, bondDamage, th.data(), nlist, horizon, infl, deltaT,
+                     alphaT);
+  force_t<Dual>(N, x, y.data(), m, vol, th.data(), bondDamage, ff.data(), nlist, K, mu, horizon,
+                infl, deltaT, alphaT);
+  for (int i = 0; i < 3 * N; i++) { if (f) f[i] = ff[i].v; Jw[i] = ff[i].d; }
+}
+
 void orc_force_directional(int N, const double* x, const double* u, const double* w,
                            const double* m, const double* vol, const double* bondDamage,
                            const int* nlist, double K, double mu, double horizon, int infl,
                            double* f, double* Jw) {
-  std::vector<Dual> y(3 * N), th(N), ff(3 * N);
-  for (int i = 0; i < 3 * N; i++) y[i] = Dual(x[i] + u[i], w[i]);
-  dilatation_t<Dual>(N, x, y.data(), m, vol, bondDamage, th.data(), nlist, horizon, infl);
-  force_t<Dual>(N, x, y.data(), m, vol, th.data(), bondDamage, ff.data(), nlist, K, mu, horizon, infl);
-  for (int i = 0; i < 3 * N; i++) { if (f) f[i] = ff[i].v; Jw[i] = ff[i].d; }
+  orc_force_directional_thermal(N, x, u, w, m, vol, bondDamage, nlist, K, mu, horizon, infl,
+                                0.0, nullptr, f, Jw);
 }
 
 // ---------------------------------------------------------------------------

@@ -15,21 +15,24 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import force_at, force_directional, weighted_volume
+from . import force_directional_thermal, weighted_volume
 
 
 class Problem:
-    def __init__(self, x, vol, nlist, K, mu, horizon):
+    def __init__(self, x, vol, nlist, K, mu, horizon, alpha_t=0.0):
         self.x, self.vol, self.nl = x, vol, nlist
         self.K, self.mu, self.hz = K, mu, horizon
         self.N = len(vol)
         self.m = weighted_volume(x, vol, nlist, horizon)
         self.bd = np.zeros(nlist.size - self.N)
+        self.alpha_t = alpha_t
+        self.dT = None  # temperature change per point (thermal strains), or None
 
     def force_v(self, u):
         """f(u) V, reference order, flattened (3N)."""
-        f = force_at(self.x, u.reshape(-1, 3), self.m, self.vol, self.bd, self.nl, self.K,
-                     self.mu, self.hz)
+        u3 = u.reshape(-1, 3)
+        f, _ = force_directional_thermal(self.x, u3, np.zeros_like(u3), self.m, self.vol, self.bd,
+                                         self.nl, self.K, self.mu, self.hz, self.alpha_t, self.dT)
         return (f * self.vol[:, None]).ravel()
 
     def tangent(self, u, fixed):
@@ -41,8 +44,9 @@ class Problem:
         for q in range(n3):
             w = np.zeros(n3)
             w[q] = 1.0
-            _, jw = force_directional(self.x, u3, w.reshape(-1, 3), self.m, self.vol, self.bd,
-                                      self.nl, self.K, self.mu, self.hz)
+            _, jw = force_directional_thermal(self.x, u3, w.reshape(-1, 3), self.m, self.vol,
+                                              self.bd, self.nl, self.K, self.mu, self.hz,
+                                              self.alpha_t, self.dT)
             Kd[:, q] = -vv * jw.ravel()
         Kd[fixed, :] = 0.0
         Kd[fixed, fixed] = 1.0
@@ -99,10 +103,14 @@ def qs_run(x, vol, nlist, K, mu, horizon, bcs, times, u0=None, maxSolverIteratio
            tolerance=1e-6, absoluteTolerance=False, disableHeuristics=False,
            dampedNewtonDiagonalScaleFactor=1.0001, dampedNewtonDiagonalShiftFactor=1e-5,
            adaptiveLoadStepping=False, maxLoadStepReductionsInOneStep=0,
-           maxTotalLoadStepReductions=0, reduceAllRemainingLoadSteps=False):
+           maxTotalLoadStepReductions=0, reduceAllRemainingLoadSteps=False, thermal=None):
     """Returns (u, steps, aborted): u (N,3) of the last accepted load step and
-    one dict per accepted step (t0, t1, newton, multiplier, status, alpha)."""
-    P = Problem(x, vol, nlist, K, mu, horizon)
+    one dict per accepted step (t0, t1, newton, multiplier, status, alpha).
+    thermal = (alpha_t, dT(t)): thermal strains with the temperature change
+    per point at each load step's time (a prescribed temperature,
+    DirichletIncrementBC relative to the initial value,
+    src/core/Peridigm_BoundaryAndInitialConditionManager.cpp:148-159)."""
+    P = Problem(x, vol, nlist, K, mu, horizon, thermal[0] if thermal else 0.0)
     n3 = 3 * P.N
     kin = {}
     for ids, coord, c0, c1 in bcs:
@@ -127,6 +135,8 @@ def qs_run(x, vol, nlist, K, mu, horizon, bcs, times, u0=None, maxSolverIteratio
         tc = ts[step]
         dt = tc - tp
         du = np.zeros(n3)
+        if thermal is not None and tc != tp:
+            P.dT = np.asarray(thermal[1](tc), float)
         if tc != tp:
             for d in kdof:
                 c0, c1 = kin[d]

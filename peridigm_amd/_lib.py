@@ -85,6 +85,12 @@ SIGNATURES = {
     "pdg_qs_force_dilatation": (_i, [_vp, _vp, _vp, _vp]),
     "pdg_qs_run": (_i, [_vp, C.POINTER(pdg_bc), _i, _vp, _i, C.POINTER(pdg_qs_params), _vp,
                         C.POINTER(pdg_qs_step), _i, C.POINTER(_i)]),
+    "pdg_qs_run_record": (_i, [_vp, C.POINTER(pdg_bc), _i, _vp, _i, C.POINTER(pdg_qs_params), _vp,
+                               C.POINTER(pdg_qs_step), _i, C.POINTER(_i), _vp]),
+    "pdg_set_thermal_expansion": (_i, [_vp, _d]),
+    "pdg_set_temperature_change": (_i, [_vp, _vp, _vp]),
+    "pdg_set_temperature_time": (_i, [_vp, _d]),
+    "pdg_compute_partial_stress": (_i, [_vp, _vp, _vp, _vp]),
     "pdg_verlet_run": (_i, [_vp, _i, _d, _d, _i]),
     "pdg_verlet_run_persistent": (_i, [_vp, _i, _d, _d, _i, C.POINTER(_i)]),
     "pdg_critical_time_step": (_i, [_vp, _d, C.POINTER(_d)]),

@@ -157,6 +157,30 @@ class Context:
                                                _p(th), _p(f)))
         return dict(u=u, v=v, a=a, bond_damage=bd, damage=dmg, theta=th, force=f)
 
+    # ------------------------------------------------------------- thermal
+    def set_thermal_expansion(self, coefficient):
+        """"Thermal Expansion Coefficient" of the elastic model (0: off)."""
+        self.check(self.lib.pdg_set_thermal_expansion(self.h, float(coefficient)))
+
+    def set_temperature_change(self, dT0=None, dT1=None):
+        """Temperature_Change per node, dT(t) = dT1 t + dT0 (None: zero)."""
+        a = None if dT0 is None else _f64(np.broadcast_to(dT0, (self.N,)), (self.N,))
+        b = None if dT1 is None else _f64(np.broadcast_to(dT1, (self.N,)), (self.N,))
+        if a is None and b is not None:
+            a = np.zeros(self.N)
+        self.check(self.lib.pdg_set_temperature_change(self.h, _p(a), _p(b)))
+
+    def set_temperature_time(self, t):
+        self.check(self.lib.pdg_set_temperature_time(self.h, float(t)))
+
+    def compute_partial_stress(self, y, bond_damage=None):
+        """Partial_Stress (N, 3, 3) at configuration y (FP64)."""
+        y = _f64(y, (self.N, 3))
+        bd = None if bond_damage is None else _f64(bond_damage, (self.B,))
+        ps = np.zeros((self.N, 3, 3))
+        self.check(self.lib.pdg_compute_partial_stress(self.h, _p(y), _p(bd), _p(ps)))
+        return ps
+
     # -------------------------------------------------------------- contact
     def set_contact(self, search_radius, search_frequency, contact_radius, spring_constant,
                     friction=0.0):

@@ -983,8 +983,9 @@ int pdg_compute_force(pdg_ctx* h, double dt, const double* y, const double* bdNP
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || (c->N > 0 && (!y || !force))) return PDG_E_ARG;
   (void)hipSetDevice(c->device);
-  if (c->material == PDG_MATERIAL_LINEAR_LPS) {
-    // the linearized model through the FP64 quasi-static kernels at u = y - x
+  if (c->material == PDG_MATERIAL_LINEAR_LPS || c->thermal()) {
+    // the linearized model and thermal strains through the FP64
+    // quasi-static kernels at u = y - x
     int r = c->upload_mask(bdNP1, true);
     if (r) return r;
     std::vector<double> u(3 * (size_t)c->N);
@@ -1099,6 +1100,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   if (!c || !c->have_consts || nsteps < 0 || !(dt > 0.0)) return PDG_E_ARG;
   if (c->material == PDG_MATERIAL_LINEAR_LPS)
     return c->fail(PDG_E_UNSUPPORTED, "explicit runs implement the elastic and bond-based models");
+  if (c->thermal()) return c->fail(PDG_E_UNSUPPORTED, "explicit runs with thermal strains");
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
   const bool damage = c->sc > 0.0;
@@ -1190,6 +1192,7 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   if (!c->sc_t.empty())
     return c->fail(PDG_E_UNSUPPORTED, "persistent run with a time-dependent critical stretch");
   if (contact_on(c)) return c->fail(PDG_E_UNSUPPORTED, "persistent run with contact");
+  if (c->thermal()) return c->fail(PDG_E_UNSUPPORTED, "explicit runs with thermal strains");
   const int tl = c->tile_lanes();
   if (tl != 4 && tl != 8 && tl != 16)
     return c->fail(PDG_E_UNSUPPORTED, "persistent run uses 4, 8 or 16 lanes per node");

@@ -112,6 +112,15 @@ struct Ctx {
   void order_bonds();
   SchedStats sched;  // LDS service groups and their conflict cost, before and after order_bonds
 
+  // thermal strains ("Thermal Expansion Coefficient" and the Temperature_Change
+  // field dT(t) = dT1 t + dT0, reference order; the quasi-static kernels
+  // evaluate it at temp_time)
+  double alpha_t = 0.0;
+  std::vector<double> dT0, dT1;
+  double temp_time = 0.0;
+  unsigned thermal_epoch = 0;
+  bool thermal() const { return alpha_t != 0.0 && !dT0.empty(); }
+
   QsState* qs = nullptr;
   GnnState* gnn = nullptr;
   ContactState* contact = nullptr;  // null: no contact model

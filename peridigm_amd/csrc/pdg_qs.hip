@@ -58,6 +58,12 @@ struct QsState {
   // diagonal shift added to every tangent apply while `damped` is set
   double *diag = nullptr, *dampv = nullptr;
   bool damped = false;
+  // temperature change per node (internal order) at dT_time, and a 9n
+  // partial-stress buffer
+  double* dT = nullptr;
+  double dT_time = 0.0;
+  unsigned dT_epoch = ~0u;
+  double* ps = nullptr;
   // load-stepping driver vectors (3n each, allocated by its first run): the
   // displacement at the load step's start, Newton increment, predictor,
   // line-search base state and residual
@@ -114,6 +120,9 @@ struct QsArgs {
   double horizon;
   int influence;
   int linear;  // linearized LPS (force along the reference bond, linear in u)
+  const double* dT;  // temperature change per node (thermal strains), or null
+  double alphaT;     // thermal expansion coefficient
+  double* ps;        // partial stress, 9 per node (force<3>)
 };
 
 // dilatation (+ its directional derivative, + d theta_i / d u_i)
@@ -152,7 +161,8 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
       const double Y2 = X2 + (a.u[3 * j + 2] - ui[2]);
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
       const double iy = 1.0 / yl;
-      const double e = yl - zeta;
+      double e = yl - zeta;
+      if (a.dT) e -= a.alphaT * a.dT[node] * zeta;  // elastic.cxx thermal strain
       const double c = infl64(a.influence, zeta, a.horizon) * zeta * a.vol[j];
       th += c * e;
       const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
@@ -177,7 +187,9 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
 }
 
 // MODE 0: residual r = f V (0 on fixed DOFs); MODE 1: K w = -V df (w on fixed);
-// MODE 2: f V on every DOF (the reactions are its kinematic entries)
+// MODE 2: f V on every DOF (the reactions are its kinematic entries);
+// MODE 3: the partial stress sum_j f_ij (x_j - x_i)^T V_j of the node's own
+// bonds (src/materials/elastic.cxx:132-142), 9 per node into a.ps
 template <int MODE>
 __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
   const int lane = threadIdx.x & 63;
@@ -185,6 +197,7 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
   const int node = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / QG) + gi;
   const bool valid = node < a.n;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (valid) {
     const int rs = a.row[node], deg = a.row[node + 1] - rs, w0 = a.mw[node];
     const double* xi = a.x + 3 * node;
@@ -224,10 +237,19 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
       const double iy = 1.0 / yl;
       const double e = yl - zeta;
+      // thermal strain of each end's bond: e_ij = e - alphaT dT_i zeta
+      const double ei = a.dT ? e - a.alphaT * a.dT[node] * zeta : e;
+      const double ej = a.dT ? e - a.alphaT * a.dT[j] * zeta : e;
       const double om = infl64(a.influence, zeta, a.horizon);
-      const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
+      const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + ai * ei + a.alpha[j] * ej);
       const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
-      if (MODE != 1) {
+      if (MODE == 3) {
+        // the node's own half t_ij only
+        const double ti = om * (thi * bi * zeta + ai * ei) * vj;
+        const double nn[3] = {n0, n1, n2}, xx[3] = {X0, X1, X2};
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++) P[3 * r + c] += ti * nn[r] * xx[c];
+      } else if (MODE != 1) {
         f0 += T * n0 * vj; f1 += T * n1 * vj; f2 += T * n2 * vj;
       } else {
         const double* wi = a.w + 3 * node;
@@ -240,6 +262,13 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
         f2 += (dT * n2 + s * (d2 - n2 * de)) * vj;
       }
     }
+  }
+  if (MODE == 3) {
+#pragma unroll
+    for (int t = 0; t < 9; t++) P[t] = gsum<QG>(P[t]);
+    if (valid && l == 0)
+      for (int t = 0; t < 9; t++) a.ps[9 * node + t] = P[t];
+    return;
   }
   f0 = gsum<QG>(f0); f1 = gsum<QG>(f1); f2 = gsum<QG>(f2);
   if (!valid || l >= 3) return;
@@ -291,8 +320,10 @@ __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
       const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
       const double e = yl - zeta;
+      const double ei = a.dT ? e - a.alphaT * a.dT[node] * zeta : e;
+      const double ej = a.dT ? e - a.alphaT * a.dT[j] * zeta : e;
       const double om = infl64(a.influence, zeta, a.horizon);
-      const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + (ai + a.alpha[j]) * e);
+      const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + ai * ei + a.alpha[j] * ej);
       const double n[3] = {Y0 / yl, Y1 / yl, Y2 / yl};
       // grad_{u_i}(t_ij + t_ji) = om [b_i zeta g_i - a_i n - b_j zeta^2 om V_i (3/m_j) n - a_j n]
       const double sn = -ai - a.alpha[j] - a.beta[j] * zeta * zeta * om * vi * cj;
@@ -586,14 +617,40 @@ static QsArgs args(Ctx* c) {
   a.out = S->out; a.dinv = S->dinv; a.diag = S->diag; a.horizon = c->horizon;
   a.influence = c->infl;
   a.linear = c->material == PDG_MATERIAL_LINEAR_LPS ? 1 : 0;
+  a.dT = c->thermal() ? S->dT : nullptr;
+  a.alphaT = c->alpha_t;
+  a.ps = S->ps;
   return a;
+}
+
+// The temperature change on the device at the context's temperature time.
+static int thermal_sync(Ctx* c) {
+  QsState* S = c->qs;
+  if (!c->thermal()) return PDG_OK;
+  const int N = c->N;
+  if (!S->dT) {
+    int rc = c->hip(hipMalloc((void**)&S->dT, std::max<size_t>(8 * (size_t)N, 64)), "qs alloc");
+    if (rc) return rc;
+    S->dT_epoch = ~0u;
+  }
+  if (S->dT_epoch == c->thermal_epoch && S->dT_time == c->temp_time) return PDG_OK;
+  std::vector<double> h(N);
+  const double t = c->temp_time;
+  for (int k = 0; k < N; k++) {
+    const int p = c->perm[k];
+    h[k] = (c->dT1.empty() ? 0.0 : c->dT1[p]) * t + c->dT0[p];
+  }
+  int rc = c->hip(hipMemcpyAsync(S->dT, h.data(), 8 * (size_t)N, hipMemcpyHostToDevice, c->stream), "h2d dT");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  if (!rc) { S->dT_epoch = c->thermal_epoch; S->dT_time = t; }
+  return rc;
 }
 
 static int ensure(Ctx* c) {
   if (c->material != PDG_MATERIAL_ELASTIC && c->material != PDG_MATERIAL_LINEAR_LPS)
     return c->fail(PDG_E_UNSUPPORTED,
                    "the quasi-static path implements the elastic and linearized LPS models");
-  if (c->qs && c->qs->n == c->N) return PDG_OK;
+  if (c->qs && c->qs->n == c->N) return thermal_sync(c);
   if (c->qs) c->free_extra();
   QsState* S = new QsState();
   c->qs = S;
@@ -640,6 +697,7 @@ static int ensure(Ctx* c) {
   S->hfixed.assign(3 * (size_t)N, 0);
   if (!rc) rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), 3 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
   if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (!rc) rc = thermal_sync(c);
   return rc;
 }
 
@@ -773,7 +831,7 @@ void Ctx::free_extra() {
     double* ptrs[] = {S->x,   S->u,  S->w,    S->vol,  S->m,      S->alpha, S->beta,
                       S->theta, S->dtheta, S->g, S->out, S->dinv, S->xs,    S->r,
                       S->z,   S->p,  S->q,    S->b,    S->part,   S->scal,  S->diag,
-                      S->dampv};
+                      S->dampv, S->dT,   S->ps};
     for (double* p : ptrs) if (p) (void)hipFree(p);
     for (double* p : S->rv) if (p) (void)hipFree(p);
     for (int k = 0; k < 4; k++) {
@@ -1178,6 +1236,73 @@ int pdg_qs_force_dilatation(pdg_ctx* h, const double* u, double* force, double* 
   return PDG_OK;
 }
 
+int pdg_set_thermal_expansion(pdg_ctx* h, double coefficient) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !std::isfinite(coefficient)) return PDG_E_ARG;
+  if (coefficient != 0.0 && c->material != PDG_MATERIAL_ELASTIC)
+    return c->fail(PDG_E_UNSUPPORTED, "thermal strains belong to the elastic (LPS) model");
+  c->alpha_t = coefficient;
+  c->thermal_epoch++;
+  if (c->qs) qs::drop_graphs(c);
+  return PDG_OK;
+}
+
+int pdg_set_temperature_change(pdg_ctx* h, const double* dT0, const double* dT1) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts) return PDG_E_ARG;
+  const int N = c->N;
+  if (!dT0 && !dT1) {
+    c->dT0.clear();
+    c->dT1.clear();
+  } else {
+    c->dT0.assign(N, 0.0);
+    if (dT0) c->dT0.assign(dT0, dT0 + N);
+    if (dT1) c->dT1.assign(dT1, dT1 + N);
+    else c->dT1.clear();
+  }
+  c->thermal_epoch++;
+  if (c->qs) qs::drop_graphs(c);
+  return PDG_OK;
+}
+
+int pdg_set_temperature_time(pdg_ctx* h, double t) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !std::isfinite(t)) return PDG_E_ARG;
+  c->temp_time = t;
+  return PDG_OK;
+}
+
+int pdg_compute_partial_stress(pdg_ctx* h, const double* y, const double* bondDamage,
+                               double* partialStress) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !c->have_consts || (c->N > 0 && (!y || !partialStress))) return PDG_E_ARG;
+  if (c->material != PDG_MATERIAL_ELASTIC)
+    return c->fail(PDG_E_UNSUPPORTED, "the partial stress belongs to the elastic (LPS) model");
+  (void)hipSetDevice(c->device);
+  int rc = qs::ensure(c);
+  if (rc) return rc;
+  QsState* S = c->qs;
+  const int N = c->N;
+  if (!S->ps && (rc = c->hip(hipMalloc((void**)&S->ps, std::max<size_t>(72 * (size_t)N, 64)), "qs alloc")))
+    return rc;
+  if ((rc = c->upload_mask(bondDamage, true))) return rc;
+  std::vector<double> u(3 * (size_t)N);
+  for (size_t q = 0; q < u.size(); q++) u[q] = y[q] - c->x[q];
+  if ((rc = qs::to_dev(c, u.data(), S->u))) return rc;
+  QsArgs a = qs::args(c);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(N)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_qs_force<3>), dim3(qblocks(N)), dim3(256), 0, s, a);
+  rc = c->hip(hipGetLastError(), "partial stress");
+  std::vector<double> hp(9 * (size_t)N);
+  if (!rc) rc = c->hip(hipMemcpyAsync(hp.data(), S->ps, 72 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h ps");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (rc) return rc;
+  for (int k = 0; k < N; k++)
+    for (int t = 0; t < 9; t++) partialStress[9 * (size_t)c->perm[k] + t] = hp[9 * (size_t)k + t];
+  return PDG_OK;
+}
+
 int pdg_qs_set_absolute_tolerance(pdg_ctx* h, int on) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return PDG_E_ARG;
@@ -1327,6 +1452,13 @@ int pdg_qs_params_default(pdg_qs_params* p) {
 int pdg_qs_run(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* times, int numTimes,
                const pdg_qs_params* prm, double* u, pdg_qs_step* steps, int maxSteps,
                int* numSteps) {
+  return pdg_qs_run_record(h, bcs, numBcs, times, numTimes, prm, u, steps, maxSteps, numSteps,
+                           nullptr);
+}
+
+int pdg_qs_run_record(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* times,
+                      int numTimes, const pdg_qs_params* prm, double* u, pdg_qs_step* steps,
+                      int maxSteps, int* numSteps, double* uSteps) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || !u || !prm || !times || numTimes < 1 || numBcs < 0 ||
       (numBcs > 0 && !bcs) || prm->precond < 0 || prm->precond > 2)
@@ -1390,6 +1522,12 @@ int pdg_qs_run(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* times, i
     failed = false;
     tc = ts[step];
     const double dt = tc - tp;
+    // the temperature change at the load step's time (prescribed
+    // temperatures are applied with the displacements)
+    if (c->thermal()) {
+      c->temp_time = tc;
+      if ((rc = qs::thermal_sync(c))) return rc;
+    }
     // deltaU: the prescribed increments (DirichletIncrementBC with coefficient
     // 1, src/core/Peridigm_BoundaryCondition.cpp:202-241; a zero increment
     // takes the field's own value as the previous one, :136-143)
@@ -1505,6 +1643,10 @@ int pdg_qs_run(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* times, i
       rc = c->hip(hipMemcpyAsync(hub.data(), R.ub, 8 * (size_t)n3, hipMemcpyDeviceToHost, s), "d2h");
       if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
       if (rc) return rc;
+      if (uSteps && nrec < maxSteps)
+        for (int k = 0; k < N; k++)
+          for (int q = 0; q < 3; q++)
+            uSteps[(size_t)nrec * n3 + 3 * (size_t)c->perm[k] + q] = hub[3 * (size_t)k + q];
       if (steps && nrec < maxSteps) {
         pdg_qs_step& st = steps[nrec];
         st.timePrevious = tp;

@@ -379,24 +379,49 @@ def contact_params(deck):
 _COORD = {"x": 0, "y": 1, "z": 2}
 
 
+def _node_sets(deck, sets):
+    """The mesh's point sets plus those listed in the deck itself (any plain
+    parameter whose name holds "Node Set",
+    src/core/Peridigm_BoundaryAndInitialConditionManager.cpp:80-86: 1-based
+    ids, space separated)."""
+    sets = dict(sets)
+    for name, val in deck.get("Boundary Conditions", {}).items():
+        if not isinstance(val, dict) and "NODE SET" in name.upper():
+            sets[name] = np.array([int(tk) - 1 for tk in str(val).split()], np.int64)
+    return sets
+
+
+def _temperature(deck, x, sets):
+    """Prescribed temperatures as T(t) = c1 t + c0 per point, (c0, c1), or
+    None without any: "Prescribed Temperature" sets the temperature and its
+    change from the initial value, T(t) - T(0), which drives the thermal
+    strains (src/core/Peridigm_BoundaryAndInitialConditionManager.cpp:148-159)."""
+    sets = _node_sets(deck, sets)
+    N = x.shape[0]
+    c0, c1, any_ = np.zeros(N), np.zeros(N), False
+    for name, bc in deck.get("Boundary Conditions", {}).items():
+        if isinstance(bc, dict) and bc.get("Type") == "Prescribed Temperature":
+            ids = sets[bc["Node Set"]]
+            a0, a1 = linear_in_t(str(bc["Value"]), x[ids, 0], x[ids, 1], x[ids, 2])
+            c0[ids], c1[ids] = a0, a1
+            any_ = True
+    return (c0, c1) if any_ else None
+
+
 def _bcs(deck, x, sets):
     """(v0, u0, prescribed [(ids, coord, c0, c1)]) from the deck's conditions."""
     N = x.shape[0]
     v0 = np.zeros((N, 3))
     u0 = np.zeros((N, 3))
     presc = []
-    sets = dict(sets)
+    sets = _node_sets(deck, sets)
     bcl = deck.get("Boundary Conditions", {})
-    # node sets listed in the deck itself (any plain parameter whose name
-    # holds "Node Set", src/core/Peridigm_BoundaryAndInitialConditionManager.cpp:80-86):
-    # 1-based ids, space separated
-    for name, val in bcl.items():
-        if not isinstance(val, dict) and "NODE SET" in name.upper():
-            sets[name] = np.array([int(tk) - 1 for tk in str(val).split()], np.int64)
     for name, bc in bcl.items():
         if not isinstance(bc, dict):
             continue
         typ = bc.get("Type")
+        if typ == "Prescribed Temperature":
+            continue  # _temperature
         ns = bc.get("Node Set")
         if ns not in sets:
             raise DeckError("node set %r of %r not found" % (ns, name))
@@ -455,7 +480,7 @@ def _material(deck, grp):
             raise DeckError("damage model %r not supported" % dm.get("Damage Model"))
     return dict(model=model, K=K, mu=mu, planar=planar, height=height,
                 rho=float(mat["Density"]), infl=INFLUENCE[grp.get("Influence Function", "One")],
-                sc=sc, sc_fn=sc_fn)
+                sc=sc, sc_fn=sc_fn, alpha_t=float(mat.get("Thermal Expansion Coefficient", 0.0)))
 
 
 def _subset(idx, N, nl, v0, u0, presc):
@@ -516,6 +541,7 @@ def run_deck(path, output=None, lanes=4):
         run_of.setdefault(s_, len(run_of))
     point_run = np.array([run_of[sig[g]] for g in point_group], np.int64)
     v0, u0, presc = _bcs(deck, x, sets)
+    temp = _temperature(deck, x, sets)
     contact = contact_params(deck)
     if contact is not None and len(run_of) > 1:
         raise DeckError("contact between block groups of different materials is not supported")
@@ -535,7 +561,8 @@ def run_deck(path, output=None, lanes=4):
         else:
             sub = _subset(idx, N, nl, v0, u0, presc)
         t_r, f_r, info = _run_points(deck, _material(deck, grp), horizon, x[idx], vol[idx], sub,
-                                     contact, lanes)
+                                     contact, lanes,
+                                     None if temp is None else (temp[0][idx], temp[1][idx]))
         if times is None:
             times = t_r
         elif len(t_r) != len(times) or np.abs(np.asarray(t_r) - times).max() > 0:
@@ -557,9 +584,10 @@ def run_deck(path, output=None, lanes=4):
     return res
 
 
-def _run_points(deck, mat, horizon, x, vol, sub, contact, lanes):
+def _run_points(deck, mat, horizon, x, vol, sub, contact, lanes, temp=None):
     """Runs the deck's solver on one set of points of one material: returns
-    (times, frames {variable: (frames, n)}, info)."""
+    (times, frames {variable: (frames, n)}, info). temp: prescribed
+    temperatures T(t) = c1 t + c0 as (c0, c1), or None."""
     nl, v0, u0, presc = sub
     N = x.shape[0]
     solver = deck["Solver"]
@@ -579,6 +607,11 @@ def _run_points(deck, mat, horizon, x, vol, sub, contact, lanes):
         ctx.set_constants(x, vol, mat["K"], mat["mu"], horizon, mat["infl"], sc)
         if contact is not None:
             ctx.set_contact(**contact)
+        if mat["alpha_t"] != 0.0:
+            ctx.set_thermal_expansion(mat["alpha_t"])
+        if temp is not None:
+            # the change from the initial value, T(t) - T(0) = c1 t
+            ctx.set_temperature_change(np.zeros(N), temp[1])
         m = ctx.initialize()
         deg = _degrees(nl, N)
         frames = {}
@@ -597,10 +630,15 @@ def _run_points(deck, mat, horizon, x, vol, sub, contact, lanes):
                 if name in want:
                     for c, ax in enumerate("XYZ"):
                         frames.setdefault(name + ax, []).append(arr[:, c].copy())
+            if temp is not None:
+                st = dict(st, temperature=temp[1] * t + temp[0], temperature_change=temp[1] * t)
             scal = {"Dilatation": st["dilatation"], "Weighted_Volume": m,
                     "Damage": st["damage"], "Number_Of_Neighbors": deg.astype(float),
                     "Volume": vol, "Element_Id": np.arange(1, N + 1, dtype=float),
                     "Proc_Num": np.zeros(N)}
+            if temp is not None:
+                scal["Temperature"] = st["temperature"]
+                scal["Temperature_Change"] = st["temperature_change"]
             for name, arr in scal.items():
                 if name in want:
                     frames.setdefault(name, []).append(np.asarray(arr, float).copy())
@@ -641,16 +679,18 @@ def _run_points(deck, mat, horizon, x, vol, sub, contact, lanes):
                        linearTolerance=float(qp.get("Belos Relative Tolerance", 1e-4)))
             if "Absolute Tolerance" in qp:
                 prm.update(absoluteTolerance=1, tolerance=float(qp["Absolute Tolerance"]))
-            record(t0, _qs_state(ctx, u0, Q))
-            # the load steps in runs of `freq`, a frame after each (the
-            # predictor starts afresh at every frame)
-            u, steps = u0, []
-            for a in range(0, nls, freq):
-                b = min(a + freq, nls)
-                u, st = Q.run(ctx, presc, ts[a:b + 1], u0=u, **prm)
-                steps += st
-                record(ts[b], _qs_state(ctx, u, Q))
-            info = dict(solver="QuasiStatic", steps=steps)
+            record(t0, _qs_state(ctx, u0, Q, t0, np.zeros_like(u0)))
+            # one run; a frame every `freq` accepted load steps, with the
+            # velocity of the step's increment (src/core/Peridigm.cpp:2683-2687)
+            u, steps = Q.run(ctx, presc, ts, u0=u0, record=True, **prm)
+            u_prev = u0
+            for k, st in enumerate(steps):
+                if (k + 1) % freq == 0 or k == len(steps) - 1:
+                    v = (st["u"] - u_prev) / (st["t1"] - st["t0"])
+                    record(st["t1"], _qs_state(ctx, st["u"], Q, st["t1"], v))
+                u_prev = st["u"]
+            info = dict(solver="QuasiStatic", steps=[{k: v for k, v in st.items() if k != "u"}
+                                                      for st in steps])
         else:
             raise DeckError("solver not supported (Verlet or QuasiStatic)")
     finally:
@@ -676,10 +716,12 @@ def _explicit_state(ctx, contact=False):
     return out
 
 
-def _qs_state(ctx, u, Q):
-    """FP64 state of a quasi-static frame: displacement, force density and
-    dilatation at u (the quasi-static kernels)."""
+def _qs_state(ctx, u, Q, t, v):
+    """FP64 state of a quasi-static frame at time t: displacement, velocity,
+    force density and dilatation at u (the quasi-static kernels, the
+    temperature change at t)."""
+    ctx.set_temperature_time(t)
     f, th = Q.force_and_dilatation(ctx, u)
     N = ctx.N
-    return dict(u=np.asarray(u, float).reshape(N, 3), v=np.zeros((N, 3)), force=f,
-                dilatation=th, damage=np.zeros(N))
+    return dict(u=np.asarray(u, float).reshape(N, 3), v=np.asarray(v, float).reshape(N, 3),
+                force=f, dilatation=th, damage=np.zeros(N))

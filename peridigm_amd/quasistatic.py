@@ -143,11 +143,12 @@ def default_params(**overrides):
     return prm
 
 
-def run(ctx: Context, bcs, times, u0=None, **params):
+def run(ctx: Context, bcs, times, u0=None, record=False, **params):
     """Peridigm::executeQuasiStatic on the device (pdg_qs_run): load steps at
     `times`, prescribed displacements bcs = [(node_ids, coord, c0, c1)] with
     u(t) = c1 t + c0, parameters as default_params. Returns (u, steps) with
-    one dict per accepted load step; raises PdgError if the run aborts."""
+    one dict per accepted load step (with `record`, each also holds that
+    step's displacement "u"); raises PdgError if the run aborts."""
     N = ctx.N
     u = np.zeros((N, 3)) if u0 is None else _f64(u0, (N, 3)).copy()
     t = np.ascontiguousarray(times, dtype=np.float64)
@@ -156,12 +157,16 @@ def run(ctx: Context, bcs, times, u0=None, **params):
     steps = (pdg_qs_step * cap)()
     ns = C.c_int()
     arr = ctx.bc_array(bcs)
-    rc = ctx.lib.pdg_qs_run(ctx.h, arr, len(bcs), _p(t), t.size, C.byref(prm), _p(u), steps, cap,
-                            C.byref(ns))
+    us = np.zeros((cap, N, 3)) if record else None
+    rc = ctx.lib.pdg_qs_run_record(ctx.h, arr, len(bcs), _p(t), t.size, C.byref(prm), _p(u),
+                                   steps, cap, C.byref(ns), _p(us))
     out = [dict(t0=st.timePrevious, t1=st.timeCurrent, newton=st.newtonIterations,
                 linear=st.linearIterations, residual=st.residualNorm,
                 multiplier=st.toleranceMultiplier, status=st.status,
                 damped=bool(st.dampedNewton), alpha=st.lastAlpha)
            for st in steps[:min(ns.value, cap)]]
+    if record:
+        for k, d in enumerate(out):
+            d["u"] = us[k].copy()
     ctx.check(rc)
     return u, out

@@ -349,6 +349,26 @@ def linear_lps_bar(ref):
     return out
 
 
+def thermal_expansion_cube(ref):
+    """ThermalExpansionCube (verification): 5x5x5 PdQuickGrid cube, elastic
+    LPS with thermal expansion coefficient 1e-5, the whole domain's
+    temperature prescribed as 273 + 1000 t / 3600, rigid motions held by
+    zero displacements on three axis node sets; quasi-static, ten load
+    steps, eleven frames, the .comp tolerances."""
+    from scipy.io import netcdf_file
+    d = os.path.join(ref, "test/verification/ThermalExpansionCube")
+    f = netcdf_file(os.path.join(d, "ThermalExpansionCube_gold.e"), "r", mmap=False)
+    time = np.asarray(f.variables["time_whole"][:])
+    out = dict(time=time,
+               coords=np.stack([np.asarray(f.variables[k][:]) for k in ("coordx", "coordy", "coordz")], 1),
+               tol_displacement=np.array(1e-9), tol_velocity=np.array(1e-9), tol_force=np.array(1e-3),
+               tol_temperature=np.array(1e-9), tol_weighted_volume=np.array(1e-12),
+               tol_dilatation=np.array(1e-12))
+    out.update(_exodus_all_blocks(f, list(range(len(time)))))
+    f.close()
+    return out
+
+
 def copy_decks(ref, here):
     """The input decks and meshes of the WaveInBar and PrecrackedPlate
     regression tests (data files, copied unchanged)."""
@@ -367,7 +387,8 @@ def copy_decks(ref, here):
                       ("test/verification/Contact_Friction/Contact_Friction.xml", None),
                       ("test/verification/Contact_Friction/Contact_Friction.g", None),
                       ("test/verification/LinearLPSBar/LinearLPSBar.yaml", None),
-                      ("test/verification/LinearLPSBar/LinearLPSBar.g", None)):
+                      ("test/verification/LinearLPSBar/LinearLPSBar.g", None),
+                      ("test/verification/ThermalExpansionCube/ThermalExpansionCube.xml", None)):
         shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, name or os.path.basename(rel)))
 
 
@@ -387,6 +408,8 @@ def main():
     print("wrote contact_2x1x1.npz, contact_friction.npz")
     np.savez_compressed(os.path.join(here, "linear_lps_bar.npz"), **linear_lps_bar(ref))
     print("wrote linear_lps_bar.npz")
+    np.savez_compressed(os.path.join(here, "thermal_expansion_cube.npz"), **thermal_expansion_cube(ref))
+    print("wrote thermal_expansion_cube.npz")
     copy_decks(ref, here)
     print("copied decks")
     fx = {

@@ -668,3 +668,94 @@ def test_linear_lps_bar_oracle_against_gold(golden_dir):
             assert np.abs(F[:, c] - g["Force" + ax][k]).max() <= float(g["tol_force"]), (k, ax)
     # the two models differ at this strain (the linearization is visible)
     assert np.abs(th_l[lin] - th_e[lin]).max() > 1e3 * float(g["tol_dilatation"])
+
+
+def test_thermal_expansion_cube_oracle_against_gold(golden_dir):
+    """ThermalExpansionCube (verification deck): a 5x5x5 cube heated by
+    1000 degrees over ten quasi-static load steps expands freely. The
+    oracle's thermal strain (src/materials/elastic.cxx:112-113,
+    material_utilities.cxx:281-282, e -= alpha dT zeta with the temperature
+    change T(t) - T(0) at each load step's time) in its restatement of the
+    load-stepping driver reproduces all eleven gold frames of displacement,
+    velocity (the last increment over the step), force, dilatation and
+    weighted volume at the .comp tolerances. Free expansion is stress free:
+    the displacement is alpha dT (x - x_corner)."""
+    import os
+    from oracle import qs_driver as QD
+    from peridigm_amd import deck as Dk
+    g = np.load(os.path.join(golden_dir, "thermal_expansion_cube.npz"))
+    d = Dk.parse_deck(os.path.join(golden_dir, "decks", "ThermalExpansionCube.xml"))
+    gen = d["Discretization"]["TensorProduct3DMeshGenerator"]
+    x, vol, nl = O.quickgrid(5, 5, 5, [gen["X Origin"], gen["Y Origin"], gen["Z Origin"]],
+                             [gen["X Length"], gen["Y Length"], gen["Z Length"]], 1.75,
+                             spherical=True)
+    assert np.abs(x - g["coords"]).max() <= 1e-12
+    N = len(vol)
+    _, _, presc = Dk._bcs(d, x, {})
+    c0, c1 = Dk._temperature(d, x, {})
+    alpha_t = d["Materials"]["My Elastic Material"]["Thermal Expansion Coefficient"]
+    K, mu, hz = 130.0e9, 78.0e9, 1.75
+    ts = [0.0] + [(i + 1) * 3600.0 / 10 for i in range(10)]
+    dT = lambda t: (c1 * t + c0) - c0
+    m = O.weighted_volume(x, vol, nl, hz)
+    bd = np.zeros(O.num_bonds(nl, N))
+    u_prev = np.zeros((N, 3))
+    for k, t in enumerate(ts):
+        assert np.abs(c1 * t + c0 - g["Temperature"][k]).max() <= float(g["tol_temperature"])
+        assert np.abs(dT(t) - g["Temperature_Change"][k]).max() <= float(g["tol_temperature"])
+        if k == 0:
+            u = np.zeros((N, 3))
+            v = np.zeros((N, 3))
+        else:
+            u, steps, aborted = QD.qs_run(x, vol, nl, K, mu, hz, presc, ts[:k + 1],
+                                          maxSolverIterations=10, tolerance=1e-2,
+                                          absoluteTolerance=True, thermal=(alpha_t, dT))
+            assert not aborted
+            v = (u - u_prev) / (t - ts[k - 1])
+        th, f, _ = O.compute_force_thermal(x, x + u, m, vol, bd, nl, K, mu, hz, alpha_t, dT(t))
+        F = f * vol[:, None]
+        for c, ax in enumerate("XYZ"):
+            assert np.abs(u[:, c] - g["Displacement" + ax][k]).max() <= float(g["tol_displacement"]), (k, ax)
+            assert np.abs(v[:, c] - g["Velocity" + ax][k]).max() <= float(g["tol_velocity"]), (k, ax)
+            assert np.abs(F[:, c] - g["Force" + ax][k]).max() <= float(g["tol_force"]), (k, ax)
+        assert np.abs(th - g["Dilatation"][k]).max() <= float(g["tol_dilatation"]), k
+        assert np.abs(m - g["Weighted_Volume"][k]).max() <= float(g["tol_weighted_volume"])
+        u_prev = u
+    corner = x[0]
+    assert np.abs(u - alpha_t * 1000.0 * (x - corner)).max() <= 1e-9
+
+
+def test_thermal_strain_and_partial_stress_invariants():
+    """What the mathematics fixes for the thermal term and the partial
+    stress: free thermal expansion u = alpha dT x leaves no force and no
+    dilatation; the same stretch without the temperature change gives the
+    LPS force of a uniform strain; the partial stress of a rigid translation
+    vanishes and, for uniform strain, is symmetric in the bulk and equals
+    sum_j f_ij (x_j - x_i)^T V_j recomputed from the pair forces."""
+    x, vol, nl = O.quickgrid(8, 8, 8, [0, 0, 0], [8.0, 8.0, 8.0], 3.015)
+    N = len(vol)
+    K, mu, hz, at = 130e9, 78e9, 3.015, 1.2e-5
+    m = O.weighted_volume(x, vol, nl, hz)
+    bd = np.zeros(O.num_bonds(nl, N))
+    rng = np.random.default_rng(4)
+    dT = rng.uniform(200.0, 300.0)
+    u = at * dT * x
+    th, f, ps = O.compute_force_thermal(x, x + u, m, vol, bd, nl, K, mu, hz, at, dT,
+                                        partial_stress=True)
+    assert np.abs(th).max() <= 1e-12 and np.abs(f).max() <= 1e-9 * K * at * dT
+    th0, f0 = O.compute_force(x, x + u, m, vol, bd, nl, K, mu, hz)
+    thn, fn_, _ = O.compute_force_thermal(x, x + u, m, vol, bd, nl, K, mu, hz, at, None)
+    assert np.array_equal(th0, thn) and np.array_equal(f0, fn_)
+    assert np.abs(th0 - 3 * at * dT).max() < 1e-2 * 3 * at * dT  # interior theta ~ 3 eps
+    # rigid translation: no partial stress
+    _, _, ps0 = O.compute_force_thermal(x, x + 0.3, m, vol, bd, nl, K, mu, hz, 0.0, None,
+                                        partial_stress=True)
+    assert np.abs(ps0).max() <= 1e-12 * K  # rounding of |Y| - |xi| only
+    # uniform shear: bulk partial stress symmetric
+    eps = np.array([[0, 2e-4, 0], [2e-4, 0, 0], [0, 0, 0]])
+    _, _, ps1 = O.compute_force_thermal(x, x + x @ eps.T, m, vol, bd, nl, K, mu, hz, 0.0, None,
+                                        partial_stress=True)
+    mid = np.nonzero(np.all(np.abs(x - 4.0) < 1.1, axis=1))[0]
+    for i in mid:
+        assert np.abs(ps1[i] - ps1[i].T).max() <= 1e-6 * np.abs(ps1[i]).max()
+        assert abs(ps1[i][0, 1]) > 0.5 * 2 * mu * 2e-4
