@@ -79,7 +79,13 @@ inline long long list_size(int numOwned, const int* nlist) {
   return li;
 }
 
-// One device context per (material, neighbourhood list).
+// One device context per neighbourhood list, shared by the material and the
+// damage model of a block (one CSR build and one set of device arrays). The
+// material sets the elastic constants and influence function, the damage
+// model the critical stretch; the context is rebuilt only when the list,
+// the coordinates or those constants change. Bindings are recognised by the
+// arrays' addresses and sizes, as Peridigm keeps them for a block's life;
+// after editing a bound array in place, call invalidate().
 class Device {
  public:
   Device() {
@@ -91,24 +97,63 @@ class Device {
   Device& operator=(const Device&) = delete;
   pdg_ctx* get() const { return ctx_; }
 
-  // (Re)builds topology and constants when the list or coordinates change.
-  void bind(int numOwned, const int* ownedIDs, const int* nlist, const double* x,
-            const double* vol, double K, double mu, double horizon, int influence, double sc) {
-    if (nlist == nlist_ && numOwned == n_ && x == x_) return;
-    check(ctx_, pdg_set_topology(ctx_, numOwned, ownedIDs, nlist, list_size(numOwned, nlist)),
-          "pdg_set_topology");
-    check(ctx_, pdg_set_constants(ctx_, x, vol, K, mu, horizon, influence, sc), "pdg_set_constants");
-    nlist_ = nlist;
-    n_ = numOwned;
-    x_ = x;
+  void bind_material(int numOwned, const int* ownedIDs, const int* nlist, const double* x,
+                     const double* vol, double K, double mu, double horizon, int influence) {
+    Consts c = c_;
+    c.K = K; c.mu = mu; c.horizon = horizon; c.influence = influence; c.material = true;
+    bind(numOwned, ownedIDs, nlist, x, vol, c);
   }
+  void bind_damage(int numOwned, const int* ownedIDs, const int* nlist, const double* x,
+                   const double* vol, double horizon, double sc) {
+    Consts c = c_;
+    if (!c.material) c.horizon = horizon;
+    c.sc = sc;
+    bind(numOwned, ownedIDs, nlist, x, vol, c);
+  }
+  void invalidate() { n_ = -1; }
 
  private:
+  struct Consts {
+    double K = 1.0, mu = 1.0, horizon = 0.0, sc = 0.0;
+    int influence = PDG_INFLUENCE_ONE;
+    bool material = false;
+    bool operator==(const Consts& o) const {
+      return K == o.K && mu == o.mu && horizon == o.horizon && sc == o.sc &&
+             influence == o.influence && material == o.material;
+    }
+  };
+  void bind(int numOwned, const int* ownedIDs, const int* nlist, const double* x,
+            const double* vol, const Consts& c) {
+    const bool same_list = nlist == nlist_ && numOwned == n_ && x == x_ && vol == vol_;
+    if (same_list && c == c_) return;
+    if (!same_list) {
+      size_ = list_size(numOwned, nlist);
+      check(ctx_, pdg_set_topology(ctx_, numOwned, ownedIDs, nlist, size_), "pdg_set_topology");
+    }
+    check(ctx_, pdg_set_constants(ctx_, x, vol, c.K, c.mu, c.horizon, c.influence, c.sc),
+          "pdg_set_constants");
+    nlist_ = nlist; n_ = numOwned; x_ = x; vol_ = vol; c_ = c;
+  }
   pdg_ctx* ctx_ = nullptr;
   const int* nlist_ = nullptr;
   const double* x_ = nullptr;
+  const double* vol_ = nullptr;
+  long long size_ = 0;
   int n_ = -1;
+  Consts c_;
 };
+
+// The context shared by every model bound to the same neighbourhood list
+// (held while any of them lives).
+inline std::shared_ptr<Device> shared_device(const int* nlist) {
+  static std::map<const int*, std::weak_ptr<Device>> reg;
+  auto it = reg.find(nlist);
+  if (it != reg.end())
+    if (auto d = it->second.lock()) return d;
+  auto d = std::make_shared<Device>();
+  reg[nlist] = d;
+  return d;
+}
 
 }  // namespace detail
 
@@ -120,7 +165,7 @@ struct ElasticParams {
 // Linear peridynamic solid ("Elastic"), src/materials/Peridigm_ElasticMaterial.cpp:59-172.
 class ElasticMaterial {
  public:
-  explicit ElasticMaterial(const ElasticParams& p) : p_(p), dev_(new detail::Device()) {}
+  explicit ElasticMaterial(const ElasticParams& p) : p_(p) {}
   double Density() const { return p_.density; }
   double BulkModulus() const { return p_.bulkModulus; }
   double ShearModulus() const { return p_.shearModulus; }
@@ -132,6 +177,10 @@ class ElasticMaterial {
     detail::check(dev_->get(), pdg_initialize(dev_->get(), dm.getData("Weighted_Volume", STEP_NONE)),
                   "pdg_initialize");
   }
+  // After editing a bound neighbourhood list or coordinates in place.
+  void invalidate() const { if (dev_) dev_->invalidate(); }
+  // The device context in use (shared with a damage model on the same list).
+  const pdg_ctx* context() const { return dev_ ? dev_->get() : nullptr; }
   // Force_Density and Dilatation (STEP_NP1) from Coordinates NP1 and
   // Bond_Damage NP1; Force_Density is overwritten (zeroed first in the reference).
   void computeForce(double dt, int numOwnedPoints, const int* ownedIDs,
@@ -147,18 +196,21 @@ class ElasticMaterial {
 
  private:
   void bind(int n, const int* ids, const int* nl, DataManager& dm) const {
-    dev_->bind(n, ids, nl, dm.getData("Model_Coordinates", STEP_NONE), dm.getData("Volume", STEP_NONE),
-               p_.bulkModulus, p_.shearModulus, p_.horizon, p_.influence, 0.0);
+    if (!dev_ || nl != list_) { dev_ = detail::shared_device(nl); list_ = nl; }
+    dev_->bind_material(n, ids, nl, dm.getData("Model_Coordinates", STEP_NONE),
+                        dm.getData("Volume", STEP_NONE), p_.bulkModulus, p_.shearModulus,
+                        p_.horizon, p_.influence);
   }
   ElasticParams p_;
-  std::shared_ptr<detail::Device> dev_;
+  mutable std::shared_ptr<detail::Device> dev_;
+  mutable const int* list_ = nullptr;
 };
 
 // Critical stretch, src/damage/Peridigm_CriticalStretchDamageModel.cpp:55-184.
 class CriticalStretchDamageModel {
  public:
   CriticalStretchDamageModel(double criticalStretch, double horizon)
-      : sc_(criticalStretch), horizon_(horizon), dev_(new detail::Device()) {}
+      : sc_(criticalStretch), horizon_(horizon) {}
   // Damage and Bond_Damage (NP1) set to zero.
   void initialize(double /*dt*/, int numOwnedPoints, const int* /*ownedIDs*/,
                   const int* /*neighborhoodList*/, DataManager& dm) const {
@@ -170,8 +222,13 @@ class CriticalStretchDamageModel {
   // Bond_Damage NP1 from Bond_Damage N and Coordinates NP1; Damage NP1.
   void computeDamage(double /*dt*/, int numOwnedPoints, const int* ownedIDs,
                      const int* neighborhoodList, DataManager& dm) const {
-    dev_->bind(numOwnedPoints, ownedIDs, neighborhoodList, dm.getData("Model_Coordinates", STEP_NONE),
-               dm.getData("Volume", STEP_NONE), 1.0, 1.0, horizon_, PDG_INFLUENCE_ONE, sc_);
+    if (!dev_ || neighborhoodList != list_) {
+      dev_ = detail::shared_device(neighborhoodList);
+      list_ = neighborhoodList;
+    }
+    dev_->bind_damage(numOwnedPoints, ownedIDs, neighborhoodList,
+                      dm.getData("Model_Coordinates", STEP_NONE), dm.getData("Volume", STEP_NONE),
+                      horizon_, sc_);
     long long near = 0;
     detail::check(dev_->get(),
                   pdg_compute_damage(dev_->get(), dm.getData("Coordinates", STEP_NP1),
@@ -182,10 +239,13 @@ class CriticalStretchDamageModel {
     nearCritical_ = near;
   }
   long long nearCriticalBonds() const { return nearCritical_; }
+  const pdg_ctx* context() const { return dev_ ? dev_->get() : nullptr; }
+  void invalidate() const { if (dev_) dev_->invalidate(); }
 
  private:
   double sc_, horizon_;
-  std::shared_ptr<detail::Device> dev_;
+  mutable std::shared_ptr<detail::Device> dev_;
+  mutable const int* list_ = nullptr;
   mutable long long nearCritical_ = 0;
 };
 

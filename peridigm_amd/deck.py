@@ -21,9 +21,10 @@ Supported subset (anything else raises DeckError):
   - elastic constants: any two of bulk, shear, Young's modulus and Poisson's
     ratio (src/materials/Peridigm_Material.cpp:335-433);
   - block groups that together hold all the mesh's blocks and share one
-    material, horizon, damage model and influence function ("One",
-    "Parabolic Decay" or "Gaussian"); "Omit Bonds Between Blocks" "All" or
-    "None" (src/io/discretization/Peridigm_ExodusDiscretization.cpp:709-760);
+    horizon; groups of different materials may share no bond (each runs in
+    a context of its own); the influence function ("One", "Parabolic Decay"
+    or "Gaussian") from the Discretization list; "Omit Bonds Between Blocks"
+    "All" or "None" (src/io/discretization/Peridigm_ExodusDiscretization.cpp:709-760);
   - short-range contact ("Contact": Search Radius, Search Frequency and one
     "Short Range Force" model, every node taking part).
 """
@@ -479,8 +480,17 @@ def _material(deck, grp):
         else:
             raise DeckError("damage model %r not supported" % dm.get("Damage Model"))
     return dict(model=model, K=K, mu=mu, planar=planar, height=height,
-                rho=float(mat["Density"]), infl=INFLUENCE[grp.get("Influence Function", "One")],
+                rho=float(mat["Density"]),
+                # one influence function for the run, from the discretization
+                # list (src/core/Peridigm.cpp:184-185)
+                infl=_influence(deck["Discretization"].get("Influence Function", "One")),
                 sc=sc, sc_fn=sc_fn, alpha_t=float(mat.get("Thermal Expansion Coefficient", 0.0)))
+
+
+def _influence(name):
+    if name not in INFLUENCE:
+        raise DeckError("influence function %r not supported (One, Parabolic Decay, Gaussian)" % name)
+    return INFLUENCE[name]
 
 
 def _subset(idx, N, nl, v0, u0, presc):
@@ -535,7 +545,7 @@ def run_deck(path, output=None, lanes=4):
     point_group = (np.array([owner[mesh_blocks[b]] for b in blk], np.int64) if owner
                    else np.zeros(N, np.int64))
     # groups of one material, damage model and influence function run together
-    sig = [(g.get("Material"), g.get("Damage Model"), g.get("Influence Function")) for g in groups]
+    sig = [(g.get("Material"), g.get("Damage Model")) for g in groups]
     run_of = {}
     for s_ in sig:
         run_of.setdefault(s_, len(run_of))

@@ -165,12 +165,63 @@ static void testCriticalStretch() {
   for (int i = 0; i < 3; i++) expect_rel("damage: node", d[i], dref[i], 0.0);
 }
 
+// A material and a damage model bound to the same neighbourhood list share
+// one device context, each with its own constants: the damage model breaks
+// the stretched bond and the material's force on the chain is unchanged by
+// the damage model's binding.
+static void testSharedContext() {
+  const double x[] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
+  const double y[] = {0, 0, 0, 1.03, 0, 0, 0, 1.01, 0};
+  std::vector<int> nl = {2, 1, 2, 1, 0, 1, 0};
+  int owned[] = {0, 1, 2};
+  auto setup = [&](pdgpu::DataManager& dm) {
+    double* X = dm.getData("Model_Coordinates", pdgpu::STEP_NONE);
+    double* Y = dm.getData("Coordinates", pdgpu::STEP_NP1);
+    double* V = dm.getData("Volume", pdgpu::STEP_NONE);
+    for (int i = 0; i < 9; i++) { X[i] = x[i]; Y[i] = y[i]; }
+    for (int i = 0; i < 3; i++) V[i] = 1.0;
+  };
+  pdgpu::ElasticParams p;
+  p.density = 7800.0; p.bulkModulus = 130.0e9; p.shearModulus = 78.0e9; p.horizon = 1.5;
+  // alone
+  pdgpu::DataManager a(3, 4);
+  setup(a);
+  std::vector<double> f_alone(9);
+  {
+    pdgpu::ElasticMaterial mat(p);
+    mat.initialize(1.0, 3, owned, nl.data(), a);
+    mat.computeForce(1.0, 3, owned, nl.data(), a);
+    const double* f = a.getData("Force_Density", pdgpu::STEP_NP1);
+    for (int i = 0; i < 9; i++) f_alone[i] = f[i];
+  }
+  // with a damage model on the same list
+  pdgpu::DataManager b(3, 4);
+  setup(b);
+  pdgpu::ElasticMaterial mat(p);
+  pdgpu::CriticalStretchDamageModel dmg(0.02, 1.5);
+  mat.initialize(1.0, 3, owned, nl.data(), b);
+  dmg.initialize(1.0, 3, owned, nl.data(), b);
+  b.updateState();
+  dmg.computeDamage(1.0, 3, owned, nl.data(), b);
+  expect_rel("shared: one context", mat.context() == dmg.context() ? 1.0 : 0.0, 1.0, 0.0);
+  const double* bd = b.getData("Bond_Damage", pdgpu::STEP_NP1);
+  const double bref[] = {1, 0, 1, 0};
+  for (int k = 0; k < 4; k++) expect_rel("shared: bond damage", bd[k], bref[k], 0.0);
+  // the force with the undamaged field equals the material's alone
+  double* bdn = b.getData("Bond_Damage", pdgpu::STEP_NP1);
+  for (int k = 0; k < 4; k++) bdn[k] = 0.0;
+  mat.computeForce(1.0, 3, owned, nl.data(), b);
+  const double* f = b.getData("Force_Density", pdgpu::STEP_NP1);
+  for (int i = 0; i < 9; i++) expect_rel("shared: force", f[i], f_alone[i], 1e-12);
+}
+
 int main() {
   try {
     testTwoPts();
     testEightPts();
     testThreePts();
     testCriticalStretch();
+    testSharedContext();
   } catch (const std::exception& e) {
     std::printf("FAIL exception: %s\n", e.what());
     return 100;
