@@ -147,6 +147,18 @@ template <typename T>
 __device__ __forceinline__ T lds_at(const char* arr, int jb) {
   return *reinterpret_cast<const T*>(arr + jb);
 }
+// One 16-byte slot as a single ds_read_b128. The read is volatile so that
+// the compiler keeps it whole: combining two of its dwords into a double
+// otherwise becomes a second, 8-byte read of the same address, issued after
+// a wait for the first (both write the same registers).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const volatile u32x4 lds_u32x4;
+__device__ __forceinline__ u32x4 lds_slot(const char* arr, int jb) {
+  return *(lds_u32x4*)(arr + jb);
+}
+__device__ __forceinline__ double dbl(unsigned lo, unsigned hi) {
+  return __hiloint2double((int)hi, (int)lo);
+}
 template <typename T>
 __device__ __forceinline__ void lds_put(char* arr, int h, const T& v) {
   *reinterpret_cast<T*>(arr + kLdsSlot * h) = v;
@@ -168,26 +180,25 @@ struct Nb2 {  // pass 2
 
 __device__ __forceinline__ Nb nb_load(const TileLds& L, int jb) {
   Nb n;
-  const uint4 a = lds_at<uint4>(lds_a(L), jb);
-  const uint4 b = lds_at<uint4>(lds_b(L), jb);
+  const u32x4 a = lds_slot(lds_a(L), jb);
+  const u32x4 b = lds_slot(lds_b(L), jb);
   n.x0 = a.x; n.x1 = a.y; n.x2 = a.z; n.vol = __uint_as_float(a.w);
-  n.u0 = __hiloint2double((int)b.y, (int)b.x);
-  n.u1 = __hiloint2double((int)b.w, (int)b.z);
+  n.u0 = dbl(b.x, b.y);
+  n.u1 = dbl(b.z, b.w);
   n.u2 = lds_at<double>(lds_c(L), jb);
   return n;
 }
 
 __device__ __forceinline__ Nb2 nb2_load(const TileLds& L, int jb) {
   Nb2 n;
-  // three 16-byte reads (uint4, so none is split into two 8-byte halves,
-  // which would conflict at this stride)
-  const uint4 a = lds_at<uint4>(lds_a(L), jb);
-  const uint4 b = lds_at<uint4>(lds_b(L), jb);
-  const uint4 c = lds_at<uint4>(lds_c(L), jb);
+  // three whole 16-byte reads (8-byte halves would conflict at this stride)
+  const u32x4 a = lds_slot(lds_a(L), jb);
+  const u32x4 b = lds_slot(lds_b(L), jb);
+  const u32x4 c = lds_slot(lds_c(L), jb);
   n.x0 = a.x; n.x1 = a.y; n.x2 = a.z; n.vol = __uint_as_float(a.w);
-  n.u0 = __hiloint2double((int)b.y, (int)b.x);
-  n.u1 = __hiloint2double((int)b.w, (int)b.z);
-  n.u2 = __hiloint2double((int)c.y, (int)c.x);
+  n.u0 = dbl(b.x, b.y);
+  n.u1 = dbl(b.z, b.w);
+  n.u2 = dbl(c.x, c.y);
   n.vd = __uint_as_float(c.z);
   n.vp = __uint_as_float(c.w);
   return n;
@@ -402,13 +413,16 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       // per-iteration ballots of new breaks stay wave-uniform (scalar
       // registers) and are merged into the group's mask words after the loop.
       unsigned long long balv[MAXDEG / G];
+#pragma unroll
+      for (int it = 0; it < MAXDEG / G; ++it) balv[it] = 0ull;
       Nb nb[2];
       nb[0] = nb_load(L, lj[0]);
+      // the next bond's slots are read whenever this bond is computed, as
+      // in pass 2
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
-          if (it + 1 < MAXDEG / G && it + 1 < nmax)
-            nb[(it + 1) & 1] = nb_load(L, lj[(it + 1) % (MAXDEG / G)]);
+          if (it + 1 < MAXDEG / G) nb[(it + 1) & 1] = nb_load(L, lj[it + 1]);
           const Nb& cur = nb[it & 1];
           // lanes whose bond exists and is unbroken; the masks stay in scalar
           // registers, so combining them costs no vector instructions
@@ -432,8 +446,6 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
           const float nacc = fmaf(term, cur.vol, acc);
           acc = __builtin_amdgcn_inverse_ballot_w64(okm & ~brkm) ? nacc : acc;
           if (DAMAGE) balv[it] = brkm;
-        } else if (DAMAGE) {
-          balv[it] = 0ull;
         }
       }
       if (DAMAGE) {
@@ -570,11 +582,13 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
       const int nmax = wave_trips<G>(deg);
       Nb2 nb[2];
       nb[0] = nb2_load(L, lj[0]);
+      // the next bond's slots are read whenever this bond is computed (every
+      // lj entry, past a row's end too, is a slot of the staging arrays), so
+      // each bond's math waits only for its own reads
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
-          if (it + 1 < MAXDEG / G && it + 1 < nmax)
-            nb[(it + 1) & 1] = nb2_load(L, lj[(it + 1) % (MAXDEG / G)]);
+          if (it + 1 < MAXDEG / G) nb[(it + 1) & 1] = nb2_load(L, lj[it + 1]);
           const Nb2& cur = nb[it & 1];
           const unsigned long long okm = intact_lanes<G>(mo, it, l);
           const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
