@@ -415,15 +415,16 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       unsigned long long balv[MAXDEG / G];
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) balv[it] = 0ull;
-      Nb nb[2];
+      Nb nb[3];
       nb[0] = nb_load(L, lj[0]);
-      // the next bond's slots are read whenever this bond is computed, as
-      // in pass 2
+      nb[1] = nb_load(L, lj[1]);
+      // the slots of the bond after next are read whenever a bond is
+      // computed, as in pass 2
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
-          if (it + 1 < MAXDEG / G) nb[(it + 1) & 1] = nb_load(L, lj[it + 1]);
-          const Nb& cur = nb[it & 1];
+          if (it + 2 < MAXDEG / G) nb[(it + 2) % 3] = nb_load(L, lj[it + 2]);
+          const Nb& cur = nb[it % 3];
           // lanes whose bond exists and is unbroken; the masks stay in scalar
           // registers, so combining them costs no vector instructions
           const unsigned long long okm = intact_lanes<G>(mo, it, l);
@@ -580,16 +581,17 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
       const int nmax = wave_trips<G>(deg);
-      Nb2 nb[2];
+      Nb2 nb[3];
       nb[0] = nb2_load(L, lj[0]);
-      // the next bond's slots are read whenever this bond is computed (every
-      // lj entry, past a row's end too, is a slot of the staging arrays), so
-      // each bond's math waits only for its own reads
+      nb[1] = nb2_load(L, lj[1]);
+      // the slots of the bond after next are read whenever a bond is
+      // computed (every lj entry, past a row's end too, is a slot of the
+      // staging arrays), so each bond's math waits only for its own reads
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) {
         if (it < nmax) {
-          if (it + 1 < MAXDEG / G) nb[(it + 1) & 1] = nb2_load(L, lj[it + 1]);
-          const Nb2& cur = nb[it & 1];
+          if (it + 2 < MAXDEG / G) nb[(it + 2) % 3] = nb2_load(L, lj[it + 2]);
+          const Nb2& cur = nb[it % 3];
           const unsigned long long okm = intact_lanes<G>(mo, it, l);
           const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
                                         cur.u0, cur.u1, cur.u2));
