@@ -156,6 +156,11 @@ typedef __attribute__((address_space(3))) const volatile u32x4 lds_u32x4;
 __device__ __forceinline__ u32x4 lds_slot(const char* arr, int jb) {
   return *(lds_u32x4*)(arr + jb);
 }
+// The same for an 8-byte value (kept where it is issued, not moved to its use).
+typedef __attribute__((address_space(3))) const volatile double lds_f64;
+__device__ __forceinline__ double lds_dbl(const char* arr, int jb) {
+  return *(lds_f64*)(arr + jb);
+}
 __device__ __forceinline__ double dbl(unsigned lo, unsigned hi) {
   return __hiloint2double((int)hi, (int)lo);
 }
@@ -185,7 +190,7 @@ __device__ __forceinline__ Nb nb_load(const TileLds& L, int jb) {
   n.x0 = a.x; n.x1 = a.y; n.x2 = a.z; n.vol = __uint_as_float(a.w);
   n.u0 = dbl(b.x, b.y);
   n.u1 = dbl(b.z, b.w);
-  n.u2 = lds_at<double>(lds_c(L), jb);
+  n.u2 = lds_dbl(lds_c(L), jb);
   return n;
 }
 
@@ -358,6 +363,57 @@ __device__ __forceinline__ unsigned long long intact_lanes(const unsigned long l
 
 // Pass 1 (damage + dilatation) of one tile; every thread of the workgroup
 // calls it with the same tile.
+// Pass 1's pipelined bond loop as a chain of template instances, as
+// P2Loop: step<IT> reads the slots of bond IT + 2, computes bond IT's
+// stretch, break decision, near-critical count and dilatation term, and
+// returns at the wave's trip count.
+template <int G, bool DAMAGE, bool NEAR, int INFL>
+struct P1Loop {
+  const TileLds& L;
+  const int (&lj)[MAXDEG / G];
+  const unsigned long long (&mo)[NWORDS];
+  int l, nmax;
+  unsigned xi0, xi1, xi2;
+  double ui0, ui1, ui2;
+  float scz, nlim, hhat;
+  Nb nb[3];
+  unsigned long long balv[MAXDEG / G];
+  float acc = 0.0f;
+  unsigned nnear_w = 0;
+
+  template <int IT>
+  __device__ __forceinline__ void step() {
+    if constexpr (IT < MAXDEG / G) {
+      if (IT >= nmax) return;
+      if constexpr (IT + 2 < MAXDEG / G) nb[(IT + 2) % 3] = nb_load(L, lj[IT + 2]);
+      const Nb& cur = nb[IT % 3];
+      // lanes whose bond exists and is unbroken; the masks stay in scalar
+      // registers, so combining them costs no vector instructions
+      const unsigned long long okm = intact_lanes<G>(mo, IT, l);
+      const BondS b = bond_stretch(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
+                                          cur.u0, cur.u1, cur.u2));
+      unsigned long long brkm = 0ull;  // lanes whose bond breaks now
+      if (DAMAGE) {
+        if (NEAR) {
+          // intact bonds with |s - s_c| <= near_rel * s_c, counted as a
+          // wave-wide lane mask (scalar population count)
+          const unsigned long long nm =
+              okm & __builtin_amdgcn_fcmpf(fabsf(b.s - scz), nlim, CMP_FOLE);
+          nnear_w += (unsigned)__builtin_popcountll(nm);
+        }
+        brkm = okm & __builtin_amdgcn_fcmpf(b.s, scz, CMP_FOGT);
+      }
+      // w zeta e V = w z2 s V
+      float term = b.z2 * b.s;
+      if (INFL != PDG_INFLUENCE_ONE) term *= omega_t<INFL>(b.z2, hhat);
+      const float nacc = fmaf(term, cur.vol, acc);
+      acc = __builtin_amdgcn_inverse_ballot_w64(okm & ~brkm) ? nacc : acc;
+      if (DAMAGE) balv[IT] = brkm;
+      step<IT + 1>();
+    }
+  }
+};
+
 template <int G, bool DAMAGE, bool NEAR, int INFL, bool PIPE>
 __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
                                            NodeRec* __restrict__ rec, float* __restrict__ theta,
@@ -412,43 +468,18 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       // are selected away; the next bond's LDS data is in flight. The
       // per-iteration ballots of new breaks stay wave-uniform (scalar
       // registers) and are merged into the group's mask words after the loop.
-      unsigned long long balv[MAXDEG / G];
-#pragma unroll
-      for (int it = 0; it < MAXDEG / G; ++it) balv[it] = 0ull;
-      Nb nb[3];
-      nb[0] = nb_load(L, lj[0]);
-      nb[1] = nb_load(L, lj[1]);
       // the slots of the bond after next are read whenever a bond is
       // computed, as in pass 2
+      P1Loop<G, DAMAGE, NEAR, INFL> lp{L, lj, mo, l, nmax, xi0, xi1, xi2, ui0, ui1, ui2,
+                                       scz, nlim, c.hhat, {}, {}};
 #pragma unroll
-      for (int it = 0; it < MAXDEG / G; ++it) {
-        if (it < nmax) {
-          if (it + 2 < MAXDEG / G) nb[(it + 2) % 3] = nb_load(L, lj[it + 2]);
-          const Nb& cur = nb[it % 3];
-          // lanes whose bond exists and is unbroken; the masks stay in scalar
-          // registers, so combining them costs no vector instructions
-          const unsigned long long okm = intact_lanes<G>(mo, it, l);
-          const BondS b = bond_stretch(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1,
-                                              cur.x2, cur.u0, cur.u1, cur.u2));
-          unsigned long long brkm = 0ull;  // lanes whose bond breaks now
-          if (DAMAGE) {
-            if (NEAR) {
-              // intact bonds with |s - s_c| <= near_rel * s_c, counted as a
-              // wave-wide lane mask (scalar population count)
-              const unsigned long long nm =
-                  okm & __builtin_amdgcn_fcmpf(fabsf(b.s - scz), nlim, CMP_FOLE);
-              nnear_w += (unsigned)__builtin_popcountll(nm);
-            }
-            brkm = okm & __builtin_amdgcn_fcmpf(b.s, scz, CMP_FOGT);
-          }
-          // w zeta e V = w z2 s V
-          float term = b.z2 * b.s;
-          if (INFL != PDG_INFLUENCE_ONE) term *= omega_t<INFL>(b.z2, c.hhat);
-          const float nacc = fmaf(term, cur.vol, acc);
-          acc = __builtin_amdgcn_inverse_ballot_w64(okm & ~brkm) ? nacc : acc;
-          if (DAMAGE) balv[it] = brkm;
-        }
-      }
+      for (int it = 0; it < MAXDEG / G; ++it) lp.balv[it] = 0ull;
+      lp.nb[0] = nb_load(L, lj[0]);
+      lp.nb[1] = nb_load(L, lj[1]);
+      lp.template step<0>();
+      acc = lp.acc;
+      nnear_w = lp.nnear_w;
+      const unsigned long long (&balv)[MAXDEG / G] = lp.balv;
       if (DAMAGE) {
         // new breaks are rare: a wave-uniform test skips the iterations
         // without any
@@ -525,6 +556,41 @@ __global__ __launch_bounds__(WG, WAVES_PER_SIMD) void k_pass1_tiled(BondArgs a, 
 }
 
 // Pass 2 (both-ends force + Verlet epilogue) of one tile.
+// Pass 2's pipelined bond loop as a chain of template instances, bond IT
+// computed by step<IT>: each instance reads the slots of bond IT + 2 and
+// returns at the wave's trip count, so bond IT + 1 is reached only from bond
+// IT (a straight chain: each bond waits only for its own reads).
+template <int G, int INFL>
+struct P2Loop {
+  const TileLds& L;
+  const int (&lj)[MAXDEG / G];
+  const unsigned long long (&mo)[NWORDS];
+  int l, nmax;
+  unsigned xi0, xi1, xi2;
+  double ui0, ui1, ui2;
+  float p_i, d_i, hhat, pmb_c;
+  Nb2 nb[3];
+  float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+
+  template <int IT>
+  __device__ __forceinline__ void step() {
+    if constexpr (IT < MAXDEG / G) {
+      if (IT >= nmax) return;
+      if constexpr (IT + 2 < MAXDEG / G) nb[(IT + 2) % 3] = nb2_load(L, lj[IT + 2]);
+      const Nb2& cur = nb[IT % 3];
+      const unsigned long long okm = intact_lanes<G>(mo, IT, l);
+      const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
+                                    cur.u0, cur.u1, cur.u2));
+      float tt = pair_tv<INFL>(b, p_i, d_i, cur.vol, cur.vp, cur.vd, hhat, pmb_c);
+      tt = __builtin_amdgcn_inverse_ballot_w64(okm) ? tt : 0.0f;
+      f0 = fmaf(tt, b.y0, f0);
+      f1 = fmaf(tt, b.y1, f1);
+      f2 = fmaf(tt, b.y2, f2);
+      step<IT + 1>();
+    }
+  }
+};
+
 template <int G, bool VERLET, int INFL, bool PIPE>
 __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
                                            const NodeRec* __restrict__ rec,
@@ -580,28 +646,15 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     const double ui0 = me.u0, ui1 = me.u1, ui2 = me.u2;
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (PIPE) {
-      const int nmax = wave_trips<G>(deg);
-      Nb2 nb[3];
-      nb[0] = nb2_load(L, lj[0]);
-      nb[1] = nb2_load(L, lj[1]);
       // the slots of the bond after next are read whenever a bond is
       // computed (every lj entry, past a row's end too, is a slot of the
-      // staging arrays), so each bond's math waits only for its own reads
-#pragma unroll
-      for (int it = 0; it < MAXDEG / G; ++it) {
-        if (it < nmax) {
-          if (it + 2 < MAXDEG / G) nb[(it + 2) % 3] = nb2_load(L, lj[it + 2]);
-          const Nb2& cur = nb[it % 3];
-          const unsigned long long okm = intact_lanes<G>(mo, it, l);
-          const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
-                                        cur.u0, cur.u1, cur.u2));
-          float tt = pair_tv<INFL>(b, p_i, d_i, cur.vol, cur.vp, cur.vd, hhat, a.pmb_c);
-          tt = __builtin_amdgcn_inverse_ballot_w64(okm) ? tt : 0.0f;
-          f0 = fmaf(tt, b.y0, f0);
-          f1 = fmaf(tt, b.y1, f1);
-          f2 = fmaf(tt, b.y2, f2);
-        }
-      }
+      // staging arrays)
+      P2Loop<G, INFL> lp{L, lj, mo, l, wave_trips<G>(deg), xi0, xi1, xi2, ui0, ui1, ui2,
+                         p_i, d_i, hhat, a.pmb_c, {}};
+      lp.nb[0] = nb2_load(L, lj[0]);
+      lp.nb[1] = nb2_load(L, lj[1]);
+      lp.template step<0>();
+      f0 = lp.f0; f1 = lp.f1; f2 = lp.f2;
     } else {
 #pragma unroll
     for (int it = 0; it < MAXDEG / G; ++it) {
