@@ -332,6 +332,15 @@ __device__ __forceinline__ void row_load(const BondArgs& a, const TileArgs& t, i
   row_arrays<G>(a, t, l, valid, r);
 }
 
+// Whether one round of node groups covers a tile. A tile holds at least one
+// node and at most kTileNodesMax (the tile builder's invariant), so the
+// node-group loop then runs exactly once with no guard, and the row loads
+// are not duplicated for later rounds.
+template <int G>
+__device__ __forceinline__ constexpr bool one_round() {
+  return WG / G >= kTileNodesMax;
+}
+
 // Bond-loop trip count of a wave: inside a tile nodes are ordered by
 // descending degree (checked when the tiles are built), so the wave's first
 // node group has the largest degree.
@@ -342,23 +351,41 @@ __device__ __forceinline__ int wave_trips(int deg) {
 
 // Lane masks of the wave (compare results kept in scalar registers):
 // predicate codes of the compare builtins.
-enum { CMP_FOGT = 2, CMP_FOLE = 5, CMP_IEQ = 32 };
+enum { CMP_FOGT = 2, CMP_FOLE = 5, CMP_INE = 33 };
 
-// Lanes whose bond k = it * G + l exists and is not marked broken in m (the
-// words carry set bits past the row's end).
+// A lane's intact bits: the complement of the mask words m (which carry set
+// bits past the row's end) shifted down by the lane's l, as 32-bit halves.
+// Bond k = it * G + l then sits at bit (it * G) & 31 of half (it * G) >> 5:
+// a compile-time position, so testing it takes no per-lane shift.
 template <int G>
-__device__ __forceinline__ unsigned long long intact_lanes(const unsigned long long (&m)[NWORDS],
-                                                           int it, int l) {
-  const int base = it * G;
-  unsigned v;
-  if (G == 64) {
-    v = (unsigned)((m[base >> 6] >> l) & 1ull);
-  } else {
-    const unsigned long long w = m[base >> 6];
-    const unsigned half = (base & 32) ? (unsigned)(w >> 32) : (unsigned)w;
-    v = half & (1u << ((base & 31) + l));
+__device__ __forceinline__ void lane_intact(const unsigned long long (&m)[NWORDS], int l,
+                                            unsigned (&ih)[2 * NWORDS]) {
+#pragma unroll
+  for (int w = 0; w < NWORDS; w++) {
+    const unsigned long long v = ~m[w] >> l;
+    ih[2 * w] = (unsigned)v;
+    ih[2 * w + 1] = (unsigned)(v >> 32);
   }
-  return __builtin_amdgcn_uicmp(v, 0u, CMP_IEQ);
+}
+
+// Lanes whose bond it * G + l exists and is not marked broken.
+template <int G>
+__device__ __forceinline__ unsigned long long intact_lanes(const unsigned (&ih)[2 * NWORDS],
+                                                           int it) {
+  const int base = it * G;
+  return __builtin_amdgcn_uicmp(ih[base >> 5] & (1u << (base & 31)), 0u, CMP_INE);
+}
+
+// All ones when the lane's bond it * G + l exists and is not marked broken,
+// zero otherwise: one sign-filled bit-field extract. Written as an asm
+// statement (an ALU instruction, no memory access) because the compiler turns
+// the extract-and-mask into a compare and a select, one instruction more.
+template <int G>
+__device__ __forceinline__ unsigned intact_fill(const unsigned (&ih)[2 * NWORDS], int it) {
+  const int base = it * G;
+  unsigned m;
+  asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(ih[base >> 5]), "i"(base & 31));
+  return m;
 }
 
 // Pass 1 (damage + dilatation) of one tile; every thread of the workgroup
@@ -371,8 +398,8 @@ template <int G, bool DAMAGE, bool NEAR, int INFL>
 struct P1Loop {
   const TileLds& L;
   const int (&lj)[MAXDEG / G];
-  const unsigned long long (&mo)[NWORDS];
-  int l, nmax;
+  const unsigned (&ih)[2 * NWORDS];
+  int nmax;
   unsigned xi0, xi1, xi2;
   double ui0, ui1, ui2;
   float scz, nlim, hhat;
@@ -389,7 +416,7 @@ struct P1Loop {
       const Nb& cur = nb[IT % 3];
       // lanes whose bond exists and is unbroken; the masks stay in scalar
       // registers, so combining them costs no vector instructions
-      const unsigned long long okm = intact_lanes<G>(mo, IT, l);
+      const unsigned long long okm = intact_lanes<G>(ih, IT);
       const BondS b = bond_stretch(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
                                           cur.u0, cur.u1, cur.u2));
       unsigned long long brkm = 0ull;  // lanes whose bond breaks now
@@ -437,7 +464,7 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
   halo_write<false>(L, hs);
   __syncthreads();
   const int own = t.own_off[tile];
-  for (int base = n0; base < n1; base += NG) {
+  for (int base = n0; one_round<G>() ? base == n0 : base < n1; base += NG) {
     const int node = base + grp;
     const bool valid = node < n1;
     if (base != n0) row_load<G>(a, t, node, valid, n0, l, rd);
@@ -470,7 +497,9 @@ __device__ __forceinline__ void tile_pass1(const BondArgs& a, const TileArgs& t,
       // registers) and are merged into the group's mask words after the loop.
       // the slots of the bond after next are read whenever a bond is
       // computed, as in pass 2
-      P1Loop<G, DAMAGE, NEAR, INFL> lp{L, lj, mo, l, nmax, xi0, xi1, xi2, ui0, ui1, ui2,
+      unsigned ih[2 * NWORDS];
+      lane_intact<G>(mo, l, ih);
+      P1Loop<G, DAMAGE, NEAR, INFL> lp{L, lj, ih, nmax, xi0, xi1, xi2, ui0, ui1, ui2,
                                        scz, nlim, c.hhat, {}, {}};
 #pragma unroll
       for (int it = 0; it < MAXDEG / G; ++it) lp.balv[it] = 0ull;
@@ -564,8 +593,8 @@ template <int G, int INFL>
 struct P2Loop {
   const TileLds& L;
   const int (&lj)[MAXDEG / G];
-  const unsigned long long (&mo)[NWORDS];
-  int l, nmax;
+  const unsigned (&ih)[2 * NWORDS];
+  int nmax;
   unsigned xi0, xi1, xi2;
   double ui0, ui1, ui2;
   float p_i, d_i, hhat, pmb_c;
@@ -578,11 +607,11 @@ struct P2Loop {
       if (IT >= nmax) return;
       if constexpr (IT + 2 < MAXDEG / G) nb[(IT + 2) % 3] = nb2_load(L, lj[IT + 2]);
       const Nb2& cur = nb[IT % 3];
-      const unsigned long long okm = intact_lanes<G>(mo, IT, l);
       const BondK b = bond_k(bond_q(xi0, xi1, xi2, ui0, ui1, ui2, cur.x0, cur.x1, cur.x2,
                                     cur.u0, cur.u1, cur.u2));
-      float tt = pair_tv<INFL>(b, p_i, d_i, cur.vol, cur.vp, cur.vd, hhat, pmb_c);
-      tt = __builtin_amdgcn_inverse_ballot_w64(okm) ? tt : 0.0f;
+      const float tv = pair_tv<INFL>(b, p_i, d_i, cur.vol, cur.vp, cur.vd, hhat, pmb_c);
+      // a bitwise select: a masked-out bond's NaN or infinity becomes +0
+      const float tt = __uint_as_float(__float_as_uint(tv) & intact_fill<G>(ih, IT));
       f0 = fmaf(tt, b.y0, f0);
       f1 = fmaf(tt, b.y1, f1);
       f2 = fmaf(tt, b.y2, f2);
@@ -625,7 +654,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
   halo_write<true>(L, hs);
   __syncthreads();
   const int own = t.own_off[tile];
-  for (int base = n0; base < n1; base += NG) {
+  for (int base = n0; one_round<G>() ? base == n0 : base < n1; base += NG) {
     const int node = base + grp;
     const bool valid = node < n1;
     if (base != n0) row_load<G>(a, t, node, valid, n0, l, rd);
@@ -649,7 +678,9 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
       // the slots of the bond after next are read whenever a bond is
       // computed (every lj entry, past a row's end too, is a slot of the
       // staging arrays)
-      P2Loop<G, INFL> lp{L, lj, mo, l, wave_trips<G>(deg), xi0, xi1, xi2, ui0, ui1, ui2,
+      unsigned ih[2 * NWORDS];
+      lane_intact<G>(mo, l, ih);
+      P2Loop<G, INFL> lp{L, lj, ih, wave_trips<G>(deg), xi0, xi1, xi2, ui0, ui1, ui2,
                          p_i, d_i, hhat, a.pmb_c, {}};
       lp.nb[0] = nb2_load(L, lj[0]);
       lp.nb[1] = nb2_load(L, lj[1]);
