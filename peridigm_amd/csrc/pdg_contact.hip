@@ -14,10 +14,12 @@
 // search radius (2^k buckets, k the smallest with 2^k >= 2N), points sorted by
 // bucket. A point visits each distinct bucket of its 27 surrounding cells
 // once; the distance predicate alone decides membership, as in the
-// reference. Bonded neighbours are found by binary search in the point's bond
-// row, kept sorted in a copy of the bond list. One pass counts every point's
-// contacts and keeps the first 32; rows that long or shorter are copied into
-// the list, and only longer ones are walked a second time.
+// reference. The counting pass tests no bonds per candidate: a point's
+// contact count is its points within the radius less its bonded neighbours
+// within it, both under one predicate, so the difference is exact. Only the
+// points with contacts are walked again to write their rows, finding bonded
+// neighbours by binary search in the point's bond row (kept sorted in a copy
+// of the bond list).
 //
 // Force: the search list is symmetric (j lists i exactly when i lists j: the
 // squared distance and the bond lists are symmetric), and the reference adds
@@ -59,10 +61,9 @@ struct ContactState {
   int* act = nullptr;   // n: points with contacts
   int* nact_d = nullptr;
   int nact = 0;
-  int* keep = nullptr;  // n x kKeep: first contacts of every point (first pass)
-  int* over = nullptr;  // n: points with more than kKeep contacts
+  double* ys = nullptr;  // 3n positions in bucket order (the counting pass)
+  int* over = nullptr;   // n: points with contacts, in any order (the filling pass)
   int* nover_d = nullptr;
-  int nover = 0;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   // drop-in call buffers (reference order)
@@ -127,6 +128,16 @@ __global__ void k_fill_f64(long long n, double* __restrict__ p) {
   if (t < n) p[t] = 0.0;
 }
 
+// Bucket-ordered copy of the positions: a bucket's candidates are then read
+// contiguously.
+__global__ void k_contact_ys(int n, const int* __restrict__ sidx, const double* __restrict__ y,
+                             double* __restrict__ ys) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int q = t / 3, l = t - 3 * q;
+  ys[t] = y[3 * (size_t)sidx[q] + l];
+}
+
 __global__ void k_bucket_bounds(int n, const int* __restrict__ skey, int* __restrict__ bstart,
                                 int* __restrict__ bend) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,77 +147,135 @@ __global__ void k_bucket_bounds(int n, const int* __restrict__ skey, int* __rest
   if (q == n - 1 || skey[q + 1] != b) bend[b] = q + 1;
 }
 
-__device__ __forceinline__ bool bonded(const int* __restrict__ bcol, int lo, int hi, int j) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const int v = bcol[mid];
-    if (v == j) return true;
-    if (v < j) lo = mid + 1;
-    else hi = mid;
-  }
-  return false;
+// The search predicate, |p - P|^2 <= R^2 with the differences taken as
+// p - P: every kernel that decides membership evaluates it this way, so the
+// counting and filling passes agree bit for bit.
+__device__ __forceinline__ bool within(const double* __restrict__ p, double P0, double P1,
+                                       double P2, double R2) {
+  const double d0 = p[0] - P0, d1 = p[1] - P1, d2 = p[2] - P2;
+  return d0 * d0 + d1 * d1 + d2 * d2 <= R2;
 }
 
-constexpr int kKeep = 32;  // contacts per point kept by the first search pass
+// Buckets of the 27 cells around cell (a, b, c), each distinct one once: a
+// bucket that two of the cells share (a hash collision) is replaced by ~0u
+// the second time. No point is then met twice; points of other cells in a
+// visited bucket fail the predicate, being farther than one cell.
+__device__ __forceinline__ void cell_buckets(int a, int b, int c, unsigned mask,
+                                             unsigned (&bk)[27]) {
+#pragma unroll
+  for (int k = 0; k < 27; k++)
+    bk[k] = bucket_of(a + k % 3 - 1, b + (k / 3) % 3 - 1, c + k / 9 - 1, mask);
+#pragma unroll
+  for (int k = 1; k < 27; k++) {
+    bool dup = false;
+#pragma unroll
+    for (int p = 0; p < k; p++) dup |= bk[p] == bk[k];
+    if (dup) bk[k] = ~0u;
+  }
+}
 
-// Contact neighbours of a point: the buckets of the 27 cells around it, each
-// visited once (a bucket that two of the cells share, a hash collision, is
-// skipped the second time, so no point is met twice; points of other cells
-// in a visited bucket fail the distance predicate, being farther than one
-// cell). FILL = false: the first pass over every point, threads taking the
-// points in bucket order so the lanes of a wave mostly share a cell (same
-// candidates, uniform trip counts); counts every contact and keeps the
-// first kKeep in `keep`. FILL = true: the points of `pts` (those with more
-// than kKeep), their whole rows written at off[i], in the same visit order.
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_contact_rows(
-    int n, const int* __restrict__ pts, const double* __restrict__ y, double R2, double inv_cs,
-    unsigned mask, const int* __restrict__ bstart, const int* __restrict__ bend,
-    const int* __restrict__ sidx, const int* __restrict__ brow, const int* __restrict__ bcol,
-    long long* __restrict__ cnt, int* __restrict__ keep, const long long* __restrict__ off,
-    int* __restrict__ list) {
+// Contact count of every point: the points within the search radius in the
+// buckets of its 27 cells, less its bonded neighbours within the radius.
+// Every bonded neighbour within the radius is one of the former, so the
+// count is that of the unbonded ones, with no bond test per candidate.
+// Threads take the points in bucket order (t: the sorted position), so the
+// lanes of a wave mostly share a cell; candidates are read from the
+// bucket-ordered positions ys, the bonded neighbours from y. Points with
+// contacts are appended to `over`.
+__global__ __launch_bounds__(256) void k_contact_count(
+    int n, const double* __restrict__ ys, const int* __restrict__ sidx,
+    const double* __restrict__ y, double R2, double inv_cs, unsigned mask,
+    const int* __restrict__ bstart, const int* __restrict__ bend, const int* __restrict__ brow,
+    const int* __restrict__ bcol, long long* __restrict__ cnt, int* __restrict__ nover,
+    int* __restrict__ over) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int i = pts[t];
-  const double Y0 = y[3 * i], Y1 = y[3 * i + 1], Y2 = y[3 * i + 2];
-  const int a = cell_of(Y0, inv_cs), b = cell_of(Y1, inv_cs), c = cell_of(Y2, inv_cs);
-  const int r0 = brow[i], r1 = brow[i + 1];
-  int* const out = FILL ? list + off[i] : keep + (size_t)i * kKeep;
-  long long count = 0;
+  const int i = sidx[t];
+  const double Y0 = ys[3 * (size_t)t], Y1 = ys[3 * (size_t)t + 1], Y2 = ys[3 * (size_t)t + 2];
+  unsigned bk[27];
+  cell_buckets(cell_of(Y0, inv_cs), cell_of(Y1, inv_cs), cell_of(Y2, inv_cs), mask, bk);
+  // four candidates (bonds) per iteration, so four position reads are in
+  // flight at once; the remainder one at a time
+  int count = 0;
   for (int k = 0; k < 27; k++) {
-    const unsigned bk = bucket_of(a + k % 3 - 1, b + (k / 3) % 3 - 1, c + k / 9 - 1, mask);
-    bool dup = false;
-    for (int p = 0; p < k; p++)
-      dup |= bucket_of(a + p % 3 - 1, b + (p / 3) % 3 - 1, c + p / 9 - 1, mask) == bk;
-    if (dup) continue;
-    const int s = bstart[bk], e = bend[bk];
-    for (int q = s; q < e; q++) {
-      const int j = sidx[q];
-      const double d0 = y[3 * j] - Y0, d1 = y[3 * j + 1] - Y1, d2 = y[3 * j + 2] - Y2;
-      if (!(d0 * d0 + d1 * d1 + d2 * d2 <= R2) || j == i) continue;
-      if (bonded(bcol, r0, r1, j)) continue;
-      if (FILL || count < kKeep) out[count] = j;
-      count++;
+    if (bk[k] == ~0u) continue;
+    const int s = bstart[bk[k]], e = bend[bk[k]];
+    int q = s;
+    for (; q + 4 <= e; q += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        count += (q + u != t && within(ys + 3 * (size_t)(q + u), Y0, Y1, Y2, R2)) ? 1 : 0;
     }
+    for (; q < e; q++) count += (q != t && within(ys + 3 * (size_t)q, Y0, Y1, Y2, R2)) ? 1 : 0;
   }
-  if (!FILL) cnt[i] = count;
+  const int r0 = brow[i], r1 = brow[i + 1];
+  int b = r0;
+  for (; b + 4 <= r1; b += 4) {
+    int jj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) jj[u] = bcol[b + u];
+#pragma unroll
+    for (int u = 0; u < 4; u++) count -= within(y + 3 * (size_t)jj[u], Y0, Y1, Y2, R2) ? 1 : 0;
+  }
+  for (; b < r1; b++) count -= within(y + 3 * (size_t)bcol[b], Y0, Y1, Y2, R2) ? 1 : 0;
+  cnt[i] = count;
+  if (count > 0) over[atomicAdd(nover, 1)] = i;
 }
 
-// Rows of at most kKeep contacts copied from `keep` into the list; the
-// points with more appended to `over` for the second pass.
-__global__ void k_contact_copy(int n, const long long* __restrict__ cnt,
-                               const long long* __restrict__ off, const int* __restrict__ keep,
-                               int* __restrict__ list, int* __restrict__ nover,
-                               int* __restrict__ over) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long m = cnt[i];
-  if (m > kKeep) {
-    over[atomicAdd(nover, 1)] = i;
-    return;
+constexpr int kFillGroup = 8;  // candidates per bond test of the filling pass
+
+// Contact rows of the points of `pts` (those with contacts; their number in
+// *npts), written at off[i] in visit order: the buckets of the 27 cells, each
+// point's within the radius kept unless bonded. Candidates go in groups of
+// kFillGroup: a group with a hit runs one branch-free search of the point's
+// sorted bond row for all its candidates at once (the same row, so one trip
+// count: the last entry <= j, then an equality test), so the wave waits for
+// one chain of dependent row reads per group instead of one per hit.
+__global__ __launch_bounds__(256) void k_contact_rows(
+    const int* __restrict__ npts, const int* __restrict__ pts, const double* __restrict__ y,
+    const double* __restrict__ ys, double R2, double inv_cs, unsigned mask,
+    const int* __restrict__ bstart,
+    const int* __restrict__ bend, const int* __restrict__ sidx, const int* __restrict__ brow,
+    const int* __restrict__ bcol, const long long* __restrict__ off, int* __restrict__ list) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= *npts) return;
+  const int i = pts[t];
+  const double Y0 = y[3 * (size_t)i], Y1 = y[3 * (size_t)i + 1], Y2 = y[3 * (size_t)i + 2];
+  unsigned bk[27];
+  cell_buckets(cell_of(Y0, inv_cs), cell_of(Y1, inv_cs), cell_of(Y2, inv_cs), mask, bk);
+  const int r0 = brow[i], r1 = brow[i + 1];
+  int* const out = list + off[i];
+  long long count = 0;
+  for (int k = 0; k < 27; k++) {
+    if (bk[k] == ~0u) continue;
+    const int s = bstart[bk[k]], e = bend[bk[k]];
+    for (int q0 = s; q0 < e; q0 += kFillGroup) {
+      int jj[kFillGroup];
+      bool hit[kFillGroup];
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < kFillGroup; u++) {
+        const int q = min(q0 + u, e - 1);  // past the bucket: a valid index, no hit
+        jj[u] = sidx[q];
+        hit[u] = q0 + u < e && jj[u] != i && within(ys + 3 * (size_t)q, Y0, Y1, Y2, R2);
+        any |= hit[u];
+      }
+      if (!any) continue;
+      int pos[kFillGroup];
+#pragma unroll
+      for (int u = 0; u < kFillGroup; u++) pos[u] = r0;
+      for (int m = r1 - r0; m > 1;) {
+        const int half = m >> 1;
+#pragma unroll
+        for (int u = 0; u < kFillGroup; u++)
+          pos[u] = bcol[pos[u] + half] <= jj[u] ? pos[u] + half : pos[u];
+        m -= half;
+      }
+#pragma unroll
+      for (int u = 0; u < kFillGroup; u++)
+        if (hit[u] && !(r1 > r0 && bcol[pos[u]] == jj[u])) out[count++] = jj[u];
+    }
   }
-  const long long o = off[i];
-  for (int k = 0; k < (int)m; k++) list[o + k] = keep[(size_t)i * kKeep + k];
 }
 
 // One pair of the short-range force model seen from `cur` (position X,
@@ -393,10 +462,13 @@ int search(Ctx* c, bool run) {
   hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(T)), dim3(256), 0, s, T, S->bstart, 0);
   hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(T)), dim3(256), 0, s, T, S->bend, 0);
   hipLaunchKernelGGL(k_bucket_bounds, dim3(nb), dim3(256), 0, s, N, S->skey, S->bstart, S->bend);
+  hipLaunchKernelGGL(k_contact_ys, dim3(blocks_for(3ll * N)), dim3(256), 0, s, N, S->sidx, S->y,
+                     S->ys);
   const double R2 = S->R * S->R;
-  hipLaunchKernelGGL((k_contact_rows<false>), dim3(nb), dim3(256), 0, s, N, S->sidx, S->y, R2,
-                     S->inv_cs, mask, S->bstart, S->bend, S->sidx, c->d_row, S->bcol, S->cnt,
-                     S->keep, nullptr, nullptr);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, 1ll, S->nover_d, 0);
+  hipLaunchKernelGGL(k_contact_count, dim3(nb), dim3(256), 0, s, N, S->ys, S->sidx, S->y, R2,
+                     S->inv_cs, mask, S->bstart, S->bend, c->d_row, S->bcol, S->cnt, S->nover_d,
+                     S->over);
   rc = c->hip(hipGetLastError(), "contact count");
   bytes = 0;
   if (!rc) rc = c->hip(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->cnt, S->off, N + 1, s),
@@ -419,20 +491,14 @@ int search(Ctx* c, bool run) {
   if (run) S->nact = nact;
   if (!rc) rc = grow(c, S->list, S->cap, total, "alloc contact list");
   if (rc) return rc;
-  // rows of at most kKeep from the first pass; the longer ones walked again
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, 1ll, S->nover_d, 0);
-  hipLaunchKernelGGL(k_contact_copy, dim3(nb), dim3(256), 0, s, N, S->cnt, S->off, S->keep,
-                     S->list, S->nover_d, S->over);
-  int nover = 0;
-  rc = c->hip(hipMemcpyAsync(&nover, S->nover_d, 4, hipMemcpyDeviceToHost, s), "d2h overflow");
-  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
-  if (!rc && nover > 0)
-    hipLaunchKernelGGL((k_contact_rows<true>), dim3(blocks_for(nover)), dim3(256), 0, s, nover,
-                       S->over, S->y, R2, S->inv_cs, mask, S->bstart, S->bend, S->sidx, c->d_row,
-                       S->bcol, nullptr, nullptr, S->off, S->list);
-  if (!rc) rc = c->hip(hipGetLastError(), "contact fill");
+  // the rows of the points with contacts (their number stays on the device:
+  // threads past it return at once)
+  if (total > 0)
+    hipLaunchKernelGGL(k_contact_rows, dim3(nb), dim3(256), 0, s, S->nover_d, S->over, S->y,
+                       S->ys, R2, S->inv_cs, mask, S->bstart, S->bend, S->sidx, c->d_row, S->bcol, S->off,
+                       S->list);
+  rc = c->hip(hipGetLastError(), "contact fill");
   S->size = total;
-  S->nover = nover;
   S->built = run;
   return rc;
 }
@@ -444,7 +510,7 @@ void Ctx::free_contact() {
   ContactState* S = contact;
   void* ptrs[] = {S->y, S->fc, S->bcol, S->key, S->idx, S->skey, S->sidx, S->bstart,
                   S->bend, S->cnt, S->off, S->list, S->tmp, S->vref, S->dv, S->dfc,
-                  S->dhead, S->dlist, S->act, S->nact_d, S->keep, S->over, S->nover_d};
+                  S->dhead, S->dlist, S->act, S->nact_d, S->ys, S->over, S->nover_d};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   delete S;
   contact = nullptr;
@@ -543,9 +609,9 @@ int pdg_set_contact(pdg_ctx* h, double searchRadius, int searchFrequency, double
   if (!rc) rc = A(S->vref, N, "alloc contact volumes");
   if (!rc) rc = A(S->act, N, "alloc contact active list");
   if (!rc) rc = A(S->nact_d, 1, "alloc contact active count");
-  if (!rc) rc = A(S->keep, (long long)N * kKeep, "alloc contact rows");
-  if (!rc) rc = A(S->over, N, "alloc contact overflow list");
-  if (!rc) rc = A(S->nover_d, 1, "alloc contact overflow count");
+  if (!rc) rc = A(S->ys, 3ll * N, "alloc contact sorted positions");
+  if (!rc) rc = A(S->over, N, "alloc contact point list");
+  if (!rc) rc = A(S->nover_d, 1, "alloc contact point count");
   if (rc) { c->free_contact(); return rc; }
   // bond rows (internal ids) sorted ascending
   std::vector<int> bc(c->col.begin(), c->col.end());
