@@ -208,7 +208,8 @@ def copy_bandwidth(mib=1024, reps=10):
 def qs_figure(nsamples=100, epochs=20, large=False):
     """Quasi-static tensile bar 60x20x20 (24k nodes; 125x40x40, 200k nodes,
     with `large`), 10 load steps to 1e-3 strain, every linear solve to 1e-8
-    relative residual: wall time with block-Jacobi CG and with the learned
+    relative residual: wall time with block-Jacobi CG, with block-Jacobi CG
+    plus the coarse-grid correction (two-level), and with the learned
     preconditioner in flexible CG (trained on the device on
     residual/correction pairs from `nsamples` block-Jacobi solves under random
     end loads, two intermediate iterates each)."""
@@ -227,6 +228,8 @@ def qs_figure(nsamples=100, epochs=20, large=False):
             print("bench.py: QS N=%d: %s" % (N, msg), file=sys.stderr, flush=True)
         u_j, st_j, wall_j = Q.run_tension(ctx, p, precond="block_jacobi", tol=1e-6, lin_tol=1e-8)
         note("block-Jacobi run %.2f s" % wall_j)
+        u_t, st_t, wall_t = Q.run_tension(ctx, p, precond="two_level", tol=1e-6, lin_tol=1e-8)
+        note("two-level run %.2f s" % wall_t)
         t0 = time.perf_counter()
         R, E, it_gen = Lr.generate_pairs(ctx, p, nsamples=nsamples, seed=11)
         t_gen = time.perf_counter() - t0
@@ -247,6 +250,9 @@ def qs_figure(nsamples=100, epochs=20, large=False):
         "learned_linear_iters": int(sum(s["linear"] for s in st_l)),
         "learned_fallbacks_to_jacobi": fallbacks,
         "newton_iters": [int(sum(s["newton"] for s in st_j)), int(sum(s["newton"] for s in st_l))],
+        "two_level_pcg_s": wall_t, "two_level_speedup_vs_jacobi": wall_j / wall_t,
+        "two_level_linear_iters": int(sum(s["linear"] for s in st_t)),
+        "two_level_max_u_diff_rel": float(np.abs(u_t - u_j).max() / np.abs(u_j).max()),
         "max_u_diff_rel": float(np.abs(u_l - u_j).max() / np.abs(u_j).max()),
         "training": {"load_samples": nsamples, "pairs": int(len(R)), "epochs": epochs, "final_rel_loss": loss,
                      "data_s": t_gen, "train_s": t_train},

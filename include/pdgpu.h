@@ -258,6 +258,8 @@ int pdg_near_critical_count(const pdg_ctx* ctx, long long* count);
 #define PDG_PRECOND_NONE 0
 #define PDG_PRECOND_BLOCK_JACOBI 1 /* 3x3 diagonal blocks, :1603-1647 */
 #define PDG_PRECOND_LEARNED 2      /* message-passing network, flexible CG */
+#define PDG_PRECOND_TWO_LEVEL 3    /* block Jacobi + Galerkin coarse correction on rigid-body
+                                      modes of node aggregates (no reference counterpart) */
 int pdg_qs_set_fixed(pdg_ctx* ctx, const int* dofs, int numDofs);
 /* r = f(u) V with kinematic rows zeroed; *norm = ||r||_2 + 20 ||r||_inf (:4316). */
 int pdg_qs_residual(pdg_ctx* ctx, const double* u, double* residual, double* norm);

@@ -30,6 +30,15 @@
 
 namespace pdg {
 
+// Regular coarse grid of the two-level preconditioner: d[k] points along axis
+// k from o[k] with spacing H; a position's cell is clamped to the grid.
+struct CoarseGrid {
+  int d[3] = {2, 2, 2};
+  double o[3] = {0.0, 0.0, 0.0};
+  double H = 1.0;
+  int points() const { return d[0] * d[1] * d[2]; }
+};
+
 struct QsState {
   int n = 0;
   double *x = nullptr, *u = nullptr, *w = nullptr, *vol = nullptr, *m = nullptr, *alpha = nullptr,
@@ -68,6 +77,21 @@ struct QsState {
   // displacement at the load step's start, Newton increment, predictor,
   // line-search base state and residual
   double* rv[5] = {};
+  // two-level preconditioner: block Jacobi plus a Galerkin coarse correction
+  // on a regular coarse grid over the bounding box (three translations per
+  // grid point, weighted by the point's trilinear hat function); built at
+  // the first two-level solve after the prescribed DOFs are set or a run
+  // starts, and kept
+  CoarseGrid cgr;
+  int nc = 0;
+  bool coarse_ok = false;
+  int* cstart = nullptr;  // grid points + 1: each point's nodes in clist
+  int* clist = nullptr;   // 8n: nodes of the cells around each grid point
+  double* wlist = nullptr;  // 8n: the grid point's weight at each entry of clist
+  double* cpart = nullptr;  // 3 kCoarseSplit points: restriction partials
+  double* cinv = nullptr;    // nc x nc inverse of P^T K P (dropped functions: zero)
+  double* cmat = nullptr;    // nc x nc build buffer
+  double* cy = nullptr;      // nc: coarse solution
 };
 
 namespace {
@@ -594,6 +618,174 @@ __global__ __launch_bounds__(RB) void k_lin2(int n, double a, const double* x, d
 }
 
 // v = 0 on the kinematic DOFs
+// Two-level preconditioner kernels. Coarse function 3g + k is the trilinear
+// hat function of grid point g times the unit vector along axis k; it is
+// zero on prescribed DOFs.
+constexpr int kCoarsePointsMax = 64;  // grid points at most (192 coarse functions)
+
+// Cell of a position (lower corner, clamped so the cell lies inside the
+// grid) and the position's local coordinates in it.
+__device__ __forceinline__ void coarse_cell(const CoarseGrid& G, const double* p, int c[3],
+                                            double t[3]) {
+  for (int k = 0; k < 3; k++) {
+    const double f = (p[k] - G.o[k]) / G.H;
+    int ci = (int)floor(f);
+    ci = ci < 0 ? 0 : (ci > G.d[k] - 2 ? G.d[k] - 2 : ci);
+    t[k] = f - ci;
+    c[k] = ci;
+  }
+}
+
+// Weight of corner q (bits: x, y, z offsets) of the cell at local t
+__device__ __forceinline__ double corner_w(int q, const double t[3]) {
+  return ((q & 1) ? t[0] : 1.0 - t[0]) * ((q & 2) ? t[1] : 1.0 - t[1]) *
+         ((q & 4) ? t[2] : 1.0 - t[2]);
+}
+
+__device__ __forceinline__ int corner_id(const CoarseGrid& G, const int c[3], int q) {
+  return (c[0] + (q & 1)) + G.d[0] * ((c[1] + ((q >> 1) & 1)) + G.d[1] * (c[2] + ((q >> 2) & 1)));
+}
+
+// Weight of grid point g at node i (0 when g is not a corner of i's cell)
+__device__ __forceinline__ double point_w(const CoarseGrid& G, const double* p, int g) {
+  int c[3];
+  double t[3];
+  coarse_cell(G, p, c, t);
+  double w = 0.0;
+  for (int q = 0; q < 8; q++) w += corner_id(G, c, q) == g ? corner_w(q, t) : 0.0;
+  return w;
+}
+
+// Weights of the restriction lists: wlist[e] = weight of grid point pt at
+// node clist[e], for the entries e of pt's list (one block per point, once
+// per set-up)
+__global__ __launch_bounds__(RB) void k_coarse_weights(CoarseGrid G, const int* __restrict__ start,
+                                                       const int* __restrict__ list,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ wlist) {
+  const int pt = blockIdx.x;
+  for (int e = start[pt] + threadIdx.x; e < start[pt + 1]; e += RB)
+    wlist[e] = point_w(G, x + 3 * (size_t)list[e], pt);
+}
+
+constexpr int kCoarseSplit = 8;  // blocks per grid point in the restriction
+
+// Restriction partials: block (pt, sp) sums phi_(pt,k) . r over the sp-th
+// eighth of pt's list into part[3 (pt kCoarseSplit + sp) + k]
+__global__ __launch_bounds__(RB) void k_coarse_partial(const int* __restrict__ start,
+                                                       const int* __restrict__ list,
+                                                       const double* __restrict__ wlist,
+                                                       const signed char* __restrict__ fixed,
+                                                       const double* __restrict__ r,
+                                                       double* __restrict__ part) {
+  __shared__ double sh[RB / 64];
+  const int pt = blockIdx.x / kCoarseSplit, sp = blockIdx.x - kCoarseSplit * pt;
+  const int e0 = start[pt], len = start[pt + 1] - e0;
+  const int b0 = e0 + (int)((long long)len * sp / kCoarseSplit);
+  const int b1 = e0 + (int)((long long)len * (sp + 1) / kCoarseSplit);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int e = b0 + threadIdx.x; e < b1; e += RB) {
+    const int i = list[e];
+    const double w = wlist[e];
+    for (int k = 0; k < 3; k++) acc[k] += fixed[3 * i + k] ? 0.0 : w * r[3 * i + k];
+  }
+  for (int k = 0; k < 3; k++) {
+    const double t = block_sum(acc[k], sh);
+    if (threadIdx.x == 0) part[3 * (size_t)blockIdx.x + k] = t;
+  }
+}
+
+// g = P^T r from the partials (fixed order), then, when SOLVE, y = Cinv g:
+// every block forms g in LDS and its waves take rows (lanes over a row);
+// without SOLVE block 0 writes g to `out` (a column of the coarse matrix at
+// set-up)
+template <bool SOLVE>
+__global__ __launch_bounds__(RB) void k_coarse_finish(int nc, const double* __restrict__ part,
+                                                      const double* __restrict__ cinv,
+                                                      double* __restrict__ out) {
+  __shared__ double gl[3 * kCoarsePointsMax];
+  if (!SOLVE && blockIdx.x > 0) return;
+  for (int t = threadIdx.x; t < nc; t += RB) {
+    const int pt = t / 3, k = t - 3 * pt;
+    double v = 0.0;
+    for (int sp = 0; sp < kCoarseSplit; sp++) v += part[3 * (size_t)(pt * kCoarseSplit + sp) + k];
+    if (SOLVE) gl[t] = v;
+    else out[t] = v;
+  }
+  if (!SOLVE) return;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int row = blockIdx.x * (RB / 64) + (threadIdx.x >> 6); row < nc; row += gridDim.x * (RB / 64)) {
+    const double* c = cinv + (size_t)row * nc;
+    double v = 0.0;
+    for (int k = lane; k < nc; k += 64) v += c[k] * gl[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) out[row] = v;
+  }
+}
+
+// The coarse function `col` as a full vector
+__global__ __launch_bounds__(RB) void k_coarse_basis(int n, int col, CoarseGrid G,
+                                                     const double* __restrict__ x,
+                                                     const signed char* __restrict__ fixed,
+                                                     double* __restrict__ v) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const int pt = col / 3, k0 = col - 3 * pt;
+  const double w = point_w(G, x + 3 * (size_t)i, pt);
+  for (int k = 0; k < 3; k++) v[3 * i + k] = (k == k0 && !fixed[3 * i + k]) ? w : 0.0;
+}
+
+// Linear index of each node's cell (the lists of the restriction are built
+// from it on the host, so they follow the device's cell assignment)
+__global__ __launch_bounds__(RB) void k_coarse_cells(int n, CoarseGrid G, const double* __restrict__ x,
+                                                     int* __restrict__ cell) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+  double t[3];
+  coarse_cell(G, x + 3 * (size_t)i, c, t);
+  cell[i] = c[0] + G.d[0] * (c[1] + G.d[1] * c[2]);
+}
+
+// z = Dinv r + P y (the two-level preconditioner), with the partial sums of
+// r.z when DOT (grid-stride over nodes, one partial per block, like k_dot)
+template <bool DOT>
+__global__ __launch_bounds__(RB) void k_two_level_apply(int n, const double* __restrict__ dinv,
+                                                        CoarseGrid G, const double* __restrict__ x,
+                                                        const signed char* __restrict__ fixed,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ r,
+                                                        double* __restrict__ z,
+                                                        double* __restrict__ part) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+    const double* D = dinv + 9 * i;
+    const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+    int c[3];
+    double t[3];
+    coarse_cell(G, x + 3 * (size_t)i, c, t);
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < 8; q++) {
+      const double w = corner_w(q, t);
+      const double* yq = y + 3 * corner_id(G, c, q);
+      for (int k = 0; k < 3; k++) p[k] += w * yq[k];
+    }
+    const double z0 = D[0] * r0 + D[1] * r1 + D[2] * r2 + (fixed[3 * i] ? 0.0 : p[0]);
+    const double z1 = D[3] * r0 + D[4] * r1 + D[5] * r2 + (fixed[3 * i + 1] ? 0.0 : p[1]);
+    const double z2 = D[6] * r0 + D[7] * r1 + D[8] * r2 + (fixed[3 * i + 2] ? 0.0 : p[2]);
+    z[3 * i] = z0;
+    z[3 * i + 1] = z1;
+    z[3 * i + 2] = z2;
+    if (DOT) s += r0 * z0 + r1 * z1 + r2 * z2;
+  }
+  if (DOT) {
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
 __global__ __launch_bounds__(RB) void k_zero_fixed(int n, const signed char* fixed, double* v) {
   const int i = blockIdx.x * RB + threadIdx.x;
   if (i < n && fixed[i]) v[i] = 0.0;
@@ -746,6 +938,138 @@ static int block_jacobi(Ctx* c) {
   return c->hip(hipGetLastError(), "qs block jacobi");
 }
 
+// Two-level preconditioner set-up at the current state: the coarse grid
+// (at most kCoarsePointsMax points, spacing equal on all axes, centred on
+// the bounding box), the node lists of the restriction, and the inverse of
+// the coarse matrix A_c = P^T K P, column by column from one tangent apply
+// per coarse function. A_c is inverted on the host by Cholesky
+// factorisation; a function whose pivot falls below 1e-12 of the largest
+// diagonal entry (a grid point whose nodes' DOFs are all prescribed, or none
+// at all) is dropped: its row and column of the inverse are 0.
+static int coarse_build(Ctx* c) {
+  QsState* S = c->qs;
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  auto A = [&](void** p, size_t bytes) { return c->hip(hipMalloc(p, std::max<size_t>(bytes, 64)), "qs coarse alloc"); };
+  int rc = 0;
+  const size_t mc = 3 * (size_t)kCoarsePointsMax;
+  if (!S->clist) {
+    rc = A((void**)&S->clist, 32 * (size_t)N);
+    if (!rc) rc = A((void**)&S->wlist, 64 * (size_t)N);
+    if (!rc) rc = A((void**)&S->cpart, 24 * (size_t)kCoarseSplit * kCoarsePointsMax);
+    if (!rc) rc = A((void**)&S->cstart, 4 * (size_t)(kCoarsePointsMax + 1));
+    if (!rc) rc = A((void**)&S->cinv, 8 * mc * mc);
+    if (!rc) rc = A((void**)&S->cmat, 8 * mc * mc);
+    if (!rc) rc = A((void**)&S->cy, 8 * mc);
+    if (rc) return rc;
+  }
+  // grid over the bounding box
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int k = 0; k < N; k++)
+    for (int d = 0; d < 3; d++) {
+      const double v = c->x[3 * (size_t)k + d];
+      lo[d] = std::min(lo[d], v);
+      hi[d] = std::max(hi[d], v);
+    }
+  double E[3], emax = 0.0;
+  for (int d = 0; d < 3; d++) { E[d] = std::max(hi[d] - lo[d], 0.0); emax = std::max(emax, E[d]); }
+  if (!(emax > 0.0)) emax = 1.0;
+  CoarseGrid G;
+  for (int div = 1; div <= 64; div++) {
+    const double H = emax / div;
+    CoarseGrid T;
+    T.H = H;
+    for (int d = 0; d < 3; d++) T.d[d] = std::max(2, (int)std::ceil(E[d] / H - 1e-9) + 1);
+    if (T.points() > kCoarsePointsMax) break;
+    for (int d = 0; d < 3; d++) T.o[d] = lo[d] - 0.5 * ((T.d[d] - 1) * H - E[d]);
+    G = T;
+  }
+  const int np = G.points(), nc = 3 * np;
+  S->cgr = G;
+  S->nc = nc;
+  // node lists of each grid point (the nodes of the up to 8 cells around it)
+  std::vector<int> cell(N);
+  const unsigned nb = (unsigned)((N + RB - 1) / RB);
+  int* dcell = S->clist;  // scratch until the lists are written
+  hipLaunchKernelGGL(k_coarse_cells, dim3(nb), dim3(RB), 0, s, N, G, S->x, dcell);
+  rc = c->hip(hipMemcpyAsync(cell.data(), dcell, 4 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (rc) return rc;
+  std::vector<int> cnt(np + 1, 0), list(8 * (size_t)N);
+  auto corner = [&](int cl, int q) {
+    const int cx = cl % G.d[0], cy = (cl / G.d[0]) % G.d[1], cz = cl / (G.d[0] * G.d[1]);
+    return (cx + (q & 1)) + G.d[0] * ((cy + ((q >> 1) & 1)) + G.d[1] * (cz + ((q >> 2) & 1)));
+  };
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < 8; q++) cnt[corner(cell[i], q) + 1]++;
+  for (int g = 0; g < np; g++) cnt[g + 1] += cnt[g];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < 8; q++) list[fill[corner(cell[i], q)]++] = i;
+  rc = c->hip(hipMemcpyAsync(S->cstart, cnt.data(), 4 * (size_t)(np + 1), hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipMemcpyAsync(S->clist, list.data(), 32 * (size_t)N, hipMemcpyHostToDevice, s), "h2d");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_coarse_weights, dim3(np), dim3(RB), 0, s, G, S->cstart, S->clist, S->x,
+                     S->wlist);
+  // column col of A_c: P^T K phi_col (S->z holds phi_col, S->q its image)
+  for (int col = 0; col < nc; col++) {
+    hipLaunchKernelGGL(k_coarse_basis, dim3(nb), dim3(RB), 0, s, N, col, G, S->x, S->fixed, S->z);
+    if ((rc = tangent(c, S->z, S->q, true))) return rc;
+    hipLaunchKernelGGL(k_coarse_partial, dim3(np * kCoarseSplit), dim3(RB), 0, s, S->cstart,
+                       S->clist, S->wlist, S->fixed, S->q, S->cpart);
+    hipLaunchKernelGGL(k_coarse_finish<false>, dim3(1), dim3(RB), 0, s, nc, S->cpart, nullptr,
+                       S->cmat + (size_t)col * nc);
+  }
+  std::vector<double> M((size_t)nc * nc);
+  rc = c->hip(hipGetLastError(), "qs coarse build");
+  if (!rc) rc = c->hip(hipMemcpyAsync(M.data(), S->cmat, 8 * M.size(), hipMemcpyDeviceToHost, s), "d2h");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  if (rc) return rc;
+  // symmetrise, factor L L^T with dropped functions, invert
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < i; j++) {
+      const double v = 0.5 * (M[(size_t)i * nc + j] + M[(size_t)j * nc + i]);
+      M[(size_t)i * nc + j] = M[(size_t)j * nc + i] = v;
+    }
+  double dmax = 0.0;
+  for (int i = 0; i < nc; i++) dmax = std::max(dmax, M[(size_t)i * nc + i]);
+  std::vector<double> L((size_t)nc * nc, 0.0);
+  std::vector<char> drop(nc, 0);
+  for (int j = 0; j < nc; j++) {
+    double d = M[(size_t)j * nc + j];
+    for (int k = 0; k < j; k++) d -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
+    if (!(d > 1e-12 * dmax)) { drop[j] = 1; continue; }
+    const double ljj = std::sqrt(d);
+    L[(size_t)j * nc + j] = ljj;
+    for (int i = j + 1; i < nc; i++) {
+      double v = M[(size_t)i * nc + j];
+      for (int k = 0; k < j; k++) v -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
+      L[(size_t)i * nc + j] = v / ljj;
+    }
+  }
+  std::vector<double> inv((size_t)nc * nc, 0.0), y(nc);
+  for (int e = 0; e < nc; e++) {
+    if (drop[e]) continue;
+    for (int i = 0; i < nc; i++) {  // L y = e_e
+      if (drop[i]) { y[i] = 0.0; continue; }
+      double v = i == e ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) v -= L[(size_t)i * nc + k] * y[k];
+      y[i] = v / L[(size_t)i * nc + i];
+    }
+    for (int i = nc - 1; i >= 0; i--) {  // L^T x = y
+      if (drop[i]) { y[i] = 0.0; continue; }
+      double v = y[i];
+      for (int k = i + 1; k < nc; k++) v -= L[(size_t)k * nc + i] * y[k];
+      y[i] = v / L[(size_t)i * nc + i];
+    }
+    for (int i = 0; i < nc; i++) inv[(size_t)i * nc + e] = y[i];
+  }
+  rc = c->hip(hipMemcpyAsync(S->cinv, inv.data(), 8 * inv.size(), hipMemcpyHostToDevice, s), "h2d");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
+  S->coarse_ok = rc == 0;
+  return rc;
+}
+
 static int dot(Ctx* c, const double* x, const double* y, double* out) {
   QsState* S = c->qs;
   const int n = 3 * c->N;
@@ -834,6 +1158,10 @@ void Ctx::free_extra() {
                       S->dampv, S->dT,   S->ps};
     for (double* p : ptrs) if (p) (void)hipFree(p);
     for (double* p : S->rv) if (p) (void)hipFree(p);
+    double* cptrs[] = {S->cinv, S->cmat, S->cy, S->wlist, S->cpart};
+    for (double* p : cptrs) if (p) (void)hipFree(p);
+    if (S->clist) (void)hipFree(S->clist);
+    if (S->cstart) (void)hipFree(S->cstart);
     for (int k = 0; k < 4; k++) {
       if (S->ph[k]) (void)hipFree(S->ph[k]);
       if (S->qh[k]) (void)hipFree(S->qh[k]);
@@ -850,7 +1178,8 @@ void Ctx::free_extra() {
 
 // K x = b on device vectors in internal order; x holds the initial guess.
 // Returns iterations in *iters, the final relative residual in *rel.
-// Preconditioned CG (precond 0: none, 1: block Jacobi) or, for the learned
+// Preconditioned CG (precond 0: none, 1: block Jacobi, 3: block Jacobi plus
+// the coarse correction, built by coarse_build) or, for the learned
 // preconditioner (precond 2), Notay's flexible CG truncated to the last
 // kFcgSlots - 1 directions: the new direction is z minus its K-projections on
 // them, alpha = p.r / p.Kp. Scalars stay on the device; the host reads the
@@ -880,8 +1209,22 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     hipLaunchKernelGGL(k_dot, red, blk, 0, s, n3, u, v, S->part);
     hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, S->part, 1, S->scal + k);
   };
+  // coarse part of the two-level preconditioner: y = A_c^-1 P^T r
+  auto coarse = [&](const double* r) -> int {
+    hipLaunchKernelGGL(k_coarse_partial, dim3(S->cgr.points() * kCoarseSplit), blk, 0, s,
+                       S->cstart, S->clist, S->wlist, S->fixed, r, S->cpart);
+    hipLaunchKernelGGL(k_coarse_finish<true>, dim3((S->nc + RB / 64 - 1) / (RB / 64)), blk, 0, s,
+                       S->nc, S->cpart, S->cinv, S->cy);
+    return c->hip(hipGetLastError(), "coarse solve");
+  };
   auto precond_apply = [&](const double* r, double* z) -> int {
     if (precond == 2) return gnn_apply(c, r, z);
+    if (precond == 3) {
+      int r2 = coarse(r);
+      if (!r2) hipLaunchKernelGGL((k_two_level_apply<false>), dim3(nb), blk, 0, s, c->N, S->dinv,
+                                  S->cgr, S->x, S->fixed, S->cy, r, z, nullptr);
+      return r2 ? r2 : c->hip(hipGetLastError(), "two-level apply");
+    }
     if (precond == 1) {
       hipLaunchKernelGGL(k_block_apply, dim3(nb), blk, 0, s, c->N, S->dinv, r, z);
       return c->hip(hipGetLastError(), "block apply");
@@ -944,6 +1287,10 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     } else if (precond == 1) {
       // z = Dinv r with the r.z partials in the same pass
       hipLaunchKernelGGL(k_block_apply_dot, red, blk, 0, s, c->N, S->dinv, S->r, S->z, S->part);
+    } else if (precond == 3) {
+      r2 = coarse(S->r);
+      if (!r2) hipLaunchKernelGGL((k_two_level_apply<true>), red, blk, 0, s, c->N, S->dinv, S->cgr,
+                                  S->x, S->fixed, S->cy, S->r, S->z, S->part);
     } else {
       r2 = precond_apply(S->r, S->z);
       if (!r2) hipLaunchKernelGGL(k_dot, red, blk, 0, s, n3, S->r, S->z, S->part);
@@ -967,14 +1314,14 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     return c->hip(hipGetLastError(), "cg direction");
   };
   // the damped tangent adds a launch: no graphs while damping
-  const bool graphs = S->graphs && !S->damped;
+  const bool graphs = S->graphs && !S->damped && precond < 3;
   // graphs captured for other vectors or older network buffers are stale
   if (graphs && (S->gx != x || S->gb != b || (flexible && S->gepoch != c->gnn_epoch))) {
     qs::drop_graphs(c);
     S->gx = x; S->gb = b; S->gepoch = c->gnn_epoch;
   }
-  hipGraphExec_t* ga = S->gexec[precond][0];
-  hipGraphExec_t* gb = S->gexec[precond][1];
+  hipGraphExec_t* ga = S->gexec[std::min(precond, 2)][0];
+  hipGraphExec_t* gb = S->gexec[std::min(precond, 2)][1];
   auto capture = [&](hipGraphExec_t* out, auto&& body) -> int {
     hipGraph_t g = nullptr;
     int r2 = c->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -1059,6 +1406,7 @@ int pdg_qs_set_fixed(pdg_ctx* h, const int* dofs, int numDofs) {
   }
   rc = c->hip(hipMemcpyAsync(S->fixed, S->hfixed.data(), S->hfixed.size(), hipMemcpyHostToDevice, c->stream), "h2d");
   if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  S->coarse_ok = false;  // the coarse functions vanish on prescribed DOFs
   return rc;
 }
 
@@ -1124,11 +1472,12 @@ int pdg_qs_block_diagonal(pdg_ctx* h, const double* u, double* blocksInv) {
 int pdg_qs_solve(pdg_ctx* h, const double* u, const double* rhs, double* x, int precond,
                  double relTol, int maxIter, int* iters, double* relRes) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !u || !rhs || !x || precond < 0 || precond > 2) return PDG_E_ARG;
+  if (!c || !c->have_consts || !u || !rhs || !x || precond < 0 || precond > 3) return PDG_E_ARG;
   int rc = qs::ensure(c);
   QsState* S = c->qs;
   if (!rc) rc = qs::to_dev(c, u, S->u);
   if (!rc && precond >= 1) rc = qs::block_jacobi(c);
+  if (!rc && precond == 3 && !S->coarse_ok) rc = qs::coarse_build(c);
   if (!rc) rc = qs::to_dev(c, rhs, S->b);
   if (!rc) rc = qs::to_dev(c, x, S->xs);
   if (rc) return rc;
@@ -1151,7 +1500,7 @@ int pdg_qs_solve(pdg_ctx* h, const double* u, const double* rhs, double* x, int 
 int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond, double linTol,
                   int maxLin, int* newtonIters, int* linIters, double* residualNorm) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  if (!c || !c->have_consts || !u || precond < 0 || precond > 2) return PDG_E_ARG;
+  if (!c || !c->have_consts || !u || precond < 0 || precond > 3) return PDG_E_ARG;
   int rc = qs::ensure(c);
   QsState* S = c->qs;
   if (!rc) rc = qs::to_dev(c, u, S->u);
@@ -1188,8 +1537,9 @@ int pdg_qs_newton(pdg_ctx* h, double* u, double tol, int maxNewton, int precond,
     rnorm = std::sqrt(n2) + 20.0 * ninf;
     if (rnorm <= tol * multiplier) { converged = true; break; }
     if (nit == maxNewton) break;
-    if (precond == 1 && (rc = qs::block_jacobi(c))) return rc;
-    if (precond == 2 && (rc = qs::block_jacobi(c))) return rc;  // learned preconditioner uses D too
+    // the learned and two-level preconditioners use the diagonal blocks too
+    if (precond >= 1 && (rc = qs::block_jacobi(c))) return rc;
+    if (precond == 3 && !S->coarse_ok && (rc = qs::coarse_build(c))) return rc;
     rc = c->hip(hipMemcpyAsync(S->b, S->out, 8 * (size_t)n3, hipMemcpyDeviceToDevice, c->stream), "copy");
     if (!rc) rc = c->hip(hipMemsetAsync(S->xs, 0, 8 * (size_t)n3, c->stream), "zero");
     if (rc) return rc;
@@ -1461,7 +1811,7 @@ int pdg_qs_run_record(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* t
                       int maxSteps, int* numSteps, double* uSteps) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !c->have_consts || !u || !prm || !times || numTimes < 1 || numBcs < 0 ||
-      (numBcs > 0 && !bcs) || prm->precond < 0 || prm->precond > 2)
+      (numBcs > 0 && !bcs) || prm->precond < 0 || prm->precond > 3)
     return PDG_E_ARG;
   if (prm->adaptiveLoadStepping &&
       prm->maxTotalLoadStepReductions < prm->maxLoadStepReductionsInOneStep)
@@ -1473,6 +1823,7 @@ int pdg_qs_run_record(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* t
   int rc = qs::ensure(c);
   if (rc) return rc;
   QsState* S = c->qs;
+  S->coarse_ok = false;  // the two-level preconditioner is rebuilt for the run's DOFs
   const int N = c->N, n3 = 3 * N;
   hipStream_t s = c->stream;
   // kinematic DOFs (internal order) with their u(t) = c1 t + c0
@@ -1564,6 +1915,7 @@ int pdg_qs_run_record(pdg_ctx* h, const pdg_bc* bcs, int numBcs, const double* t
         // tangent at the current state: diagonal blocks (preconditioner and
         // damping), then the linear solve K lhs = r
         if ((rc = qs::block_jacobi(c))) return rc;
+        if (prec == 3 && !S->coarse_ok && (rc = qs::coarse_build(c))) return rc;
         if (damped && (rc = R.damp(prm->dampedNewtonDiagonalScaleFactor,
                                    prm->dampedNewtonDiagonalShiftFactor)))
           return rc;

@@ -16,7 +16,7 @@ import numpy as np
 from ._lib import pdg_qs_params, pdg_qs_step
 from .core import Context, _f64, _i32, _p
 
-PRECOND = {"none": 0, "block_jacobi": 1, "learned": 2}
+PRECOND = {"none": 0, "block_jacobi": 1, "learned": 2, "two_level": 3}
 
 
 def set_fixed(ctx: Context, dofs):
