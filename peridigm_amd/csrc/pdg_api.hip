@@ -117,7 +117,7 @@ int Ctx::hip(hipError_t e, const char* what) {
 void Ctx::free_device() {
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(d_row); F(d_col); F(d_mw); F(d_mask); F(d_nc); F(d_rec[0]); F(d_rec[1]); F(d_theta);
-  F(d_nbroken); F(d_v); F(d_a); F(d_f); F(d_rho); F(d_bcslot); F(d_bcs); F(d_err); F(d_near);
+  F(d_nbroken); F(d_v); F(d_a); F(d_f); F(d_inv_rho); F(d_bcslot); F(d_bcs); F(d_err); F(d_near);
   F(d_perm); F(d_fout); F(d_y); F(d_vol); F(d_x);
   F(d_tile_start); F(d_halo_off); F(d_halo); F(d_own_off); F(d_lcol);
   d_tile_start = d_halo_off = d_halo = d_own_off = nullptr;
@@ -125,7 +125,7 @@ void Ctx::free_device() {
   d_row = d_col = d_mw = d_nbroken = d_bcslot = d_err = d_perm = nullptr;
   d_mask = d_near = nullptr;
   d_nc = nullptr; d_rec[0] = d_rec[1] = nullptr; d_theta = nullptr;
-  d_v = d_a = d_f = d_rho = d_fout = d_y = d_vol = d_x = nullptr;
+  d_v = d_a = d_f = d_inv_rho = d_fout = d_y = d_vol = d_x = nullptr;
   d_bcs = nullptr;
 }
 
@@ -751,7 +751,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if ((rc = A((void**)&c->d_f, (size_t)N * 24, "alloc f"))) return rc;
   if ((rc = A((void**)&c->d_fout, (size_t)N * 24, "alloc fout"))) return rc;
   if ((rc = A((void**)&c->d_y, (size_t)N * 24, "alloc y"))) return rc;
-  if ((rc = A((void**)&c->d_rho, (size_t)N * 8, "alloc rho"))) return rc;
+  if ((rc = A((void**)&c->d_inv_rho, (size_t)N * 8, "alloc rho"))) return rc;
   if ((rc = A((void**)&c->d_bcslot, (size_t)N * 4, "alloc bcslot"))) return rc;
   if ((rc = A((void**)&c->d_bcs, sizeof(BcSlot), "alloc bcs"))) return rc;
   if ((rc = A((void**)&c->d_err, 4, "alloc err"))) return rc;
@@ -853,7 +853,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_theta, 0, 4 * (size_t)N, s), "zero theta");
   if (!rc) rc = c->hip(hipMemsetAsync(c->d_nbroken, 0, 4 * (size_t)N, s), "zero nbroken");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_bcslot, noslot.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d bcslot");
-  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_rho, rho1.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d rho");
+  if (!rc) rc = c->hip(hipMemcpyAsync(c->d_inv_rho, rho1.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d rho");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_perm, c->perm.data(), 4 * (size_t)N, hipMemcpyHostToDevice, s), "h2d perm");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_vol, vol_int.data(), 8 * (size_t)N, hipMemcpyHostToDevice, s), "h2d vol");
   if (!rc) rc = c->hip(hipMemcpyAsync(c->d_x, x_int.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s), "h2d x");
@@ -932,10 +932,11 @@ int pdg_set_density(pdg_ctx* h, const double* rho) {
   if (!c || !c->have_consts || (c->N > 0 && !rho)) return PDG_E_ARG;
   std::vector<double> r(c->N);
   for (int k = 0; k < c->N; k++) {
-    r[k] = rho[c->perm[k]];
-    if (!(r[k] > 0.0)) return c->fail(PDG_E_ARG, "density must be positive");
+    const double d = rho[c->perm[k]];
+    if (!(d > 0.0)) return c->fail(PDG_E_ARG, "density must be positive");
+    r[k] = 1.0 / d;
   }
-  int rc = c->hip(hipMemcpyAsync(c->d_rho, r.data(), 8 * (size_t)c->N, hipMemcpyHostToDevice, c->stream), "h2d rho");
+  int rc = c->hip(hipMemcpyAsync(c->d_inv_rho, r.data(), 8 * (size_t)c->N, hipMemcpyHostToDevice, c->stream), "h2d rho");
   if (rc) return rc;
   return c->hip(hipStreamSynchronize(c->stream), "sync");
 }
@@ -1106,7 +1107,7 @@ int pdg_verlet_run(pdg_ctx* h, int nsteps, double dt, double t0, int initialEval
   const bool damage = c->sc > 0.0;
   BondArgs ba = c->bond_args();
   VerletArgs va{};
-  va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
+  va.inv_rho = c->d_inv_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
   va.bcs = c->d_bcs; va.dt = dt; va.uq = c->q; va.inv_uq = 1.0 / c->q;
   va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");
@@ -1200,7 +1201,7 @@ int pdg_verlet_run_persistent(pdg_ctx* h, int nsteps, double dt, double t0, int 
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
   VerletArgs va{};
-  va.rho = c->d_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
+  va.inv_rho = c->d_inv_rho; va.v = c->d_v; va.a = c->d_a; va.f = c->d_f; va.bc_slot = c->d_bcslot;
   va.bcs = c->d_bcs; va.dt = dt; va.uq = c->q; va.inv_uq = 1.0 / c->q;
   va.err = c->d_err;
   int rc = c->hip(hipMemsetAsync(c->d_err, 0, 4, s), "zero err");

@@ -73,7 +73,7 @@ struct Ctx {
   double* d_f = nullptr;
   double* d_fout = nullptr;
   double* d_y = nullptr;
-  double* d_rho = nullptr;
+  double* d_inv_rho = nullptr;  // n: 1 / density (internal order)
   int* d_bcslot = nullptr;
   BcSlot* d_bcs = nullptr;
   int* d_err = nullptr;

@@ -175,14 +175,14 @@ __device__ __forceinline__ float pair_tv(const BondK& b, float p_i, float d_i, f
 
 // The global inputs of one Verlet component, loadable ahead of the force.
 struct VerletPre {
-  double rho, v;
+  double inv_rho, v;  // 1 / density, velocity
   double fadd;  // added force density (contact), 0 without
   int slot;     // boundary-condition slot (-1: none, or no new step starts)
 };
 
 __device__ __forceinline__ VerletPre verlet_pre(const VerletArgs& va, int node, int l) {
   VerletPre p;
-  p.rho = va.rho[(unsigned)node];
+  p.inv_rho = va.inv_rho[(unsigned)node];
   p.v = va.v[3u * (unsigned)node + (unsigned)l];
   const int slot = va.bc_slot[(unsigned)node];  // loaded unconditionally, selected after
   p.slot = va.start_step ? slot : -1;
@@ -206,7 +206,7 @@ __device__ __forceinline__ void verlet_epilogue(const VerletArgs& va, int node, 
   if (!isfinite(tot)) *va.err = 1;
   double acc = tot;
   acc += 0.0;  // external force (none)
-  acc /= pre.rho;
+  acc *= pre.inv_rho;  // a = f / rho, with 1 / rho formed once at set-up
   double v = pre.v;
   double u = u_scaled * va.uq;
   const double dt2 = va.dt / 2.0;

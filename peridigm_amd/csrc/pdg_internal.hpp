@@ -78,7 +78,7 @@ struct BondArgs {
 constexpr int kPairModelPmb = 3;
 
 struct VerletArgs {
-  const double* rho;  // n
+  const double* inv_rho;  // n: 1 / density
   double* v;          // 3n, FP64
   double* a;          // 3n, FP64 (written on the last step only)
   double* f;          // 3n, FP64 force density (written on the last step only)

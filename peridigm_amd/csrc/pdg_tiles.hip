@@ -667,7 +667,7 @@ __device__ __forceinline__ void tile_pass2(const BondArgs& a, const TileArgs& t,
     if (valid) {
       // the node's own alpha - p and p from its staged V (alpha - p), V p
       me = nb2_load(L, kLdsSlot * (own + (node - n0)));
-      const float rv = me.vol > 0.0f ? 1.0f / me.vol : 0.0f;
+      const float rv = me.vol > 0.0f ? __builtin_amdgcn_rcpf(me.vol) : 0.0f;
       d_i = me.vd * rv;
       p_i = me.vp * rv;
     }
