@@ -454,12 +454,24 @@ enum { SC_RZ = 0, SC_PQ = 1, SC_RR = 2, SC_RZN = 3, SC_ZQ = 4, SC_BETA = 5, SC_B
 // flexible CG: PR = p.r, then per history slot k: PQH + k = p_k.Kp_k, ZQH + k = z.Kp_k
 enum { SC_PR = 7, SC_PQH = 8, SC_ZQH = 12 };
 
-// x += al p, r -= al q, partial sums of r.r; al = RZ / PQ (0 after a breakdown)
-__global__ __launch_bounds__(RB) void k_cg_step(int n, const double* sc, const double* p,
-                                                const double* q, double* x, double* r,
-                                                double* part) {
+// x += al p, r -= al q, partial sums of r.r (into rr_part); al = RZ / PQ (0
+// after a breakdown), PQ summed from the partials of k_dot (pq_part)
+__global__ __launch_bounds__(RB) void k_cg_step(int n, int nb, double* sc, const double* pq_part,
+                                                const double* p, const double* q, double* x,
+                                                double* r, double* rr_part) {
+  // p.Kp from k_dot's partials, summed by every block in k_final's order (so
+  // the same value); block 0 stores it for the breakdown test
   __shared__ double sh[RB / 64];
-  const double pq = sc[SC_PQ];
+  __shared__ double pq_s;
+  double t = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) t += pq_part[i];
+  t = block_sum(t, sh);
+  if (threadIdx.x == 0) {
+    pq_s = t;
+    if (blockIdx.x == 0) sc[SC_PQ] = t;
+  }
+  __syncthreads();
+  const double pq = pq_s;
   const double al = pq > 0.0 ? sc[SC_RZ] / pq : 0.0;
   double s = 0.0;
   for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
@@ -469,7 +481,7 @@ __global__ __launch_bounds__(RB) void k_cg_step(int n, const double* sc, const d
     s += ri * ri;
   }
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) rr_part[blockIdx.x] = s;
 }
 
 // RZN = sum of the nb partials, then the CG beta step (as k_cg_beta with
@@ -1264,20 +1276,24 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   // comes round. With graphs on, each (preconditioner, sequence, slot) is
   // captured once (kept in the solver state until its vectors or the
   // network's buffers change) and replayed.
-  auto seq_a = [&](int cs) -> int {
+  // rr: also reduce r.r to its slot (needed only where the host checks it)
+  auto seq_a = [&](int cs, bool rr) -> int {
     double* pp = flexible ? S->ph[cs] : S->p;
     double* qq = flexible ? S->qh[cs] : S->q;
     int r2 = qs::tangent(c, pp, qq, true);
     if (r2) return r2;
+    double* rrp = S->part;  // r.r partials
     if (flexible) {
       ddot(pp, qq, SC_PQH + cs);
       ddot(pp, S->r, SC_PR);
       hipLaunchKernelGGL(k_fcg_step, red, blk, 0, s, n3, S->scal, cs, pp, qq, x, S->r, S->part);
     } else {
-      ddot(pp, qq, SC_PQ);
-      hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->scal, pp, qq, x, S->r, S->part);
+      hipLaunchKernelGGL(k_dot, red, blk, 0, s, n3, pp, qq, S->part);
+      rrp = S->part + S->nblk;
+      hipLaunchKernelGGL(k_cg_step, red, blk, 0, s, n3, S->nblk, S->scal, S->part, pp, qq, x, S->r,
+                         rrp);
     }
-    hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, S->part, 1, S->scal + SC_RR);
+    if (rr) hipLaunchKernelGGL(k_final, dim3(1), blk, 0, s, S->nblk, rrp, 1, S->scal + SC_RR);
     return c->hip(hipGetLastError(), "cg step");
   };
   auto seq_b = [&](int cs, int nh) -> int {
@@ -1336,9 +1352,10 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   };
   // runs sequence A or B for slot cs: replayed from a graph when the history
   // is full and graphs are on, otherwise launched directly
-  auto run_a = [&](int cs, bool full) -> int {
-    if (!graphs || !full) return seq_a(cs);
-    if (!ga[cs]) { int r2 = capture(&ga[cs], [&]() { return seq_a(cs); }); if (r2) return r2; }
+  auto run_a = [&](int cs, bool full, bool rr) -> int {
+    if (!graphs || !full) return seq_a(cs, rr);
+    // a captured sequence is replayed before every check: it reduces r.r
+    if (!ga[cs]) { int r2 = capture(&ga[cs], [&]() { return seq_a(cs, true); }); if (r2) return r2; }
     return c->hip(hipGraphLaunch(ga[cs], s), "graph launch");
   };
   auto run_b = [&](int cs, int nh, bool full) -> int {
@@ -1347,10 +1364,12 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
     return c->hip(hipGraphLaunch(gb[cs], s), "graph launch");
   };
   const int hist_full = flexible ? kFcgSlots - 1 : 0;
+  // the host checks the residual after iterations 1-8, every fourth, and the last
+  auto checked = [&](int k) { return k <= 8 || k % 4 == 0 || k == maxit; };
   while (relres > tol && it < maxit) {
-    if ((rc = run_a(cur, nhist >= hist_full))) return rc;
+    if ((rc = run_a(cur, nhist >= hist_full, checked(it + 1)))) return rc;
     it++;
-    if (it <= 8 || it % 4 == 0 || it == maxit) {
+    if (checked(it)) {
       if ((rc = check(&relres, &brk))) return rc;
       if (brk) return c->fail(PDG_E_NOCONV, "CG breakdown: p.Kp <= 0");
       if (relres <= tol) break;
