@@ -472,7 +472,10 @@ __global__ __launch_bounds__(RB) void k_cg_step(int n, int nb, double* sc, const
   }
   __syncthreads();
   const double pq = pq_s;
-  const double al = pq > 0.0 ? sc[SC_RZ] / pq : 0.0;
+  // r.z of this iteration is in RZN (from k_cg_dir or the set-up); block 0
+  // moves it to RZ for the next direction's beta (no block reads RZ here)
+  const double al = pq > 0.0 ? sc[SC_RZN] / pq : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_RZ] = sc[SC_RZN];
   double s = 0.0;
   for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
     x[i] += al * p[i];
@@ -484,32 +487,6 @@ __global__ __launch_bounds__(RB) void k_cg_step(int n, int nb, double* sc, const
   if (threadIdx.x == 0) rr_part[blockIdx.x] = s;
 }
 
-// RZN = sum of the nb partials, then the CG beta step (as k_cg_beta with
-// flexible = 0) in the same launch
-__global__ __launch_bounds__(RB) void k_final_beta(int nb, const double* part, double* sc) {
-  __shared__ double sh[RB / 64];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
-  s = block_sum(s, sh);
-  if (threadIdx.x != 0) return;
-  sc[SC_RZN] = s;
-  const double pq = sc[SC_PQ];
-  if (!(pq > 0.0)) sc[SC_BRK] = 1.0;
-  sc[SC_BETA] = s / sc[SC_RZ];
-  sc[SC_RZ] = s;
-}
-
-// beta = RZN / RZ (CG) or -ZQ / PQ (flexible CG, one-term truncation); RZ = RZN
-__global__ void k_cg_beta(double* sc, int flexible) {
-  if (threadIdx.x != 0) return;
-  const double pq = sc[SC_PQ];
-  if (!(pq > 0.0)) sc[SC_BRK] = 1.0;
-  sc[SC_BETA] = flexible ? -sc[SC_ZQ] / pq : sc[SC_RZN] / sc[SC_RZ];
-  sc[SC_RZ] = sc[SC_RZN];
-}
-
-// Flexible CG step on history slot `slot`: al = PR / PQH[slot] (0 after a
-// breakdown), x += al p, r -= al q, partial sums of r.r.
 __global__ __launch_bounds__(RB) void k_fcg_step(int n, double* sc, int slot, const double* p,
                                                  const double* q, double* x, double* r,
                                                  double* part) {
@@ -549,9 +526,30 @@ __global__ __launch_bounds__(RB) void k_fcg_dir(int n, const double* sc, const d
 }
 
 // p = z + beta p
-__global__ __launch_bounds__(RB) void k_cg_dir(int n, const double* sc, const double* z, double* p) {
-  const int i = blockIdx.x * RB + threadIdx.x;
-  if (i < n) p[i] = z[i] + sc[SC_BETA] * p[i];
+// p = z + beta p with beta = RZN / RZ (grid-stride, launched with as many
+// blocks as there are partials): every block sums the r.z partials
+// (rz_part) in the order of a single-block reduction; block 0 stores the sum
+// in RZN (read by the next step; no block reads RZN here), BETA, and the
+// breakdown flag
+__global__ __launch_bounds__(RB) void k_cg_dir(int n, int nb, double* sc, const double* rz_part,
+                                               const double* z, double* p) {
+  __shared__ double sh[RB / 64];
+  __shared__ double beta_s;
+  double t = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) t += rz_part[i];
+  t = block_sum(t, sh);
+  if (threadIdx.x == 0) {
+    const double beta = t / sc[SC_RZ];
+    beta_s = beta;
+    if (blockIdx.x == 0) {
+      sc[SC_RZN] = t;
+      sc[SC_BETA] = beta;
+      if (!(sc[SC_PQ] > 0.0)) sc[SC_BRK] = 1.0;
+    }
+  }
+  __syncthreads();
+  const double beta = beta_s;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) p[i] = z[i] + beta * p[i];
 }
 
 __global__ __launch_bounds__(RB) void k_axpby(int n, double a, const double* x, double b, double* y) {
@@ -1252,7 +1250,7 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
   if (!rc) rc = precond_apply(S->r, S->z);
   if (!rc) rc = c->hip(hipMemcpyAsync(p, S->z, 8 * (size_t)n3, hipMemcpyDeviceToDevice, s), "copy");
   if (rc) return rc;
-  ddot(S->r, S->z, SC_RZ);
+  ddot(S->r, S->z, SC_RZN);  // r.z of the first iteration (read by k_cg_step)
   ddot(S->r, S->r, SC_RR);
   auto check = [&](double* relres, bool* brk) -> int {
     int r2 = c->hip(hipMemcpyAsync(S->hscal, S->scal, 8 * kScal, hipMemcpyDeviceToHost, s), "d2h");
@@ -1324,8 +1322,7 @@ int qs_solve(Ctx* c, double* x, const double* b, int precond, double tol, int ma
       hipLaunchKernelGGL(k_fcg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, h,
                          S->ph[(cs + 1) % kFcgSlots]);
     } else {
-      hipLaunchKernelGGL(k_final_beta, dim3(1), blk, 0, s, S->nblk, S->part, S->scal);
-      hipLaunchKernelGGL(k_cg_dir, dim3(vb), blk, 0, s, n3, S->scal, S->z, S->p);
+      hipLaunchKernelGGL(k_cg_dir, red, blk, 0, s, n3, S->nblk, S->scal, S->part, S->z, S->p);
     }
     return c->hip(hipGetLastError(), "cg direction");
   };
