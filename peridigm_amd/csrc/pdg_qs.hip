@@ -98,7 +98,10 @@ namespace {
 
 using namespace dev;
 
-constexpr int QG = 8;   // lanes per node
+// lanes per node of the tangent, residual and diagonal kernels: 32 on
+// meshes below kQsWideNodes nodes (more waves to hide the gathers' latency),
+// 16 above (measured on 24,000- and 200,000-node bars)
+constexpr int kQsWideNodes = 100000;
 constexpr int kScal = 32;   // solver scalar slots on the device
 constexpr int kFcgSlots = 4;  // flexible CG keeps kFcgSlots - 1 previous directions
 constexpr int RB = 256; // reduction block
@@ -150,7 +153,7 @@ struct QsArgs {
 };
 
 // dilatation (+ its directional derivative, + d theta_i / d u_i)
-template <bool DUAL, bool DIAG>
+template <int QG, bool DUAL, bool DIAG>
 __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
 // MODE 2: f V on every DOF (the reactions are its kinematic entries);
 // MODE 3: the partial stress sum_j f_ij (x_j - x_i)^T V_j of the node's own
 // bonds (src/materials/elastic.cxx:132-142), 9 per node into a.ps
-template <int MODE>
+template <int QG, int MODE>
 __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
@@ -305,6 +308,7 @@ __global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
 }
 
 // Inverse of the 3x3 diagonal block of K at each node.
+template <int QG>
 __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
@@ -801,7 +805,24 @@ __global__ __launch_bounds__(RB) void k_zero_fixed(int n, const signed char* fix
   if (i < n && fixed[i]) v[i] = 0.0;
 }
 
+template <int QG>
 inline unsigned qblocks(int n) { return (unsigned)((n + 4 * (64 / QG) - 1) / (4 * (64 / QG))); }
+
+// launches with the lane count chosen for the mesh size
+template <bool DUAL, bool DIAG>
+void qs_theta(const QsArgs& a, hipStream_t s) {
+  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_theta<32, DUAL, DIAG>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qs_theta<16, DUAL, DIAG>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+}
+template <int MODE>
+void qs_force(const QsArgs& a, hipStream_t s) {
+  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_force<32, MODE>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qs_force<16, MODE>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+}
+inline void qs_diag(const QsArgs& a, hipStream_t s) {
+  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_diag<32>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qs_diag<16>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+}
 
 }  // namespace
 
@@ -919,8 +940,8 @@ static int to_host(Ctx* c, const double* d, double* h) { return c->download_vec3
 static int residual(Ctx* c) {
   QsArgs a = args(c);
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_qs_force<0>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  qs_theta<false, false>(a, s);
+  qs_force<0>(a, s);
   return c->hip(hipGetLastError(), "qs residual");
 }
 
@@ -931,8 +952,8 @@ static int tangent(Ctx* c, const double* w, double* out, bool theta_ready) {
   a.out = out;
   hipStream_t s = c->stream;
   (void)theta_ready;  // the dual pass recomputes theta with its derivative
-  hipLaunchKernelGGL((k_qs_theta<true, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_qs_force<1>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  qs_theta<true, false>(a, s);
+  qs_force<1>(a, s);
   if (c->qs->damped) {
     const int n3 = 3 * c->N;
     hipLaunchKernelGGL(k_damp_add, dim3((n3 + RB - 1) / RB), dim3(RB), 0, s, n3, c->qs->dampv, w, out);
@@ -943,8 +964,8 @@ static int tangent(Ctx* c, const double* w, double* out, bool theta_ready) {
 static int block_jacobi(Ctx* c) {
   QsArgs a = args(c);
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL((k_qs_theta<false, true>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_qs_diag, dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  qs_theta<false, true>(a, s);
+  qs_diag(a, s);
   return c->hip(hipGetLastError(), "qs block jacobi");
 }
 
@@ -1120,8 +1141,8 @@ static int residual_norm(Ctx* c, double* norm) {
 static int reaction_norm(Ctx* c, double* norm) {
   QsArgs a = args(c);
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_qs_force<2>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  qs_theta<false, false>(a, s);
+  qs_force<2>(a, s);
   double ninf = 0.0;
   int rc = c->hip(hipGetLastError(), "qs reactions");
   if (!rc) rc = norms(c, c->qs->out, c->qs->fixed, norm, &ninf);
@@ -1585,8 +1606,8 @@ int pdg_qs_force_dilatation(pdg_ctx* h, const double* u, double* force, double* 
   if (rc) return rc;
   QsArgs a = qs::args(c);
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_qs_force<2>), dim3(qblocks(c->N)), dim3(256), 0, s, a);
+  qs_theta<false, false>(a, s);
+  qs_force<2>(a, s);
   rc = c->hip(hipGetLastError(), "qs force");
   if (!rc) rc = qs::to_host(c, S->out, force);
   if (rc) return rc;
@@ -1657,8 +1678,8 @@ int pdg_compute_partial_stress(pdg_ctx* h, const double* y, const double* bondDa
   if ((rc = qs::to_dev(c, u.data(), S->u))) return rc;
   QsArgs a = qs::args(c);
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL((k_qs_theta<false, false>), dim3(qblocks(N)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_qs_force<3>), dim3(qblocks(N)), dim3(256), 0, s, a);
+  qs_theta<false, false>(a, s);
+  qs_force<3>(a, s);
   rc = c->hip(hipGetLastError(), "partial stress");
   std::vector<double> hp(9 * (size_t)N);
   if (!rc) rc = c->hip(hipMemcpyAsync(hp.data(), S->ps, 72 * (size_t)N, hipMemcpyDeviceToHost, s), "d2h ps");
