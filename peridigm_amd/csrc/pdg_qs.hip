@@ -217,8 +217,10 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
 // MODE 2: f V on every DOF (the reactions are its kinematic entries);
 // MODE 3: the partial stress sum_j f_ij (x_j - x_i)^T V_j of the node's own
 // bonds (src/materials/elastic.cxx:132-142), 9 per node into a.ps
+// at least 5 waves per SIMD (<= 96 VGPRs; the compiler's own choice, 100,
+// allowed 4), with no spills
 template <int QG, int MODE>
-__global__ __launch_bounds__(256) void k_qs_force(QsArgs a) {
+__global__ __launch_bounds__(256, 5) void k_qs_force(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
   const int node = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / QG) + gi;
