@@ -153,7 +153,7 @@ struct QsArgs {
 };
 
 // dilatation (+ its directional derivative, + d theta_i / d u_i)
-template <int QG, bool DUAL, bool DIAG>
+template <int QG, bool DUAL, bool DIAG, bool GEN>
 __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
       const int j = a.col[rs + k];
       const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
       const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
-      if (a.linear) {
+      if (GEN && a.linear) {
         // theta_i = 3/m_i sum w V_j xi.(u_j - u_i)
         // (src/materials/Peridigm_LinearLPSPVMaterial.cpp:229-289)
         const double c = infl64(a.influence, zeta, a.horizon) * a.vol[j];
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
       const double iy = 1.0 / yl;
       double e = yl - zeta;
-      if (a.dT) e -= a.alphaT * a.dT[node] * zeta;  // elastic.cxx thermal strain
+      if (GEN && a.dT) e -= a.alphaT * a.dT[node] * zeta;  // elastic.cxx thermal strain
       const double c = infl64(a.influence, zeta, a.horizon) * zeta * a.vol[j];
       th += c * e;
       const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_qs_theta(QsArgs a) {
 // bonds (src/materials/elastic.cxx:132-142), 9 per node into a.ps
 // at least 5 waves per SIMD (<= 96 VGPRs; the compiler's own choice, 100,
 // allowed 4), with no spills
-template <int QG, int MODE>
+template <int QG, int MODE, bool GEN>
 __global__ __launch_bounds__(256, 5) void k_qs_force(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256, 5) void k_qs_force(QsArgs a) {
       const int j = a.col[rs + k];
       const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
       const double vj = a.vol[j];
-      if (a.linear) {
+      if (GEN && a.linear) {
         // both ends of bond i-j: w V_j [(theta_i b_i + theta_j b_j)
         //   + (a_i + a_j) xi.(u_j - u_i) / |xi|^2] xi
         // (src/materials/Peridigm_LinearLPSPVMaterial.cpp:296-394)
@@ -267,8 +267,8 @@ __global__ __launch_bounds__(256, 5) void k_qs_force(QsArgs a) {
       const double iy = 1.0 / yl;
       const double e = yl - zeta;
       // thermal strain of each end's bond: e_ij = e - alphaT dT_i zeta
-      const double ei = a.dT ? e - a.alphaT * a.dT[node] * zeta : e;
-      const double ej = a.dT ? e - a.alphaT * a.dT[j] * zeta : e;
+      const double ei = (GEN && a.dT) ? e - a.alphaT * a.dT[node] * zeta : e;
+      const double ej = (GEN && a.dT) ? e - a.alphaT * a.dT[j] * zeta : e;
       const double om = infl64(a.influence, zeta, a.horizon);
       const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + ai * ei + a.alpha[j] * ej);
       const double n0 = Y0 * iy, n1 = Y1 * iy, n2 = Y2 * iy;
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256, 5) void k_qs_force(QsArgs a) {
 }
 
 // Inverse of the 3x3 diagonal block of K at each node.
-template <int QG>
+template <int QG, bool GEN>
 __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
   const int lane = threadIdx.x & 63;
   const int gi = lane / QG, l = lane & (QG - 1);
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
       const double X0 = a.x[3 * j] - xi[0], X1 = a.x[3 * j + 1] - xi[1], X2 = a.x[3 * j + 2] - xi[2];
       const double vj = a.vol[j];
       const double cj = a.m[j] > 0.0 ? 3.0 / a.m[j] : 0.0;
-      if (a.linear) {
+      if (GEN && a.linear) {
         // d/du_i of s xi with s = w (theta_i b_i + theta_j b_j) + T2 xi.(u_j - u_i):
         // d theta_i/du_i = g_i, d theta_j/du_i = -(3/m_j) w V_i xi
         const double X[3] = {X0, X1, X2};
@@ -350,8 +350,8 @@ __global__ __launch_bounds__(256) void k_qs_diag(QsArgs a) {
       const double zeta = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
       const double yl = sqrt(Y0 * Y0 + Y1 * Y1 + Y2 * Y2);
       const double e = yl - zeta;
-      const double ei = a.dT ? e - a.alphaT * a.dT[node] * zeta : e;
-      const double ej = a.dT ? e - a.alphaT * a.dT[j] * zeta : e;
+      const double ei = (GEN && a.dT) ? e - a.alphaT * a.dT[node] * zeta : e;
+      const double ej = (GEN && a.dT) ? e - a.alphaT * a.dT[j] * zeta : e;
       const double om = infl64(a.influence, zeta, a.horizon);
       const double T = om * ((thi * bi + a.theta[j] * a.beta[j]) * zeta + ai * ei + a.alpha[j] * ej);
       const double n[3] = {Y0 / yl, Y1 / yl, Y2 / yl};
@@ -810,20 +810,37 @@ __global__ __launch_bounds__(RB) void k_zero_fixed(int n, const signed char* fix
 template <int QG>
 inline unsigned qblocks(int n) { return (unsigned)((n + 4 * (64 / QG) - 1) / (4 * (64 / QG))); }
 
-// launches with the lane count chosen for the mesh size
+// launches with the lane count chosen for the mesh size, and the general
+// kernels (GEN: the linearized model or thermal strains) only when needed, so
+// the plain elastic loops carry no branch for them
+template <bool DUAL, bool DIAG, bool GEN>
+void qs_theta_g(const QsArgs& a, hipStream_t s) {
+  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_theta<32, DUAL, DIAG, GEN>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qs_theta<16, DUAL, DIAG, GEN>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+}
 template <bool DUAL, bool DIAG>
 void qs_theta(const QsArgs& a, hipStream_t s) {
-  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_theta<32, DUAL, DIAG>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_qs_theta<16, DUAL, DIAG>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+  if (a.linear || a.dT) qs_theta_g<DUAL, DIAG, true>(a, s);
+  else qs_theta_g<DUAL, DIAG, false>(a, s);
+}
+template <int MODE, bool GEN>
+void qs_force_g(const QsArgs& a, hipStream_t s) {
+  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_force<32, MODE, GEN>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qs_force<16, MODE, GEN>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
 }
 template <int MODE>
 void qs_force(const QsArgs& a, hipStream_t s) {
-  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_force<32, MODE>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_qs_force<16, MODE>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+  if (a.linear || a.dT) qs_force_g<MODE, true>(a, s);
+  else qs_force_g<MODE, false>(a, s);
+}
+template <bool GEN>
+void qs_diag_g(const QsArgs& a, hipStream_t s) {
+  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_diag<32, GEN>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qs_diag<16, GEN>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
 }
 inline void qs_diag(const QsArgs& a, hipStream_t s) {
-  if (a.n < kQsWideNodes) hipLaunchKernelGGL((k_qs_diag<32>), dim3(qblocks<32>(a.n)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_qs_diag<16>), dim3(qblocks<16>(a.n)), dim3(256), 0, s, a);
+  if (a.linear || a.dT) qs_diag_g<true>(a, s);
+  else qs_diag_g<false>(a, s);
 }
 
 }  // namespace
