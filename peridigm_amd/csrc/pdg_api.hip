@@ -718,6 +718,7 @@ int pdg_set_constants(pdg_ctx* h, const double* x, const double* vol, double K, 
       }
     });
     const double maxlen = *std::max_element(mx.begin(), mx.end()) / c->L;
+    c->bond_cmax = maxlen * c->L;
     int e = 0;
     if (maxlen > 0.0) (void)std::frexp(maxlen, &e);  // maxlen < 2^e
     c->qexp = 27 - e;

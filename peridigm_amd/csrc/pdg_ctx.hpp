@@ -54,6 +54,7 @@ struct Ctx {
   // position quantum of the bond kernels: q = L * 2^-qexp (records hold
   // u / q and the fixed-point x / q)
   double q = 1;
+  double bond_cmax = 0.0;  // largest bond component |x_j - x_i| (any axis), from set-up
   int qexp = 0;
   int infl = 0;
   std::vector<double> x, vol, m;  // reference order

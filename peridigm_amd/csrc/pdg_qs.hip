@@ -740,15 +740,24 @@ __global__ __launch_bounds__(RB) void k_coarse_finish(int nc, const double* __re
   }
 }
 
-// The coarse function `col` as a full vector
-__global__ __launch_bounds__(RB) void k_coarse_basis(int n, int col, CoarseGrid G,
-                                                     const double* __restrict__ x,
+// Probe vector of the coarse set-up: the sum of the coarse functions (g, k0)
+// over the grid points g of one colour (gx, gy, gz congruent to cx, cy, cz
+// modulo `stride`)
+__global__ __launch_bounds__(RB) void k_coarse_probe(int n, CoarseGrid G, int stride, int cx, int cy,
+                                                     int cz, int k0, const double* __restrict__ x,
                                                      const signed char* __restrict__ fixed,
                                                      double* __restrict__ v) {
   const int i = blockIdx.x * RB + threadIdx.x;
   if (i >= n) return;
-  const int pt = col / 3, k0 = col - 3 * pt;
-  const double w = point_w(G, x + 3 * (size_t)i, pt);
+  int c[3];
+  double t[3];
+  coarse_cell(G, x + 3 * (size_t)i, c, t);
+  double w = 0.0;
+  for (int q = 0; q < 8; q++) {
+    const bool in = (c[0] + (q & 1)) % stride == cx && (c[1] + ((q >> 1) & 1)) % stride == cy &&
+                    (c[2] + ((q >> 2) & 1)) % stride == cz;
+    w += in ? corner_w(q, t) : 0.0;
+  }
   for (int k = 0; k < 3; k++) v[3 * i + k] = (k == k0 && !fixed[3 * i + k]) ? w : 0.0;
 }
 
@@ -1061,20 +1070,60 @@ static int coarse_build(Ctx* c) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_coarse_weights, dim3(np), dim3(RB), 0, s, G, S->cstart, S->clist, S->x,
                      S->wlist);
-  // column col of A_c: P^T K phi_col (S->z holds phi_col, S->q its image)
-  for (int col = 0; col < nc; col++) {
-    hipLaunchKernelGGL(k_coarse_basis, dim3(nb), dim3(RB), 0, s, N, col, G, S->x, S->fixed, S->z);
-    if ((rc = tangent(c, S->z, S->q, true))) return rc;
-    hipLaunchKernelGGL(k_coarse_partial, dim3(np * kCoarseSplit), dim3(RB), 0, s, S->cstart,
-                       S->clist, S->wlist, S->fixed, S->q, S->cpart);
-    hipLaunchKernelGGL(k_coarse_finish<false>, dim3(1), dim3(RB), 0, s, nc, S->cpart, nullptr,
-                       S->cmat + (size_t)col * nc);
-  }
-  std::vector<double> M((size_t)nc * nc);
+  // A_c by probing: the tangent couples nodes at most 2 bond lengths apart
+  // (the dilatation term reaches the neighbours' neighbours), so phi_g' . K
+  // phi_g vanishes unless g and g' are within `reach` grid steps on every
+  // axis (supports H either side of a point, bond components at most
+  // bond_cmax). Points of one colour (indices congruent modulo 2 reach + 1 on
+  // every axis) then never meet the same target, and one tangent apply of
+  // their summed functions gives all their columns at once. S->z holds the
+  // probe, S->q its image; probe p's restriction goes to column p of S->cmat.
+  const double bc = c->bond_cmax;
+  const int reach = std::max(1, (int)std::ceil((2.0 * G.H + 2.0 * bc) / G.H - 1e-12) - 1);
+  const int stride = 2 * reach + 1;
+  const int sc[3] = {std::min(stride, G.d[0]), std::min(stride, G.d[1]), std::min(stride, G.d[2])};
+  std::vector<int> pcol;  // probe -> colour code (cx, cy, cz, k0)
+  for (int cz = 0; cz < sc[2]; cz++)
+    for (int cy = 0; cy < sc[1]; cy++)
+      for (int cx = 0; cx < sc[0]; cx++)
+        for (int k0 = 0; k0 < 3; k0++) {
+          const int pr = (int)pcol.size();
+          pcol.push_back(((cz * sc[1] + cy) * sc[0] + cx) * 3 + k0);
+          hipLaunchKernelGGL(k_coarse_probe, dim3(nb), dim3(RB), 0, s, N, G, stride, cx, cy, cz, k0,
+                             S->x, S->fixed, S->z);
+          if ((rc = tangent(c, S->z, S->q, true))) return rc;
+          hipLaunchKernelGGL(k_coarse_partial, dim3(np * kCoarseSplit), dim3(RB), 0, s, S->cstart,
+                             S->clist, S->wlist, S->fixed, S->q, S->cpart);
+          hipLaunchKernelGGL(k_coarse_finish<false>, dim3(1), dim3(RB), 0, s, nc, S->cpart, nullptr,
+                             S->cmat + (size_t)pr * nc);
+        }
+  const int nprobe = (int)pcol.size();  // <= nc, so S->cmat holds every probe
+  std::vector<double> R((size_t)nprobe * nc), M((size_t)nc * nc, 0.0);
   rc = c->hip(hipGetLastError(), "qs coarse build");
-  if (!rc) rc = c->hip(hipMemcpyAsync(M.data(), S->cmat, 8 * M.size(), hipMemcpyDeviceToHost, s), "d2h");
+  if (!rc) rc = c->hip(hipMemcpyAsync(R.data(), S->cmat, 8 * R.size(), hipMemcpyDeviceToHost, s), "d2h");
   if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
   if (rc) return rc;
+  // entry (target t, source g): the source of the target's colour within
+  // reach on every axis, if any (M[col * nc + row] = A_c[row][col])
+  for (int pr = 0; pr < nprobe; pr++) {
+    const int k0 = pcol[pr] % 3, code = pcol[pr] / 3;
+    const int cx = code % sc[0], cy = (code / sc[0]) % sc[1], cz = code / (sc[0] * sc[1]);
+    const int cc[3] = {cx, cy, cz};
+    for (int t = 0; t < np; t++) {
+      const int tp[3] = {t % G.d[0], (t / G.d[0]) % G.d[1], t / (G.d[0] * G.d[1])};
+      int gp[3];
+      bool ok = true;
+      for (int d = 0; d < 3 && ok; d++) {
+        int delta = ((cc[d] - tp[d]) % stride + stride) % stride;
+        if (delta > reach) delta -= stride;
+        gp[d] = tp[d] + delta;
+        ok = gp[d] >= 0 && gp[d] < G.d[d] && gp[d] % stride == cc[d];
+      }
+      if (!ok) continue;
+      const int g = gp[0] + G.d[0] * (gp[1] + G.d[1] * gp[2]);
+      for (int k = 0; k < 3; k++) M[(size_t)(3 * g + k0) * nc + 3 * t + k] = R[(size_t)pr * nc + 3 * t + k];
+    }
+  }
   // symmetrise, factor L L^T with dropped functions, invert
   for (int i = 0; i < nc; i++)
     for (int j = 0; j < i; j++) {
