@@ -64,30 +64,6 @@ uint64_t hilbert3(uint64_t a, uint64_t b, uint64_t c) {
   return spread3(X[2]) | (spread3(X[1]) << 1) | (spread3(X[0]) << 2);
 }
 
-// Runs f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads
-// (one chunk in the calling thread when n < min_items).
-template <class F>
-void parallel_for(long long n, F&& f, long long min_items = 65536) {
-  const unsigned hw = std::thread::hardware_concurrency();
-  const unsigned nt = std::min(16u, std::max(1u, hw));
-  if (n < min_items || nt == 1) { f(0LL, n); return; }
-  const long long chunk = (n + nt - 1) / nt;
-  std::vector<std::thread> th;
-  th.reserve(nt);
-  long long done = 0;  // chunks below `done` are handed to threads
-  for (unsigned t = 0; t < nt; t++) {
-    const long long lo = (long long)t * chunk, hi = std::min(n, lo + chunk);
-    if (lo >= hi) break;
-    try {
-      th.emplace_back([&f, lo, hi] { f(lo, hi); });
-    } catch (...) {  // no thread available: the caller runs the rest
-      break;
-    }
-    done = hi;
-  }
-  if (done < n) f(done, n);
-  for (auto& x : th) x.join();
-}
 
 uint64_t spread3(uint64_t v) {
   v &= 0x1fffff;

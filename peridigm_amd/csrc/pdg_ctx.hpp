@@ -3,12 +3,38 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pdg_internal.hpp"
 #include "pdg_sched.hpp"
 
 namespace pdg {
+
+// Runs f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads
+// (one chunk in the calling thread when n < min_items).
+template <class F>
+inline void parallel_for(long long n, F&& f, long long min_items = 65536) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nt = std::min(16u, std::max(1u, hw));
+  if (n < min_items || nt == 1) { f(0LL, n); return; }
+  const long long chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  long long done = 0;  // chunks below `done` are handed to threads
+  for (unsigned t = 0; t < nt; t++) {
+    const long long lo = (long long)t * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    try {
+      th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    } catch (...) {  // no thread available: the caller runs the rest
+      break;
+    }
+    done = hi;
+  }
+  if (done < n) f(done, n);
+  for (auto& x : th) x.join();
+}
 
 struct QsState;   // quasi-static solver workspace (pdg_qs.hip)
 struct GnnState;  // learned preconditioner (pdg_gnn.hip)

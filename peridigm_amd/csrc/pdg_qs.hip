@@ -637,7 +637,10 @@ __global__ __launch_bounds__(RB) void k_lin2(int n, double a, const double* x, d
 // Two-level preconditioner kernels. Coarse function 3g + k is the trilinear
 // hat function of grid point g times the unit vector along axis k; it is
 // zero on prescribed DOFs.
-constexpr int kCoarsePointsMax = 64;  // grid points at most (192 coarse functions)
+constexpr int kCoarsePointsMax = 160;  // grid points at most (480 coarse functions)
+// grid points allowed for n nodes: 64, rising to kCoarsePointsMax at 80,000
+// nodes (the probed set-up costs at most 3 x 125 tangent applies)
+inline int coarse_points_cap(int n) { return std::max(64, std::min(kCoarsePointsMax, n / 500)); }
 
 // Cell of a position (lower corner, clamped so the cell lies inside the
 // grid) and the position's local coordinates in it.
@@ -998,7 +1001,7 @@ static int block_jacobi(Ctx* c) {
 }
 
 // Two-level preconditioner set-up at the current state: the coarse grid
-// (at most kCoarsePointsMax points, spacing equal on all axes, centred on
+// (at most coarse_points_cap(n) points, spacing equal on all axes, centred on
 // the bounding box), the node lists of the restriction, and the inverse of
 // the coarse matrix A_c = P^T K P, column by column from one tangent apply
 // per coarse function. A_c is inverted on the host by Cholesky
@@ -1039,7 +1042,7 @@ static int coarse_build(Ctx* c) {
     CoarseGrid T;
     T.H = H;
     for (int d = 0; d < 3; d++) T.d[d] = std::max(2, (int)std::ceil(E[d] / H - 1e-9) + 1);
-    if (T.points() > kCoarsePointsMax) break;
+    if (T.points() > coarse_points_cap(N)) break;
     for (int d = 0; d < 3; d++) T.o[d] = lo[d] - 0.5 * ((T.d[d] - 1) * H - E[d]);
     G = T;
   }
@@ -1146,8 +1149,10 @@ static int coarse_build(Ctx* c) {
       L[(size_t)i * nc + j] = v / ljj;
     }
   }
-  std::vector<double> inv((size_t)nc * nc, 0.0), y(nc);
-  for (int e = 0; e < nc; e++) {
+  std::vector<double> inv((size_t)nc * nc, 0.0);
+  parallel_for(nc, [&](long long e0, long long e1) {  // columns on host threads
+  std::vector<double> y(nc);
+  for (int e = (int)e0; e < (int)e1; e++) {
     if (drop[e]) continue;
     for (int i = 0; i < nc; i++) {  // L y = e_e
       if (drop[i]) { y[i] = 0.0; continue; }
@@ -1163,6 +1168,7 @@ static int coarse_build(Ctx* c) {
     }
     for (int i = 0; i < nc; i++) inv[(size_t)i * nc + e] = y[i];
   }
+  }, 64);
   rc = c->hip(hipMemcpyAsync(S->cinv, inv.data(), 8 * inv.size(), hipMemcpyHostToDevice, s), "h2d");
   if (!rc) rc = c->hip(hipStreamSynchronize(s), "sync");
   S->coarse_ok = rc == 0;
